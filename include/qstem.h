@@ -1,0 +1,122 @@
+/*
+ * qstem.h -- C ABI of the MI355X multislice library (libqstem_hip.so).
+ *
+ * Drop-in boundary for QSTEM's STEM hot path.  The reference exposes it as
+ * C++ free functions over a process-global MULS and per-thread WAVEFUNCs
+ * (stem3/stemlib.h:36-62); this ABI replaces them with an opaque context
+ * that owns the device buffers and one HIP stream.  Plain pointers and sizes
+ * only; every call returns a status code instead of the reference's
+ * printf+exit(0).
+ *
+ * Reference interface each entry point replaces:
+ *   qs_create          readFile() MULS sizing + WAVEFUNC ctor      stem3/stem3.cpp:612-1559, libs/data_containers.cpp:24-58
+ *   qs_set_potential   initSTEMSlices(MULS*, int nlayer)           stem3/stemlib.cpp:2199-2357
+ *   qs_set_trans       direct write of muls.trans                  libs/data_containers.h:112-116
+ *   qs_get_trans       read back of muls.trans                     stem3/stemlib.cpp:1280-1306 (save potential)
+ *   qs_make_probe      probe(MULS*, WavePtr, double dx, double dy) stem3/stemlib.cpp:1952-2191
+ *   qs_run_stem        doSTEM scan loop body: runMulsSTEM +        stem3/stem3.cpp:2820-2946,
+ *                      transmit/propagate_slow/collectIntensity/   stem3/stemlib.cpp:2375-2534, 2576-2687,
+ *                      fft_normalize for a batch of positions     2760-2906
+ *   qs_build_potential make3DSlices + getAtomPotential3D            stem3/stemlib.cpp:383-1541
+ *   qs_record_bins     tCount/interval bookkeeping of collectIntensity stem3/stemlib.cpp:2588-2607
+ *   qs_last_error      (none: the reference prints and exits)
+ *
+ * Threading: one context = one stream; calls on one context must not overlap.
+ * Distinct contexts may be used from distinct threads.
+ */
+#ifndef QSTEM_H
+#define QSTEM_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define QS_OK 0
+#define QS_ERR_ARG 1          /* invalid argument or inconsistent sizes */
+#define QS_ERR_UNSUPPORTED 2  /* size or option this build does not handle */
+#define QS_ERR_HIP 3          /* HIP runtime failure (message in qs_last_error) */
+#define QS_ERR_STATE 4        /* call order violated (e.g. run before potential) */
+
+#define QS_MAX_DET 16
+
+/* Bit flags for qs_params.compat: reproduce reference quirks. */
+#define QS_COMPAT_Q1 1 /* initSTEMSlices uses muls->by for negative ky (stemlib.cpp:2237) */
+#define QS_COMPAT_Q2 2 /* probe(): "(a62 = 0)" assignment in the C6 condition (stemlib.cpp:2082) */
+
+/* The MULS fields the STEM path reads (libs/data_containers.h:103-308).
+ * Float where MULS stores float_tt, so the same rounded values are used. */
+typedef struct qs_params {
+  int nx, ny;                 /* wave grid; powers of two, 64..4096 */
+  float resolutionX, resolutionY;
+  float v0;                   /* kV */
+  int slices;                 /* slices per slab */
+  float sliceThickness;       /* Angstrom (cz[0]) */
+  int potNx, potNy;           /* transmission array, >= nx, ny */
+  float potSizeX, potSizeY;   /* potNx*resolutionX, potNy*resolutionY */
+  float by;                   /* model extent in y (only used with QS_COMPAT_Q1) */
+  int bandlimittrans;         /* "bandlimit f_trans:" */
+  float Cs, C5, df0, astigMag, astigAngle; /* Angstrom / rad, as stored in MULS */
+  float a33, a31, a44, a42, a55, a53, a51, a66, a64, a62;
+  float phi33, phi31, phi44, phi42, phi55, phi53, phi51, phi66, phi64, phi62;
+  float alpha;                /* mrad */
+  float aAIS;                 /* Angstrom, 0 = off */
+  float Cc, dE_E;
+  int ismoth, gaussFlag;
+  float gaussScale;
+  int ndet;                   /* <= QS_MAX_DET */
+  float det_rin[QS_MAX_DET], det_rout[QS_MAX_DET]; /* mrad */
+  float det_shiftx[QS_MAX_DET], det_shifty[QS_MAX_DET];
+  float electronScale;        /* beamCurrent*dwellTime*MILLISEC_PICOAMP */
+  int compat;                 /* QS_COMPAT_* bits */
+} qs_params;
+
+typedef struct qs_ctx qs_ctx;
+
+/* Creates a context on the current HIP device.  On failure *out is NULL and
+ * qs_last_error(NULL) describes the problem. */
+int qs_create(const qs_params *p, qs_ctx **out);
+void qs_destroy(qs_ctx *ctx);
+/* Last error message of ctx, or of the calling thread's last failed qs_create when ctx is NULL. */
+const char *qs_last_error(const qs_ctx *ctx);
+
+/* Projected potential V[nslice][potNx][potNy] (host, float) -> transmission
+ * functions exp(i*gamma*lambda*V), band limited when bandlimittrans is set. */
+int qs_set_potential(qs_ctx *ctx, const float *V, int nslice);
+/* Transmission functions directly, complex64 [nslice][potNx][potNy] (interleaved re,im). */
+int qs_set_trans(qs_ctx *ctx, const float *trans, int nslice);
+int qs_get_trans(qs_ctx *ctx, float *trans_out);
+
+/* Forms the probe at the window centre (the only position the driver uses,
+ * stem3.cpp:2838); keeps it on the device; copies it to out (nx*ny complex64) when non-NULL. */
+int qs_make_probe(qs_ctx *ctx, float *out);
+
+/* Bin bookkeeping of collectIntensity: for a slab of `slices` slices starting at
+ * absolute slice totalSliceCount, bin_of_slice[s] receives the output-thickness
+ * bin that slice s leaves as its final value (or -1).  Returns the number of bins
+ * (tCount+1), or -1 on bad arguments. */
+int qs_record_bins(int slices, int cellDiv, int outputInterval, int totalSliceCount, int *bin_of_slice);
+
+/* Runs nslice slices for nprobe scan positions.
+ *   ipos          [nprobe][2] integer window offsets into the transmission array
+ *   start         0: start from the probe; 1: continue the waves left by the previous call
+ *   bin_of_slice  [nslice], -1 or the detector bin to record after that slice
+ *   det           in/out [nbins][ndet][nprobe]; recorded bins are overwritten (nullable)
+ *   diff          out [nprobe][nx][ny] fftshifted |Psi|^2/sqrt(nx*ny) after the last slice (nullable)
+ *   exit_wave     out [nprobe][nx][ny] complex64 real-space waves after the last slice (nullable)
+ *   int_intensity out [nprobe] sum|psi|^2/(nx*ny) of the exit waves (nullable)
+ * All pointers are host pointers. */
+int qs_run_stem(qs_ctx *ctx, const int *ipos, int nprobe, int start, const int *bin_of_slice, int nslice, int nbins,
+                float *det, float *diff, float *exit_wave, float *int_intensity);
+
+/* Probes advanced together through all slices before the next group starts
+ * (0 = library default).  batch = 1 is the per-probe launch sequence. */
+int qs_set_batch(qs_ctx *ctx, int batch);
+int qs_get_batch(const qs_ctx *ctx);
+
+/* Device time of the last qs_run_stem in milliseconds (HIP events on the context stream). */
+float qs_last_run_ms(const qs_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

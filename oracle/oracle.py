@@ -1,0 +1,192 @@
+"""ctypes wrapper of the FP64 CPU oracle (oracle/_build/libqs_oracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, as the checker.  The product path never uses it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libqs_oracle.so")
+MAX_DET = 16
+_F, _I, _D = ctypes.c_float, ctypes.c_int, ctypes.c_double
+
+
+class QoParams(ctypes.Structure):
+    _fields_ = [
+        ("nx", _I), ("ny", _I), ("resolutionX", _F), ("resolutionY", _F), ("v0", _F),
+        ("slices", _I), ("sliceThickness", _F), ("potNx", _I), ("potNy", _I),
+        ("potSizeX", _F), ("potSizeY", _F), ("by", _F), ("compat_q1", _I), ("bandlimittrans", _I),
+        ("Cs", _F), ("C5", _F), ("df0", _F), ("astigMag", _F), ("astigAngle", _F),
+        ("a33", _F), ("a31", _F), ("a44", _F), ("a42", _F), ("a55", _F),
+        ("a53", _F), ("a51", _F), ("a66", _F), ("a64", _F), ("a62", _F),
+        ("phi33", _F), ("phi31", _F), ("phi44", _F), ("phi42", _F), ("phi55", _F),
+        ("phi53", _F), ("phi51", _F), ("phi66", _F), ("phi64", _F), ("phi62", _F),
+        ("alpha", _F), ("aAIS", _F), ("Cc", _F), ("dE_E", _F),
+        ("ismoth", _I), ("gaussFlag", _I), ("gaussScale", _F), ("compat_q2", _I),
+        ("ndet", _I), ("det_rin", _F * MAX_DET), ("det_rout", _F * MAX_DET),
+        ("det_shiftx", _F * MAX_DET), ("det_shifty", _F * MAX_DET), ("electronScale", _F),
+    ]
+
+
+class LutGeom(ctypes.Structure):
+    _fields_ = [("nx", _I), ("nz", _I), ("nzPerSlice", _I), ("Nr", _I), ("Nz_lut", _I),
+                ("dkx", _D), ("dkz", _D), ("smax2", _D)]
+
+
+class QoAtom(ctypes.Structure):
+    _fields_ = [("x", _F), ("y", _F), ("z", _F), ("dw", _F), ("occ", _F), ("q", _F), ("Znum", _I)]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def load():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.qo_wavelength.restype = _D
+        _lib.qo_wavelength.argtypes = [_D]
+        _lib.qo_sigma.restype = _D
+        _lib.qo_sigma.argtypes = [_D]
+        _lib.qo_seval.restype = _D
+        _lib.qo_record_bins.restype = _I
+    return _lib
+
+
+def _p(a, t=ctypes.c_double):
+    return None if a is None else a.ctypes.data_as(ctypes.POINTER(t))
+
+
+def params(cfg):
+    """QoParams from a qstem_amd.StemConfig-like object (same field names)."""
+    p = QoParams()
+    for k in ("nx", "ny", "slices", "potNx", "potNy", "bandlimittrans", "ismoth", "gaussFlag"):
+        setattr(p, k, int(getattr(cfg, k)))
+    for k in ("resolutionX", "resolutionY", "v0", "sliceThickness", "by", "Cs", "C5", "df0", "astigMag",
+              "astigAngle", "alpha", "aAIS", "Cc", "dE_E", "gaussScale", "electronScale"):
+        setattr(p, k, float(getattr(cfg, k)))
+    p.potSizeX = float(np.float32(cfg.potNx) * np.float32(cfg.resolutionX))
+    p.potSizeY = float(np.float32(cfg.potNy) * np.float32(cfg.resolutionY))
+    for k, v in getattr(cfg, "aberrations", {}).items():
+        setattr(p, k, float(v))
+    p.compat_q1 = 1 if (cfg.compat & 1) else 0
+    p.compat_q2 = 1 if (cfg.compat & 2) else 0
+    p.ndet = len(cfg.detectors)
+    for i, d in enumerate(cfg.detectors):
+        p.det_rin[i], p.det_rout[i] = d.rInside, d.rOutside
+        p.det_shiftx[i], p.det_shifty[i] = d.shiftX, d.shiftY
+    return p
+
+
+def fft2(a, sign):
+    a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    load().qo_fft2d(_p(a), a.shape[0], a.shape[1], sign)
+    return a
+
+
+def fft1(a, sign):
+    a = np.ascontiguousarray(a, dtype=np.complex128).copy()
+    load().qo_fft1d(_p(a), a.shape[0], sign)
+    return a
+
+
+def wavelength(kev):
+    return load().qo_wavelength(kev)
+
+
+def probe(cfg):
+    out = np.empty((cfg.nx, cfg.ny), dtype=np.complex128)
+    p = params(cfg)
+    load().qo_probe(ctypes.byref(p), _p(out))
+    return out
+
+
+def k_tables(cfg):
+    kx2 = np.empty(cfg.nx, np.float32)
+    ky2 = np.empty(cfg.ny, np.float32)
+    km = ctypes.c_float()
+    p = params(cfg)
+    load().qo_k_tables(ctypes.byref(p), _p(kx2, _F), _p(ky2, _F), ctypes.byref(km))
+    return kx2, ky2, km.value
+
+
+def make_trans(cfg, V):
+    V = np.ascontiguousarray(V, dtype=np.float32)
+    out = np.empty(V.shape, dtype=np.complex128)
+    p = params(cfg)
+    load().qo_make_trans(ctypes.byref(p), _p(V, _F), V.shape[0], _p(out))
+    return out
+
+
+def record_bins(slices, cellDiv=1, outputInterval=None, totalSliceCount=0, mulsRepeat1=1):
+    outputInterval = slices if outputInterval is None else outputInterval
+    out = np.empty(slices * mulsRepeat1, dtype=np.int32)
+    nb = load().qo_record_bins(slices, cellDiv, outputInterval, totalSliceCount, mulsRepeat1, _p(out, _I))
+    return out, nb
+
+
+def multislice(cfg, trans, ipos, psi0, bins=None, nbins=0, want_diff=False):
+    """Returns (exit waves [n][nx][ny], det [nbins][ndet][n] or None, diff or None)."""
+    trans = np.ascontiguousarray(trans, dtype=np.complex128)
+    ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
+    n = ipos.shape[0]
+    psi = np.ascontiguousarray(np.broadcast_to(psi0, (n, cfg.nx, cfg.ny)), dtype=np.complex128).copy()
+    det = np.zeros((nbins, len(cfg.detectors), n)) if bins is not None and nbins > 0 else None
+    diff = np.zeros((n, cfg.nx, cfg.ny)) if want_diff else None
+    b = None if bins is None else np.ascontiguousarray(bins, dtype=np.int32)
+    p = params(cfg)
+    load().qo_multislice(ctypes.byref(p), _p(trans), trans.shape[0], _p(ipos, _I), n, _p(psi),
+                         _p(b, _I), nbins, _p(det), _p(diff))
+    return psi, det, diff
+
+
+def lut_geometry(atomRadius, resX, sliceThickness):
+    g = LutGeom()
+    load().qo_lut_geometry(_F(atomRadius), _F(resX), _F(sliceThickness), ctypes.byref(g))
+    return g
+
+
+def rez_table():
+    path = os.path.join(os.path.dirname(HERE), "qstem_amd", "data", "rez_scatpar_f64.bin")
+    return np.fromfile(path, dtype="<f8").reshape(93, 123)
+
+
+def atom_luts(g, Zs, Bs, sliceThickness):
+    """LUT per (Z, B) pair; applies the s-grid taper of the reference first."""
+    tab = rez_table().copy()
+    sgrid = np.ascontiguousarray(tab[0])
+    rows = np.ascontiguousarray(tab[1:])
+    load().qo_taper_scatpar(_p(sgrid), _p(rows), rows.shape[0], _D(g.smax2))
+    luts = []
+    for Z, B in zip(Zs, Bs):
+        frow = np.ascontiguousarray(rows[Z - 1])
+        lut = np.zeros((g.Nz_lut, g.Nr), np.float32)
+        load().qo_atom_lut(ctypes.byref(g), _p(sgrid), _p(frow), _D(B), _F(sliceThickness), _p(lut, _F))
+        luts.append(lut)
+    return luts
+
+
+def make_slices(cfg, g, luts, kinds, atoms, potOffsetX, potOffsetY, czOffset=0.0, centerSlices=0,
+                atomRadius=5.0, periodic=0):
+    """atoms: structured array or list of (x, y, z, dw, occ, q, Znum), z-sorted."""
+    arr = (QoAtom * len(atoms))()
+    for i, a in enumerate(atoms):
+        arr[i] = QoAtom(*[float(v) for v in a[:6]], int(a[6]))
+    lut_ptrs = (ctypes.POINTER(_F) * len(luts))(*[_p(l, _F) for l in luts])
+    kinds = np.ascontiguousarray(kinds, dtype=np.int32)
+    V = np.zeros((cfg.slices, cfg.potNx, cfg.potNy), np.float32)
+    p = params(cfg)
+    load().qo_make_slices(ctypes.byref(p), ctypes.byref(g), lut_ptrs, _p(kinds, _I), arr, len(atoms),
+                          _F(potOffsetX), _F(potOffsetY), _F(czOffset), centerSlices, _F(atomRadius), periodic,
+                          _p(V, _F))
+    return V

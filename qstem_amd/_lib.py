@@ -1,0 +1,90 @@
+"""ctypes binding of libqstem_hip.so (include/qstem.h).
+
+The library is the product path; there is no CPU fallback.  Loading fails
+with an ImportError-style message when the in-tree build is missing.
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libqstem_hip.so")
+
+QS_OK = 0
+QS_ERR_ARG = 1
+QS_ERR_UNSUPPORTED = 2
+QS_ERR_HIP = 3
+QS_ERR_STATE = 4
+QS_MAX_DET = 16
+QS_COMPAT_Q1 = 1
+QS_COMPAT_Q2 = 2
+
+_F = ctypes.c_float
+_I = ctypes.c_int
+
+
+class QsParams(ctypes.Structure):
+    """Mirror of struct qs_params (include/qstem.h)."""
+    _fields_ = [
+        ("nx", _I), ("ny", _I),
+        ("resolutionX", _F), ("resolutionY", _F),
+        ("v0", _F),
+        ("slices", _I),
+        ("sliceThickness", _F),
+        ("potNx", _I), ("potNy", _I),
+        ("potSizeX", _F), ("potSizeY", _F),
+        ("by", _F),
+        ("bandlimittrans", _I),
+        ("Cs", _F), ("C5", _F), ("df0", _F), ("astigMag", _F), ("astigAngle", _F),
+        ("a33", _F), ("a31", _F), ("a44", _F), ("a42", _F), ("a55", _F),
+        ("a53", _F), ("a51", _F), ("a66", _F), ("a64", _F), ("a62", _F),
+        ("phi33", _F), ("phi31", _F), ("phi44", _F), ("phi42", _F), ("phi55", _F),
+        ("phi53", _F), ("phi51", _F), ("phi66", _F), ("phi64", _F), ("phi62", _F),
+        ("alpha", _F),
+        ("aAIS", _F),
+        ("Cc", _F), ("dE_E", _F),
+        ("ismoth", _I), ("gaussFlag", _I),
+        ("gaussScale", _F),
+        ("ndet", _I),
+        ("det_rin", _F * QS_MAX_DET), ("det_rout", _F * QS_MAX_DET),
+        ("det_shiftx", _F * QS_MAX_DET), ("det_shifty", _F * QS_MAX_DET),
+        ("electronScale", _F),
+        ("compat", _I),
+    ]
+
+
+# Symbols declared in include/qstem.h, with their ctypes signatures.
+_P = ctypes.c_void_p
+_FP = ctypes.POINTER(ctypes.c_float)
+_IP = ctypes.POINTER(ctypes.c_int)
+SIGNATURES = {
+    "qs_create": (_I, [ctypes.POINTER(QsParams), ctypes.POINTER(_P)]),
+    "qs_destroy": (None, [_P]),
+    "qs_last_error": (ctypes.c_char_p, [_P]),
+    "qs_set_potential": (_I, [_P, _FP, _I]),
+    "qs_set_trans": (_I, [_P, _FP, _I]),
+    "qs_get_trans": (_I, [_P, _FP]),
+    "qs_make_probe": (_I, [_P, _FP]),
+    "qs_record_bins": (_I, [_I, _I, _I, _I, _IP]),
+    "qs_run_stem": (_I, [_P, _IP, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
+    "qs_set_batch": (_I, [_P, _I]),
+    "qs_get_batch": (_I, [_P]),
+    "qs_last_run_ms": (_F, [_P]),
+}
+
+_lib = None
+
+
+def load():
+    """Load the in-tree library once; raise if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise ImportError("qstem_amd: %s is missing; build it with `python -m qstem_amd.build`" % LIB_PATH)
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = lib
+    return lib

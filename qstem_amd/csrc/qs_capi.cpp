@@ -1,0 +1,585 @@
+// qs_capi.cpp -- C ABI over the gfx950 multislice kernels.
+//
+// Host responsibilities: validate parameters, derive the small
+// single-precision lookup tables exactly as the reference derives them
+// (k^2 tables and band limit of propagate_slow, detector k^2 limits of
+// readFile), allocate device buffers, and sequence the sweeps.  All
+// per-pixel work runs in the kernels of qs_kernels.hip.
+#include <hip/hip_runtime.h>
+
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/qstem.h"
+#include "qs_internal.hpp"
+
+using qs::ProbeArgs;
+using qs::SweepArgs;
+
+namespace {
+
+thread_local std::string g_create_error;
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+  hipError_t ensure(size_t n) {
+    if (n <= bytes && p) return hipSuccess;
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipMalloc(&p, n > 0 ? n : 16);
+    if (e == hipSuccess) bytes = n;
+    return e;
+  }
+  void release() {
+    if (p) hipFree(p);
+    p = nullptr;
+    bytes = 0;
+  }
+  template <class T> T* as() const { return static_cast<T*>(p); }
+};
+
+// Small double-precision FFT for table set-up (Bluestein kernels' vhat).
+void host_fft_pow2(std::vector<std::complex<double>>& a, int sign) {
+  const int n = (int)a.size();
+  for (int i = 1, j = 0; i < n; i++) {
+    int bit = n >> 1;
+    for (; j & bit; bit >>= 1) j ^= bit;
+    j ^= bit;
+    if (i < j) std::swap(a[i], a[j]);
+  }
+  for (int len = 2; len <= n; len <<= 1) {
+    for (int k = 0; k < len / 2; k++) {
+      const double ang = sign * 2.0 * M_PI * k / len;
+      const std::complex<double> w(cos(ang), sin(ang));
+      for (int i = 0; i < n; i += len) {
+        auto u = a[i + k], v = a[i + k + len / 2] * w;
+        a[i + k] = u + v;
+        a[i + k + len / 2] = u - v;
+      }
+    }
+  }
+}
+
+bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
+double wavelength(double kev) {  // stem3/stemutil.cpp:714-725
+  const double emass = 510.99906, hc = 12.3984244;
+  return hc / sqrt(kev * (2.0 * emass + kev));
+}
+
+std::vector<float2> twiddles(int n) {
+  std::vector<float2> t(n);
+  for (int m = 0; m < n; m++) {
+    const double a = -2.0 * M_PI * m / n;
+    t[m] = make_float2((float)cos(a), (float)sin(a));
+  }
+  return t;
+}
+
+// Row transform of length n for the transmission band limit: either a direct
+// power-of-two Stockham transform or Bluestein through M = 2^k >= 2n-1.
+struct RowXform {
+  int n = 0, M = 0;
+  DevBuf tw, chirp_f, vhat_f, chirp_b, vhat_b;
+};
+
+}  // namespace
+
+struct qs_ctx {
+  qs_params p{};
+  hipStream_t stream = nullptr;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  float last_ms = 0.f;
+  std::string err;
+  int batch = 0;
+  // derived tables
+  int band_a = 0, band_b = 0, kc = 0, kcp = 0, ntiles = 0, tc = 0;
+  float k2max = 0.f;
+  std::vector<float> kx2, ky2;
+  float det_k2in[QS_MAX_DET]{}, det_k2out[QS_MAX_DET]{};
+  // device
+  DevBuf twx, twy, px, py, dkx2, dky2;
+  DevBuf probe, trans, scratch, V;
+  DevBuf state, ipos, partial, det, diff, exitw, total;
+  DevBuf pkx2, pky2, probe_partial;
+  RowXform rx, ry;  // potNx and potNy row transforms
+  int trans_slices = 0;
+  bool have_probe = false;
+  int state_probes = 0;
+  bool state_valid = false;
+};
+
+static int fail(qs_ctx* c, int code, const std::string& m) {
+  if (c) c->err = m;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                   \
+  do {                                                                                      \
+    hipError_t e__ = (expr);                                                                \
+    if (e__ != hipSuccess)                                                                  \
+      return fail(ctx, QS_ERR_HIP, std::string(#expr) + ": " + hipGetErrorString(e__));     \
+  } while (0)
+
+template <class T>
+static hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
+  hipError_t e = b.ensure(sizeof(T) * v.size());
+  if (e != hipSuccess) return e;
+  return hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st);
+}
+
+static int setup_rowxform(qs_ctx* c, RowXform& r, int n) {
+  r.n = n;
+  if (is_pow2(n) && n >= 64 && n <= 4096) {
+    r.M = 0;
+    HIPCHK(c, upload(r.tw, twiddles(n), c->stream));
+    return QS_OK;
+  }
+  int M = 64;
+  while (M < 2 * n - 1) M <<= 1;
+  if (M > 4096)
+    return fail(c, QS_ERR_UNSUPPORTED, "transmission size " + std::to_string(n) +
+                                           " needs a Bluestein length above 4096 (only sizes <= 2048 or powers of two "
+                                           "<= 4096 are supported)");
+  r.M = M;
+  HIPCHK(c, upload(r.tw, twiddles(M), c->stream));
+  for (int dir = -1; dir <= 1; dir += 2) {
+    std::vector<float2> ch(n), vh(M);
+    std::vector<std::complex<double>> cd(n), v(M, 0.0);
+    for (int k = 0; k < n; k++) {
+      const long long k2 = ((long long)k * k) % (2LL * n);
+      const double ang = dir * M_PI * (double)k2 / n;
+      cd[k] = std::complex<double>(cos(ang), sin(ang));
+      ch[k] = make_float2((float)cd[k].real(), (float)cd[k].imag());
+    }
+    v[0] = std::conj(cd[0]);
+    for (int k = 1; k < n; k++) v[k] = v[M - k] = std::conj(cd[k]);
+    host_fft_pow2(v, -1);
+    for (int k = 0; k < M; k++) vh[k] = make_float2((float)v[k].real(), (float)v[k].imag());
+    HIPCHK(c, upload(dir < 0 ? r.chirp_f : r.chirp_b, ch, c->stream));
+    HIPCHK(c, upload(dir < 0 ? r.vhat_f : r.vhat_b, vh, c->stream));
+  }
+  return QS_OK;
+}
+
+static void run_rowxform(qs_ctx* c, RowXform& r, float2* data, long nrows, int dir, float scale) {
+  if (r.M == 0) qs::launch_rows_fft(data, nrows, r.n, dir, r.tw.as<float2>(), scale, c->stream);
+  else
+    qs::launch_rows_bluestein(data, nrows, r.n, r.M, (dir < 0 ? r.chirp_f : r.chirp_b).as<float2>(),
+                              (dir < 0 ? r.vhat_f : r.vhat_b).as<float2>(), r.tw.as<float2>(), scale, c->stream);
+}
+
+extern "C" {
+
+const char* qs_last_error(const qs_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
+
+int qs_create(const qs_params* p, qs_ctx** out) {
+  if (!out) return QS_ERR_ARG;
+  *out = nullptr;
+  g_create_error.clear();
+  if (!p) { g_create_error = "params is NULL"; return QS_ERR_ARG; }
+  if (!qs::supported_size(p->nx) || !qs::supported_size(p->ny)) {
+    g_create_error = "nx and ny must be powers of two in [64, 4096]";
+    return QS_ERR_UNSUPPORTED;
+  }
+  if (p->potNx < p->nx || p->potNy < p->ny) { g_create_error = "potNx/potNy smaller than nx/ny"; return QS_ERR_ARG; }
+  if (p->ndet < 0 || p->ndet > QS_MAX_DET) { g_create_error = "ndet out of range"; return QS_ERR_ARG; }
+  if (!(p->resolutionX > 0) || !(p->resolutionY > 0) || !(p->v0 > 0) || p->slices < 1) {
+    g_create_error = "resolution, v0 and slices must be positive";
+    return QS_ERR_ARG;
+  }
+  qs_ctx* c = new qs_ctx();
+  c->p = *p;
+  auto bad = [&](int code) {
+    g_create_error = c->err;
+    qs_destroy(c);
+    return code;
+  };
+  if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+      hipEventCreate(&c->ev1) != hipSuccess) {
+    c->err = "hipStreamCreate/hipEventCreate failed (no HIP device?)";
+    return bad(QS_ERR_HIP);
+  }
+  if (qs::set_col_lds_limits() != hipSuccess) {
+    c->err = "hipFuncSetAttribute for the column sweep LDS size failed";
+    return bad(QS_ERR_HIP);
+  }
+  const int nx = p->nx, ny = p->ny;
+  // k^2 tables and band limit as propagate_slow builds them (stemlib.cpp:2782-2822), float arithmetic
+  const float ax = p->resolutionX * nx, by = p->resolutionY * ny;
+  c->kx2.resize(nx);
+  c->ky2.resize(ny);
+  for (int i = 0; i < nx; i++) {
+    float k = (i > nx / 2) ? (float)(i - nx) / ax : (float)i / ax;
+    c->kx2[i] = k * k;
+  }
+  for (int i = 0; i < ny; i++) {
+    float k = (i > ny / 2) ? (float)(i - ny) / by : (float)i / by;
+    c->ky2[i] = k * k;
+  }
+  float km = nx / (2.0F * ax);
+  if (ny / (2.0F * by) < km) km = ny / (2.0F * by);
+  km = (float)(2.0 / 3.0 * km);
+  c->k2max = km * km;
+  // band of ky columns that survive: [0, band_a] U [ny-band_b, ny-1]
+  int a = -1;
+  while (a + 1 <= ny / 2 && c->ky2[a + 1] < c->k2max) a++;
+  int b = 0;
+  while (b < ny / 2 - 1 && c->ky2[ny - 1 - b] < c->k2max) b++;
+  for (int i = 0; i < ny; i++) {
+    const bool in = (i <= a) || (i >= ny - b);
+    if (in != (c->ky2[i] < c->k2max)) {
+      c->err = "band-limit columns are not contiguous";
+      return bad(QS_ERR_UNSUPPORTED);
+    }
+  }
+  if (a < 0) { c->err = "band limit removes every column"; return bad(QS_ERR_ARG); }
+  c->band_a = a;
+  c->band_b = b;
+  c->kc = a + 1 + b;
+  c->tc = qs::col_tile(nx);
+  c->ntiles = (c->kc + c->tc - 1) / c->tc;
+  c->kcp = c->ntiles * c->tc;
+  // propagator e^{-i pi dz lambda k^2}, double -> float, 1/(nx ny) folded into px
+  const double wl = wavelength(p->v0), dz = p->sliceThickness;
+  const double axd = (double)p->resolutionX * nx, byd = (double)p->resolutionY * ny;
+  const double norm = 1.0 / ((double)nx * ny);
+  std::vector<float2> hpx(nx), hpy(ny);
+  for (int i = 0; i < nx; i++) {
+    const double k = ((i > nx / 2) ? (double)(i - nx) : (double)i) / axd;
+    const double t = M_PI * dz * wl * k * k;
+    hpx[i] = make_float2((float)(cos(t) * norm), (float)(-sin(t) * norm));
+  }
+  for (int i = 0; i < ny; i++) {
+    const double k = ((i > ny / 2) ? (double)(i - ny) : (double)i) / byd;
+    const double t = M_PI * dz * wl * k * k;
+    hpy[i] = make_float2((float)cos(t), (float)(-sin(t)));
+  }
+  // detector limits (stem3.cpp:1338-1347)
+  for (int d = 0; d < p->ndet; d++) {
+    float s0 = (float)(sin(p->det_rin[d] * 0.001) / wl);
+    float s1 = (float)(sin(p->det_rout[d] * 0.001) / wl);
+    c->det_k2in[d] = s0 * s0;
+    c->det_k2out[d] = s1 * s1;
+  }
+  if (upload(c->twx, twiddles(nx), c->stream) != hipSuccess || upload(c->twy, twiddles(ny), c->stream) != hipSuccess ||
+      upload(c->px, hpx, c->stream) != hipSuccess || upload(c->py, hpy, c->stream) != hipSuccess ||
+      upload(c->dkx2, c->kx2, c->stream) != hipSuccess || upload(c->dky2, c->ky2, c->stream) != hipSuccess) {
+    c->err = "device allocation of lookup tables failed";
+    return bad(QS_ERR_HIP);
+  }
+  // transmission band-limit tables (stemlib.cpp:2219-2262), float arithmetic, Q1 optional
+  if (p->bandlimittrans) {
+    std::vector<float> kx2t(p->potNx), ky2t(p->potNy);
+    for (int i = 0; i < p->potNx; i++) {
+      float k = (i > p->potNx / 2) ? (float)(i - p->potNx) / p->potSizeX : (float)i / p->potSizeX;
+      kx2t[i] = k * k;
+    }
+    const float byq = (p->compat & QS_COMPAT_Q1) ? p->by : p->potSizeY;
+    for (int i = 0; i < p->potNy; i++) {
+      float k = (i > p->potNy / 2) ? (float)(i - p->potNy) / byq : (float)i / p->potSizeY;
+      ky2t[i] = k * k;
+    }
+    if (upload(c->pkx2, kx2t, c->stream) != hipSuccess || upload(c->pky2, ky2t, c->stream) != hipSuccess) {
+      c->err = "device allocation failed";
+      return bad(QS_ERR_HIP);
+    }
+    int rc = setup_rowxform(c, c->rx, p->potNx);
+    if (rc == QS_OK) rc = setup_rowxform(c, c->ry, p->potNy);
+    if (rc != QS_OK) return bad(rc);
+  }
+  if (hipStreamSynchronize(c->stream) != hipSuccess) { c->err = "stream sync failed"; return bad(QS_ERR_HIP); }
+  *out = c;
+  return QS_OK;
+}
+
+void qs_destroy(qs_ctx* c) {
+  if (!c) return;
+  if (c->stream) hipStreamSynchronize(c->stream);
+  DevBuf* bufs[] = {&c->twx, &c->twy, &c->px, &c->py, &c->dkx2, &c->dky2, &c->probe, &c->trans, &c->scratch, &c->V,
+                    &c->state, &c->ipos, &c->partial, &c->det, &c->diff, &c->exitw, &c->total, &c->pkx2, &c->pky2,
+                    &c->probe_partial, &c->rx.tw, &c->rx.chirp_f, &c->rx.vhat_f, &c->rx.chirp_b, &c->rx.vhat_b,
+                    &c->ry.tw, &c->ry.chirp_f, &c->ry.vhat_f, &c->ry.chirp_b, &c->ry.vhat_b};
+  for (DevBuf* b : bufs) b->release();
+  if (c->ev0) hipEventDestroy(c->ev0);
+  if (c->ev1) hipEventDestroy(c->ev1);
+  if (c->stream) hipStreamDestroy(c->stream);
+  delete c;
+}
+
+static int bandlimit_trans(qs_ctx* c, int nslice) {
+  const qs_params& p = c->p;
+  const int nx = p.potNx, ny = p.potNy;
+  float2* tr = c->trans.as<float2>();
+  float2* sc = c->scratch.as<float2>();
+  run_rowxform(c, c->ry, tr, (long)nslice * nx, -1, 1.0f);
+  qs::launch_transpose(tr, sc, nslice, nx, ny, c->stream);
+  run_rowxform(c, c->rx, sc, (long)nslice * ny, -1, 1.0f);
+  float k2max = nx / (2.0F * p.potSizeX);
+  const float t = ny / (2.0F * p.potSizeY);
+  if (t < k2max) k2max = t;
+  k2max = (float)((2.0F / 3.0F) * k2max);
+  k2max = k2max * k2max;
+  qs::launch_bandlimit_T(sc, nslice, ny, nx, c->pkx2.as<float>(), c->pky2.as<float>(), k2max,
+                         (float)(1.0 / ((double)nx * ny)), c->stream);
+  run_rowxform(c, c->rx, sc, (long)nslice * ny, +1, 1.0f);
+  qs::launch_transpose(sc, tr, nslice, ny, nx, c->stream);
+  run_rowxform(c, c->ry, tr, (long)nslice * nx, +1, 1.0f);
+  HIPCHK(c, hipGetLastError());
+  return QS_OK;
+}
+
+int qs_set_potential(qs_ctx* c, const float* V, int nslice) {
+  if (!c || !V || nslice < 1) return fail(c, QS_ERR_ARG, "qs_set_potential: bad arguments");
+  const qs_params& p = c->p;
+  const size_t n = (size_t)nslice * p.potNx * p.potNy;
+  HIPCHK(c, c->V.ensure(sizeof(float) * n));
+  HIPCHK(c, c->trans.ensure(sizeof(float2) * n));
+  HIPCHK(c, hipMemcpyAsync(c->V.p, V, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+  // sigma = gamma*lambda with gamma = 1 + v0/511 in float (stemlib.cpp:2271-2274)
+  const float mm0 = 1.0F + p.v0 / 511.0F;
+  const double scale = mm0 * wavelength(p.v0);
+  qs::launch_phase_grating(c->V.as<float>(), c->trans.as<float2>(), n, scale, c->stream);
+  HIPCHK(c, hipGetLastError());
+  if (p.bandlimittrans) {
+    HIPCHK(c, c->scratch.ensure(sizeof(float2) * n));
+    int rc = bandlimit_trans(c, nslice);
+    if (rc != QS_OK) return rc;
+  }
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->trans_slices = nslice;
+  return QS_OK;
+}
+
+int qs_set_trans(qs_ctx* c, const float* trans, int nslice) {
+  if (!c || !trans || nslice < 1) return fail(c, QS_ERR_ARG, "qs_set_trans: bad arguments");
+  const size_t n = (size_t)nslice * c->p.potNx * c->p.potNy;
+  HIPCHK(c, c->trans.ensure(sizeof(float2) * n));
+  HIPCHK(c, hipMemcpyAsync(c->trans.p, trans, sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->trans_slices = nslice;
+  return QS_OK;
+}
+
+int qs_get_trans(qs_ctx* c, float* out) {
+  if (!c || !out) return fail(c, QS_ERR_ARG, "qs_get_trans: bad arguments");
+  if (c->trans_slices < 1) return fail(c, QS_ERR_STATE, "no transmission function set");
+  const size_t n = (size_t)c->trans_slices * c->p.potNx * c->p.potNy;
+  HIPCHK(c, hipMemcpyAsync(out, c->trans.p, sizeof(float2) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  return QS_OK;
+}
+
+int qs_make_probe(qs_ctx* c, float* out) {
+  if (!c) return QS_ERR_ARG;
+  const qs_params& p = c->p;
+  const int nx = p.nx, ny = p.ny;
+  ProbeArgs a{};
+  a.nx = nx;
+  a.ny = ny;
+  a.ax = nx * (double)p.resolutionX;
+  a.by = ny * (double)p.resolutionY;
+  a.rx = 1.0 / a.ax;
+  a.ry = 1.0 / a.by;
+  a.rx2 = a.rx * a.rx;
+  a.ry2 = a.ry * a.ry;
+  const double v0 = p.v0;
+  a.wavlen = 12.26 / sqrt(v0 * 1.e3 + v0 * v0 * 0.9788);  // probe's own formula, stemlib.cpp:2014
+  const double alpha = 0.001 * p.alpha;
+  a.k2max = sin(alpha) / a.wavlen;
+  a.k2max *= a.k2max;
+  a.pixel = a.rx2 + a.ry2;
+  a.scale = (double)(float)(1.0 / sqrt((double)nx * (double)ny));  // "(float) scale" at stemlib.cpp:2101
+  a.df_eff = (double)p.df0 + (double)p.Cc * (double)p.dE_E;
+  a.astigMag = p.astigMag;
+  a.astigAngle = p.astigAngle;
+  a.a33 = p.a33; a.a31 = p.a31; a.a44 = p.a44; a.a42 = p.a42; a.a55 = p.a55;
+  a.a53 = p.a53; a.a51 = p.a51; a.a66 = p.a66; a.a64 = p.a64;
+  a.a62 = (p.compat & QS_COMPAT_Q2) ? 0.0 : (double)p.a62;
+  a.phi33 = p.phi33; a.phi31 = p.phi31; a.phi44 = p.phi44; a.phi42 = p.phi42; a.phi55 = p.phi55;
+  a.phi53 = p.phi53; a.phi51 = p.phi51; a.phi66 = p.phi66; a.phi64 = p.phi64; a.phi62 = p.phi62;
+  a.Cs = p.Cs;
+  a.C5 = p.C5;
+  a.on3 = p.a33 > 0 || p.a31 > 0;
+  a.on4 = p.a44 > 0 || p.a42 > 0 || p.Cs != 0;
+  a.on5 = p.a55 > 0 || p.a53 > 0 || p.a51 > 0;
+  a.on6 = p.a66 > 0 || p.a64 > 0 || p.C5 != 0 || (!(p.compat & QS_COMPAT_Q2) && p.a62 > 0);
+  const float cxf = (float)(nx / 2) * p.resolutionX, cyf = (float)(ny / 2) * p.resolutionY;
+  a.sx = a.ax - (double)cxf;
+  a.sy = a.by - (double)cyf;
+  a.ismoth = p.ismoth;
+  a.gaussFlag = p.gaussFlag;
+  a.gaussScale = p.gaussScale;
+  a.aAIS = p.aAIS;
+  a.resX = p.resolutionX;
+  a.resY = p.resolutionY;
+  a.edge = 5.0 * sqrt(0.5 * ((double)p.resolutionX * p.resolutionX + (double)p.resolutionY * p.resolutionY));
+  const size_t n = (size_t)nx * ny;
+  HIPCHK(c, c->probe.ensure(sizeof(float2) * n));
+  HIPCHK(c, c->scratch.ensure(sizeof(float2) * n > c->scratch.bytes ? sizeof(float2) * n : c->scratch.bytes));
+  HIPCHK(c, c->probe_partial.ensure(sizeof(double) * 1024));
+  float2* pr = c->probe.as<float2>();
+  float2* sc = c->scratch.as<float2>();
+  qs::launch_probe_kspace(a, pr, c->stream);
+  // 2D inverse FFT (unnormalised): rows over ky, transpose, rows over kx, transpose back
+  qs::launch_rows_fft(pr, nx, ny, +1, c->twy.as<float2>(), 1.0f, c->stream);
+  qs::launch_transpose(pr, sc, 1, nx, ny, c->stream);
+  qs::launch_rows_fft(sc, ny, nx, +1, c->twx.as<float2>(), 1.0f, c->stream);
+  qs::launch_transpose(sc, pr, 1, ny, nx, c->stream);
+  qs::launch_probe_window(a, pr, c->probe_partial.as<double>(), c->stream);
+  HIPCHK(c, hipGetLastError());
+  if (out) HIPCHK(c, hipMemcpyAsync(out, pr, sizeof(float2) * n, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_probe = true;
+  return QS_OK;
+}
+
+int qs_record_bins(int slices, int cellDiv, int outputInterval, int totalSliceCount, int* bin_of_slice) {
+  if (slices < 1 || cellDiv < 1 || outputInterval < 0 || !bin_of_slice) return -1;
+  const int total = slices * cellDiv;
+  const int tCount = outputInterval > 0 ? total / outputInterval : 0;
+  const int nb = tCount + 1;
+  std::vector<int> last(nb, -1);
+  for (int s = 0; s < slices; s++) {
+    const int slice = totalSliceCount + s;
+    int t;
+    if (outputInterval == 0) t = 0;
+    else if (slice < total - 1) t = slice / outputInterval;
+    else t = tCount;
+    bin_of_slice[s] = -1;
+    if (t >= 0 && t < nb) last[t] = s;
+  }
+  for (int t = 0; t < nb; t++)
+    if (last[t] >= 0) bin_of_slice[last[t]] = t;
+  return nb;
+}
+
+int qs_set_batch(qs_ctx* c, int batch) {
+  if (!c || batch < 0) return fail(c, QS_ERR_ARG, "qs_set_batch: bad arguments");
+  c->batch = batch;
+  return QS_OK;
+}
+
+int qs_get_batch(const qs_ctx* c) { return c ? c->batch : -1; }
+
+float qs_last_run_ms(const qs_ctx* c) { return c ? c->last_ms : 0.f; }
+
+int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bin_of_slice, int nslice, int nbins,
+                float* det, float* diff, float* exit_wave, float* int_intensity) {
+  if (!c) return QS_ERR_ARG;
+  const qs_params& p = c->p;
+  if (!ipos || nprobe < 1 || nslice < 1 || (start != 0 && start != 1))
+    return fail(c, QS_ERR_ARG, "qs_run_stem: bad arguments");
+  if (nslice > c->trans_slices) return fail(c, QS_ERR_STATE, "qs_run_stem: fewer transmission slices than nslice");
+  if (start == 0 && !c->have_probe) return fail(c, QS_ERR_STATE, "qs_run_stem: qs_make_probe not called");
+  if (start == 1 && (!c->state_valid || c->state_probes != nprobe))
+    return fail(c, QS_ERR_STATE, "qs_run_stem: no wave state to continue for this probe count");
+  for (int i = 0; i < nprobe; i++) {
+    const int x = ipos[2 * i], y = ipos[2 * i + 1];
+    if (x < 0 || y < 0 || x + p.nx > p.potNx || y + p.ny > p.potNy)
+      return fail(c, QS_ERR_ARG, "qs_run_stem: window offset outside the transmission array");
+  }
+  if (det && p.ndet > 0) {
+    if (!bin_of_slice || nbins < 1) return fail(c, QS_ERR_ARG, "qs_run_stem: det needs bin_of_slice and nbins");
+    for (int s = 0; s < nslice; s++)
+      if (bin_of_slice[s] >= nbins) return fail(c, QS_ERR_ARG, "qs_run_stem: bin index >= nbins");
+  }
+  const int nx = p.nx, ny = p.ny;
+  const int B = c->batch > 0 ? (c->batch < nprobe ? c->batch : nprobe) : (nprobe < 256 ? nprobe : 256);
+  const size_t state_elems = (size_t)nprobe * nx * c->kcp;
+  HIPCHK(c, c->state.ensure(sizeof(float2) * state_elems));
+  if (start == 0) qs::launch_fill_zero(c->state.as<float2>(), state_elems, c->stream);  // padding columns stay zero
+  HIPCHK(c, c->ipos.ensure(sizeof(int) * 2 * nprobe));
+  HIPCHK(c, hipMemcpyAsync(c->ipos.p, ipos, sizeof(int) * 2 * nprobe, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, c->partial.ensure(sizeof(float) * (size_t)B * c->ntiles * (qs::kMaxDet + 1)));
+  const bool want_det = det && p.ndet > 0;
+  if (want_det) {
+    const size_t nd = (size_t)nbins * p.ndet * nprobe;
+    HIPCHK(c, c->det.ensure(sizeof(float) * nd));
+    HIPCHK(c, hipMemcpyAsync(c->det.p, det, sizeof(float) * nd, hipMemcpyHostToDevice, c->stream));
+  }
+  if (diff) {
+    HIPCHK(c, c->diff.ensure(sizeof(float) * (size_t)nprobe * nx * ny));
+    HIPCHK(c, hipMemsetAsync(c->diff.p, 0, sizeof(float) * (size_t)nprobe * nx * ny, c->stream));
+  }
+  if (exit_wave) HIPCHK(c, c->exitw.ensure(sizeof(float2) * (size_t)nprobe * nx * ny));
+  if (int_intensity) HIPCHK(c, c->total.ensure(sizeof(float) * nprobe));
+
+  SweepArgs a{};
+  a.nx = nx;
+  a.ny = ny;
+  a.kc = c->kc;
+  a.kcp = c->kcp;
+  a.band_a = c->band_a;
+  a.band_b = c->band_b;
+  a.ntiles = c->ntiles;
+  a.potNy = p.potNy;
+  a.probe = c->probe.as<float2>();
+  a.ipos = c->ipos.as<int2>();
+  a.state = c->state.as<float2>();
+  a.out_wave = c->exitw.as<float2>();
+  a.twx = c->twx.as<float2>();
+  a.twy = c->twy.as<float2>();
+  a.px = c->px.as<float2>();
+  a.py = c->py.as<float2>();
+  a.kx2 = c->dkx2.as<float>();
+  a.ky2 = c->dky2.as<float>();
+  a.k2max = c->k2max;
+  a.ndet = p.ndet;
+  for (int d = 0; d < p.ndet; d++) {
+    a.det_sx[d] = (int)p.det_shiftx[d];
+    a.det_sy[d] = (int)p.det_shifty[d];
+    a.det_k2in[d] = c->det_k2in[d];
+    a.det_k2out[d] = c->det_k2out[d];
+  }
+  a.partial = c->partial.as<float>();
+  a.diff_scale = (float)pow((double)nx * ny, 1.5);
+  const size_t slice_elems = (size_t)p.potNx * p.potNy;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  for (int p0 = 0; p0 < nprobe; p0 += B) {
+    const int nb = (nprobe - p0) < B ? (nprobe - p0) : B;
+    a.nprobe = nb;
+    a.probe0 = p0;
+    for (int s = 0; s < nslice; s++) {
+      a.trans = c->trans.as<float2>() + (size_t)s * slice_elems;
+      const bool last = (s == nslice - 1);
+      qs::launch_sweep_rows(a, (s == 0 && start == 0) ? 0 : 1, c->stream);
+      const int bin = (want_det && bin_of_slice) ? bin_of_slice[s] : -1;
+      const bool tot = last && int_intensity;
+      a.rec_bin = (bin >= 0 || tot) ? (bin >= 0 ? bin : 0) : -1;
+      a.diff = (last && diff) ? c->diff.as<float>() : nullptr;
+      qs::launch_sweep_cols(a, a.rec_bin >= 0, a.diff != nullptr, c->stream);
+      if (bin >= 0 || tot) {
+        float* dbin = bin >= 0 ? c->det.as<float>() + (size_t)bin * p.ndet * nprobe : nullptr;
+        qs::launch_reduce_partials(c->partial.as<float>(), nb, c->ntiles, bin >= 0 ? p.ndet : 0, p.electronScale, dbin,
+                                   nprobe, tot ? c->total.as<float>() : nullptr, p0, c->stream);
+      }
+    }
+    if (exit_wave) qs::launch_sweep_rows(a, 2, c->stream);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  if (want_det)
+    HIPCHK(c, hipMemcpyAsync(det, c->det.p, sizeof(float) * (size_t)nbins * p.ndet * nprobe, hipMemcpyDeviceToHost,
+                             c->stream));
+  if (diff)
+    HIPCHK(c, hipMemcpyAsync(diff, c->diff.p, sizeof(float) * (size_t)nprobe * nx * ny, hipMemcpyDeviceToHost, c->stream));
+  if (exit_wave)
+    HIPCHK(c, hipMemcpyAsync(exit_wave, c->exitw.p, sizeof(float2) * (size_t)nprobe * nx * ny, hipMemcpyDeviceToHost,
+                             c->stream));
+  if (int_intensity)
+    HIPCHK(c, hipMemcpyAsync(int_intensity, c->total.p, sizeof(float) * nprobe, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  c->state_valid = true;
+  c->state_probes = nprobe;
+  return QS_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,210 @@
+// qs_fft.hpp -- wavefront-resident Stockham FFT building blocks for gfx950.
+//
+// A length-N transform is owned by T = N/E consecutive lanes of one 64-wide
+// wavefront; each lane holds E complex values in registers.  In the natural
+// layout lane t holds elements t + T*m, m = 0..E-1, which is also the layout
+// coalesced global loads and stores produce, so the first pass takes its
+// input straight from registers and the last pass leaves its output there.
+// Between passes the values are exchanged through a per-transform LDS slot.
+//
+// Pass structure (Stockham autosort): for radix R and span Ns, butterfly
+// j = t + T*b (b < E/R) reads x[j + k*N/R], applies twiddles
+// W_{Ns R}^{k (j mod Ns)} and writes y_k to out[(j/Ns)*Ns*R + j%Ns + k*Ns].
+// Directions follow FFTW/QSTEM: DIR = -1 forward, +1 backward, both
+// unnormalised (libs/data_containers.cpp:50-51).
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace qs {
+
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ float2 cmul(float2 a, float2 b) {
+  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ float2 cmulc(float2 a, float2 b) {  // a * conj(b)
+  return make_float2(a.x * b.x + a.y * b.y, a.y * b.x - a.x * b.y);
+}
+__device__ __forceinline__ float2 cscale(float2 a, float s) { return make_float2(a.x * s, a.y * s); }
+__device__ __forceinline__ float cnorm(float2 a) { return a.x * a.x + a.y * a.y; }
+// multiply by -i (DIR=-1) or +i (DIR=+1)
+template <int DIR>
+__device__ __forceinline__ float2 mul_i(float2 a) {
+  return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+}
+
+// ---- in-register DFTs ------------------------------------------------------
+template <int DIR>
+__device__ __forceinline__ void dft2(float2& a, float2& b) {
+  float2 t = a;
+  a = cadd(t, b);
+  b = csub(t, b);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2& a3) {
+  float2 t0 = cadd(a0, a2), t1 = csub(a0, a2);
+  float2 t2 = cadd(a1, a3), t3 = mul_i<DIR>(csub(a1, a3));
+  a0 = cadd(t0, t2);
+  a2 = csub(t0, t2);
+  a1 = cadd(t1, t3);
+  a3 = csub(t1, t3);
+}
+
+// W8^k for k = 1,3 with sign DIR: e^{DIR*i*pi*k/4}
+template <int DIR>
+__device__ __forceinline__ float2 w8_1(float2 a) {  // a * e^{DIR i pi/4}
+  const float h = 0.70710678118654752440f;
+  return DIR < 0 ? make_float2((a.x + a.y) * h, (a.y - a.x) * h) : make_float2((a.x - a.y) * h, (a.y + a.x) * h);
+}
+template <int DIR>
+__device__ __forceinline__ float2 w8_3(float2 a) {  // a * e^{DIR i 3pi/4}
+  const float h = 0.70710678118654752440f;
+  return DIR < 0 ? make_float2((a.y - a.x) * h, -(a.x + a.y) * h) : make_float2(-(a.x + a.y) * h, (a.x - a.y) * h);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft8(float2* a) {
+  // decimation in time: even/odd halves, then combine with W8^k
+  float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
+  float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
+  dft4<DIR>(e0, e1, e2, e3);
+  dft4<DIR>(o0, o1, o2, o3);
+  o1 = w8_1<DIR>(o1);
+  o2 = mul_i<DIR>(o2);
+  o3 = w8_3<DIR>(o3);
+  a[0] = cadd(e0, o0); a[4] = csub(e0, o0);
+  a[1] = cadd(e1, o1); a[5] = csub(e1, o1);
+  a[2] = cadd(e2, o2); a[6] = csub(e2, o2);
+  a[3] = cadd(e3, o3); a[7] = csub(e3, o3);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft16(float2* a) {
+  // 16 = 4 x 4: columns of stride 4, twiddles W16^{n1 k2}, rows of length 4
+  const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
+  const float h = 0.70710678118654752440f;
+  float2 b[16];
+#pragma unroll
+  for (int n1 = 0; n1 < 4; n1++) {
+    float2 x0 = a[n1], x1 = a[n1 + 4], x2 = a[n1 + 8], x3 = a[n1 + 12];
+    dft4<DIR>(x0, x1, x2, x3);
+    b[n1 * 4 + 0] = x0; b[n1 * 4 + 1] = x1; b[n1 * 4 + 2] = x2; b[n1 * 4 + 3] = x3;
+  }
+  // twiddle b[n1*4+k2] *= W16^{n1*k2}
+  const float sg = (float)DIR;
+  const float2 w1 = make_float2(c1, sg * s1), w2 = make_float2(h, sg * h), w3 = make_float2(s1, sg * c1);
+  const float2 w6 = make_float2(-h, sg * h), w9 = make_float2(-c1, -sg * s1);
+  b[1 * 4 + 1] = cmul(b[5], w1);
+  b[1 * 4 + 2] = cmul(b[6], w2);
+  b[1 * 4 + 3] = cmul(b[7], w3);
+  b[2 * 4 + 1] = cmul(b[9], w2);
+  b[2 * 4 + 2] = mul_i<DIR>(b[10]);
+  b[2 * 4 + 3] = cmul(b[11], w6);
+  b[3 * 4 + 1] = cmul(b[13], w3);
+  b[3 * 4 + 2] = cmul(b[14], w6);
+  b[3 * 4 + 3] = cmul(b[15], w9);
+#pragma unroll
+  for (int k2 = 0; k2 < 4; k2++) {
+    float2 x0 = b[k2], x1 = b[4 + k2], x2 = b[8 + k2], x3 = b[12 + k2];
+    dft4<DIR>(x0, x1, x2, x3);
+    a[k2] = x0; a[k2 + 4] = x1; a[k2 + 8] = x2; a[k2 + 12] = x3;
+  }
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2* a) {
+  if constexpr (R == 2) dft2<DIR>(a[0], a[1]);
+  else if constexpr (R == 4) dft4<DIR>(a[0], a[1], a[2], a[3]);
+  else if constexpr (R == 8) dft8<DIR>(a);
+  else if constexpr (R == 16) dft16<DIR>(a);
+}
+
+// ---- radix plans -------------------------------------------------------------
+// E: values per lane; R0,R1,R2: radices of the three passes (each divides E).
+template <int N> struct Plan;
+template <> struct Plan<64>   { static constexpr int E = 4,  R0 = 4,  R1 = 4,  R2 = 4; };
+template <> struct Plan<128>  { static constexpr int E = 8,  R0 = 8,  R1 = 8,  R2 = 2; };
+template <> struct Plan<256>  { static constexpr int E = 8,  R0 = 8,  R1 = 8,  R2 = 4; };
+template <> struct Plan<512>  { static constexpr int E = 8,  R0 = 8,  R1 = 8,  R2 = 8; };
+template <> struct Plan<1024> { static constexpr int E = 16, R0 = 16, R1 = 8,  R2 = 8; };
+template <> struct Plan<2048> { static constexpr int E = 32, R0 = 16, R1 = 16, R2 = 8; };
+template <> struct Plan<4096> { static constexpr int E = 64, R0 = 16, R1 = 16, R2 = 16; };
+
+// LDS slot padding: one float2 every 16 breaks the power-of-two strides of the
+// Stockham scatter across banks.
+__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
+template <int N> constexpr int lds_slot() { return N + N / 16; }
+
+// Orders this lane's LDS reads before later LDS writes of other lanes of the
+// same wavefront and vice versa (all lanes of a transform live in one wave).
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// One Stockham pass.  FIRST: input from registers (natural layout).  LAST:
+// output to registers (natural layout).  Otherwise through the LDS slot.
+// tw is the forward twiddle table W_N^m = exp(-2 pi i m/N), m < N.
+template <int N, int R, int NS, int DIR, bool FIRST, bool LAST>
+__device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const float2* __restrict__ tw, int t) {
+  constexpr int E = Plan<N>::E;
+  constexpr int T = N / E;
+  constexpr int NB = E / R;
+  float2 a[NB][R];
+  if constexpr (FIRST) {
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int k = 0; k < R; k++) a[b][k] = v[b + k * NB];
+  } else {
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int k = 0; k < R; k++) a[b][k] = slot[lpad(t + T * b + k * (N / R))];
+  }
+  if constexpr (NS > 1) {
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int jm = (t + T * b) % NS;
+#pragma unroll
+      for (int k = 1; k < R; k++) {
+        float2 w = tw[k * jm * (N / (NS * R))];
+        a[b][k] = DIR < 0 ? cmul(a[b][k], w) : cmulc(a[b][k], w);
+      }
+    }
+  }
+#pragma unroll
+  for (int b = 0; b < NB; b++) dft<R, DIR>(a[b]);
+  if constexpr (LAST) {
+#pragma unroll
+    for (int b = 0; b < NB; b++)
+#pragma unroll
+      for (int k = 0; k < R; k++) v[b + k * NB] = a[b][k];
+  } else {
+    wave_lds_sync();  // all earlier reads of the slot precede these writes
+#pragma unroll
+    for (int b = 0; b < NB; b++) {
+      const int j = t + T * b;
+      const int base = (j / NS) * NS * R + (j % NS);
+#pragma unroll
+      for (int k = 0; k < R; k++) slot[lpad(base + k * NS)] = a[b][k];
+    }
+    wave_lds_sync();
+  }
+}
+
+// Full length-N transform of the lane-distributed vector v (natural layout in
+// and out).  slot: this transform's LDS region of lds_slot<N>() float2.
+template <int N, int DIR>
+__device__ __forceinline__ void fft_wave(float2* v, float2* slot, const float2* __restrict__ tw, int t) {
+  using P = Plan<N>;
+  constexpr int S1 = P::R0;
+  constexpr int S2 = P::R0 * P::R1;
+  stockham_pass<N, P::R0, 1, DIR, true, false>(v, slot, tw, t);
+  stockham_pass<N, P::R1, S1, DIR, false, false>(v, slot, tw, t);
+  stockham_pass<N, P::R2, S2, DIR, false, true>(v, slot, tw, t);
+}
+
+}  // namespace qs

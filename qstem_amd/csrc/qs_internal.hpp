@@ -1,0 +1,74 @@
+// qs_internal.hpp -- shared declarations between the C-ABI layer and the kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+
+namespace qs {
+
+constexpr int kMaxDet = 16;
+
+// Arguments of both multislice sweeps (passed by value, lives in the kernarg segment).
+struct SweepArgs {
+  int nx, ny;          // wave grid (powers of two)
+  int nprobe;          // probes in this launch
+  int probe0;          // global index of the first probe of this launch
+  int kc, kcp;         // band columns stored per row, and padded row length
+  int band_a, band_b;  // band = iy in [0, band_a] U [ny-band_b, ny-1]
+  int ntiles;          // column tiles per probe in sweep B
+  int potNy;           // row pitch of the transmission slice
+  const float2* probe;     // shared real-space probe [nx][ny] (sweep A mode 0)
+  const float2* trans;     // transmission slice [potNx][potNy]
+  const int2* ipos;        // window offset per global probe
+  float2* state;           // [nprobe_total][nx][kcp]
+  float2* out_wave;        // exit waves [nprobe_total][nx][ny] (sweep A mode 2)
+  const float2* twx;       // forward twiddles W_nx^m
+  const float2* twy;       // forward twiddles W_ny^m
+  const float2* px;        // propagator x factor (includes 1/(nx ny))
+  const float2* py;        // propagator y factor
+  const float* kx2;        // single-precision k^2 tables as propagate_slow builds them
+  const float* ky2;
+  float k2max;
+  int rec_bin;             // >= 0: record detector sums on this slice
+  int ndet;
+  int det_sx[kMaxDet], det_sy[kMaxDet];
+  float det_k2in[kMaxDet], det_k2out[kMaxDet];
+  float* partial;          // [nprobe][ntiles][kMaxDet+1]
+  float* diff;             // nullable: fftshifted diffraction intensities [nprobe_total][nx][ny]
+  float diff_scale;
+};
+
+struct ProbeArgs {
+  int nx, ny;
+  double ax, by, rx, ry, rx2, ry2;
+  double wavlen, k2max, pixel, scale;
+  double df_eff, astigMag, astigAngle;
+  double a33, a31, a44, a42, a55, a53, a51, a66, a64, a62;
+  double phi33, phi31, phi44, phi42, phi55, phi53, phi51, phi66, phi64, phi62;
+  double Cs, C5;
+  int on3, on4, on5, on6;
+  double sx, sy;  // probe centre terms (ax - nx/2*resX, by - ny/2*resY)
+  int ismoth, gaussFlag;
+  double gaussScale, aAIS, edge, resX, resY;
+};
+
+bool supported_size(int n);
+int col_tile(int nx);
+
+void launch_sweep_rows(const SweepArgs& a, int mode, hipStream_t st);
+void launch_sweep_cols(const SweepArgs& a, bool rec, bool diff, hipStream_t st);
+void launch_reduce_partials(const float* partial, int nprobe, int ntiles, int ndet, float escale, float* det_bin,
+                            int det_stride, float* total, int probe0, hipStream_t st);
+void launch_rows_fft(float2* d, long nrows, int L, int dir, const float2* tw, float scale, hipStream_t st);
+void launch_rows_bluestein(float2* d, long nrows, int n, int M, const float2* chirp, const float2* vhat,
+                           const float2* tw, float scale, hipStream_t st);
+void launch_transpose(const float2* src, float2* dst, int batch, int na, int nb, hipStream_t st);
+void launch_phase_grating(const float* V, float2* trans, size_t n, double scale, hipStream_t st);
+void launch_bandlimit_T(float2* d, int nslice, int nky, int nkx, const float* kx2, const float* ky2, float k2max,
+                        float scale, hipStream_t st);
+void launch_probe_kspace(const ProbeArgs& a, float2* out, hipStream_t st);
+int probe_window_blocks(int n);
+void launch_probe_window(const ProbeArgs& a, float2* psi, double* partial, hipStream_t st);
+void launch_fill_zero(float2* p, size_t n, hipStream_t st);
+hipError_t set_col_lds_limits();
+
+}  // namespace qs

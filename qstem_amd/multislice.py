@@ -1,0 +1,232 @@
+"""Python mirror of QSTEM's STEM path over the C ABI.
+
+`Multislice` plays the role of the reference's MULS + WAVEFUNC pair
+(libs/data_containers.h:29-308): it owns the transmission functions, the
+probe and the wave state of a batch of scan positions on the GPU.  The
+method names follow the reference routines they replace:
+
+    set_potential  -> initSTEMSlices   (stem3/stemlib.cpp:2199-2357)
+    make_probe     -> probe            (stem3/stemlib.cpp:1952-2191)
+    run_stem       -> runMulsSTEM over the scan loop of doSTEM
+                      (stem3/stemlib.cpp:2375-2534, stem3/stem3.cpp:2820-2946)
+
+Errors raise QstemError with the library's message instead of the
+reference's printf+exit(0).
+"""
+import ctypes
+from dataclasses import dataclass, field
+from typing import List, Optional, Sequence
+
+import numpy as np
+
+from . import _lib
+
+MILLISEC_PICOAMP = 1e-3 * (1e-12 / 1.6021773e-19)  # stemtypes_fftw3.h:47-49
+
+
+class QstemError(RuntimeError):
+    pass
+
+
+@dataclass
+class Detector:
+    """One annular detector line `detector: rin rout name shiftX shiftY` (stem3.cpp:1323-1347)."""
+    rInside: float
+    rOutside: float
+    name: str = "detector"
+    shiftX: float = 0.0
+    shiftY: float = 0.0
+
+
+@dataclass
+class StemConfig:
+    """The MULS fields the STEM path reads, in MULS units (Angstrom, mrad, kV, rad)."""
+    nx: int
+    ny: int
+    resolutionX: float
+    resolutionY: float
+    v0: float
+    slices: int
+    sliceThickness: float
+    potNx: int
+    potNy: int
+    by: float = 0.0
+    bandlimittrans: int = 1
+    Cs: float = 0.0
+    C5: float = 0.0
+    df0: float = 0.0
+    astigMag: float = 0.0
+    astigAngle: float = 0.0
+    aberrations: dict = field(default_factory=dict)  # a33.., phi33.. in MULS units
+    alpha: float = 15.0
+    aAIS: float = 0.0
+    Cc: float = 0.0
+    dE_E: float = 0.0
+    ismoth: int = 0
+    gaussFlag: int = 0
+    gaussScale: float = 0.05
+    detectors: List[Detector] = field(default_factory=list)
+    electronScale: float = MILLISEC_PICOAMP  # 1 pA * 1 ms, the reference default
+    compat: int = 0
+
+    def to_c(self) -> _lib.QsParams:
+        p = _lib.QsParams()
+        for name in ("nx", "ny", "slices", "potNx", "potNy", "bandlimittrans", "ismoth", "gaussFlag", "compat"):
+            setattr(p, name, int(getattr(self, name)))
+        for name in ("resolutionX", "resolutionY", "v0", "sliceThickness", "by", "Cs", "C5", "df0", "astigMag",
+                     "astigAngle", "alpha", "aAIS", "Cc", "dE_E", "gaussScale", "electronScale"):
+            setattr(p, name, float(getattr(self, name)))
+        p.potSizeX = float(np.float32(self.potNx) * np.float32(self.resolutionX))
+        p.potSizeY = float(np.float32(self.potNy) * np.float32(self.resolutionY))
+        for k, v in self.aberrations.items():
+            setattr(p, k, float(v))
+        if len(self.detectors) > _lib.QS_MAX_DET:
+            raise QstemError("at most %d detectors" % _lib.QS_MAX_DET)
+        p.ndet = len(self.detectors)
+        for i, d in enumerate(self.detectors):
+            p.det_rin[i] = d.rInside
+            p.det_rout[i] = d.rOutside
+            p.det_shiftx[i] = d.shiftX
+            p.det_shifty[i] = d.shiftY
+        return p
+
+
+def _fptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+
+
+def _iptr(a: Optional[np.ndarray]):
+    if a is None:
+        return None
+    return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+
+
+def record_bins(slices: int, cellDiv: int = 1, outputInterval: int = 0, totalSliceCount: int = 0):
+    """Bin written by each slice as its final value (collectIntensity, stemlib.cpp:2588-2607)."""
+    lib = _lib.load()
+    out = np.empty(slices, dtype=np.int32)
+    nb = lib.qs_record_bins(slices, cellDiv, outputInterval if outputInterval > 0 else slices,
+                            totalSliceCount, _iptr(out))
+    if nb < 0:
+        raise QstemError("bad bin arguments")
+    return out, nb
+
+
+def scan_offsets(scanXStart, scanXStop, scanXN, scanYStart, scanYStop, scanYN, resX, resY, potNx, potNy, nx, ny):
+    """Integer window offsets of all scan positions, in the driver's order i = ix*scanYN + iy
+    (stem3.cpp:2825-2871, float arithmetic as the reference)."""
+    f = np.float32
+    ix = np.repeat(np.arange(scanXN), scanYN).astype(np.float32)
+    iy = np.tile(np.arange(scanYN), scanXN).astype(np.float32)
+    px = (ix * (f(scanXStop) - f(scanXStart)) / (f(scanXN) * f(resX))).astype(np.int32)
+    py = (iy * (f(scanYStop) - f(scanYStart)) / (f(scanYN) * f(resY))).astype(np.int32)
+    px = np.minimum(px, potNx - nx)
+    py = np.minimum(py, potNy - ny)
+    return np.stack([px, py], axis=1).astype(np.int32)
+
+
+class Multislice:
+    """GPU multislice context (one HIP stream)."""
+
+    def __init__(self, cfg: StemConfig):
+        self.cfg = cfg
+        self._lib = _lib.load()
+        self._p = cfg.to_c()
+        h = ctypes.c_void_p()
+        rc = self._lib.qs_create(ctypes.byref(self._p), ctypes.byref(h))
+        if rc != _lib.QS_OK:
+            raise QstemError("qs_create failed (%d): %s" % (rc, self._lib.qs_last_error(None).decode()))
+        self._h = h
+        self._nslice = 0
+
+    def _check(self, rc, what):
+        if rc != _lib.QS_OK:
+            raise QstemError("%s failed (%d): %s" % (what, rc, self._lib.qs_last_error(self._h).decode()))
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._lib.qs_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    # -- potential / transmission ------------------------------------------------
+    def set_potential(self, V: np.ndarray):
+        V = np.ascontiguousarray(V, dtype=np.float32)
+        assert V.ndim == 3 and V.shape[1:] == (self.cfg.potNx, self.cfg.potNy), V.shape
+        self._check(self._lib.qs_set_potential(self._h, _fptr(V), V.shape[0]), "qs_set_potential")
+        self._nslice = V.shape[0]
+
+    def set_trans(self, t: np.ndarray):
+        t = np.ascontiguousarray(t, dtype=np.complex64)
+        assert t.ndim == 3 and t.shape[1:] == (self.cfg.potNx, self.cfg.potNy), t.shape
+        self._check(self._lib.qs_set_trans(self._h, _fptr(t.view(np.float32)), t.shape[0]), "qs_set_trans")
+        self._nslice = t.shape[0]
+
+    def get_trans(self) -> np.ndarray:
+        out = np.empty((self._nslice, self.cfg.potNx, self.cfg.potNy), dtype=np.complex64)
+        self._check(self._lib.qs_get_trans(self._h, _fptr(out.view(np.float32))), "qs_get_trans")
+        return out
+
+    # -- probe ---------------------------------------------------------------------
+    def make_probe(self) -> np.ndarray:
+        out = np.empty((self.cfg.nx, self.cfg.ny), dtype=np.complex64)
+        self._check(self._lib.qs_make_probe(self._h, _fptr(out.view(np.float32))), "qs_make_probe")
+        return out
+
+    # -- multislice ------------------------------------------------------------------
+    def set_batch(self, batch: int):
+        self._check(self._lib.qs_set_batch(self._h, int(batch)), "qs_set_batch")
+
+    @property
+    def last_run_ms(self) -> float:
+        return float(self._lib.qs_last_run_ms(self._h))
+
+    def run_stem(self, ipos: np.ndarray, nslice: Optional[int] = None, start: int = 0,
+                 bins: Optional[Sequence[int]] = None, nbins: int = 0, det: Optional[np.ndarray] = None,
+                 want_det: bool = True, want_diff: bool = False, want_exit: bool = False,
+                 want_intensity: bool = False) -> dict:
+        """Advance len(ipos) probes through nslice slices.  Returns a dict with
+        'det' [nbins][ndet][nprobe], 'diff', 'exit', 'intensity' as requested."""
+        ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
+        n = ipos.shape[0]
+        nslice = self._nslice if nslice is None else nslice
+        if bins is None:
+            b, nb = record_bins(nslice)
+            bins, nbins = b, nb
+        bins = np.ascontiguousarray(bins, dtype=np.int32)
+        ndet = len(self.cfg.detectors)
+        out = {}
+        if want_det and ndet > 0:
+            if det is None:
+                det = np.zeros((nbins, ndet, n), dtype=np.float32)
+            det = np.ascontiguousarray(det, dtype=np.float32)
+            out["det"] = det
+        else:
+            det = None
+        diff = np.empty((n, self.cfg.nx, self.cfg.ny), dtype=np.float32) if want_diff else None
+        exitw = np.empty((n, self.cfg.nx, self.cfg.ny), dtype=np.complex64) if want_exit else None
+        inten = np.empty(n, dtype=np.float32) if want_intensity else None
+        rc = self._lib.qs_run_stem(self._h, _iptr(ipos), n, int(start), _iptr(bins), int(nslice), int(nbins),
+                                   _fptr(det), _fptr(diff), _fptr(exitw.view(np.float32) if exitw is not None else None),
+                                   _fptr(inten))
+        self._check(rc, "qs_run_stem")
+        if diff is not None:
+            out["diff"] = diff
+        if exitw is not None:
+            out["exit"] = exitw
+        if inten is not None:
+            out["intensity"] = inten
+        return out

@@ -1,0 +1,129 @@
+"""GPU parity: the HIP path through the C ABI against the FP64 oracle.
+
+Tolerance: relative L2 <= 1e-4 for every output (the FP32 target stated for
+this path); the probe and transmission functions, which involve one or two
+transforms only, are held to 2e-5.
+"""
+import numpy as np
+import pytest
+
+from cases import probe_positions, rel_l2, small_config, smooth_potential
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-4
+
+
+def _ctx(cfg):
+    from qstem_amd import Multislice
+    return Multislice(cfg)
+
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512])
+def test_probe_matches_oracle(oracle, n):
+    cfg = small_config(nx=n, ny=n, pad=16, dx=0.03)
+    with _ctx(cfg) as ms:
+        g = ms.make_probe()
+    o = oracle.probe(cfg)
+    assert abs(np.sum(np.abs(g) ** 2) / (n * n) - 1) < 1e-5
+    assert rel_l2(g, o) < 2e-5
+
+
+@pytest.mark.parametrize("pot", [(160, 160), (128, 128), (200, 136)])
+def test_transmission_matches_oracle(oracle, pot):
+    cfg = small_config(nx=128, ny=128, pad=0)
+    cfg.potNx, cfg.potNy = pot
+    V = smooth_potential(3, pot[0], pot[1], seed=3)
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        t = ms.get_trans()
+    o = oracle.make_trans(cfg, V)
+    assert rel_l2(t, o) < 2e-5
+
+
+def test_transmission_quirk_q1(oracle):
+    cfg = small_config(nx=128, ny=128, pad=32, compat=1)
+    V = smooth_potential(2, cfg.potNx, cfg.potNy, seed=4)
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        t = ms.get_trans()
+    assert rel_l2(t, oracle.make_trans(cfg, V)) < 2e-5
+
+
+@pytest.mark.parametrize("shape", [(128, 128), (256, 256), (64, 128), (512, 512)])
+def test_multislice_matches_oracle(oracle, shape):
+    nx, ny = shape
+    slices = 4 if nx == 512 else 6
+    cfg = small_config(nx=nx, ny=ny, pad=24, slices=slices, dx=0.03 if nx >= 256 else 0.05)
+    V = smooth_potential(slices, cfg.potNx, cfg.potNy, seed=11)
+    ipos = probe_positions(cfg, 3, seed=5)
+    bins, nbins = oracle.record_bins(slices, outputInterval=2)
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        t32 = ms.get_trans()
+        psi0 = ms.make_probe()
+        out = ms.run_stem(ipos, bins=bins, nbins=nbins, want_diff=True, want_exit=True, want_intensity=True)
+    # oracle runs on the same FP32 transmission and probe so the comparison isolates the multislice
+    ex, det, diff = oracle.multislice(cfg, t32.astype(np.complex128), ipos, psi0.astype(np.complex128),
+                                      bins=bins, nbins=nbins, want_diff=True)
+    assert rel_l2(out["exit"], ex) < TOL
+    assert rel_l2(out["diff"], diff) < TOL
+    for b in range(nbins):
+        for d in range(det.shape[1]):
+            assert rel_l2(out["det"][b, d], det[b, d]) < TOL, (b, d)
+    norm = np.sum(np.abs(ex) ** 2, axis=(1, 2)) / (nx * ny)
+    assert np.max(np.abs(out["intensity"] - norm) / norm) < TOL
+
+
+def test_full_chain_against_fp64_oracle(oracle):
+    """Probe, transmission and multislice all from the oracle in FP64 vs all on the GPU in FP32."""
+    cfg = small_config(nx=256, ny=256, pad=40, slices=10, dx=0.03, v0=300.0)
+    V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=21)
+    ipos = probe_positions(cfg, 4, seed=9)
+    bins, nbins = oracle.record_bins(cfg.slices)
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        ms.make_probe()
+        out = ms.run_stem(ipos, bins=bins, nbins=nbins, want_exit=True)
+    ex, det, _ = oracle.multislice(cfg, oracle.make_trans(cfg, V), ipos, oracle.probe(cfg), bins=bins, nbins=nbins)
+    assert rel_l2(out["exit"], ex) < TOL
+    for b in range(nbins):
+        for d in range(det.shape[1]):
+            if np.any(det[b, d] != 0):
+                assert rel_l2(out["det"][b, d], det[b, d]) < TOL, (b, d)
+
+
+def test_batch_sizes_agree():
+    """Per-probe launches (batch 1) and batched launches give the same numbers."""
+    cfg = small_config(nx=128, ny=128, pad=24, slices=5)
+    V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=2)
+    ipos = probe_positions(cfg, 7, seed=1)
+    res = []
+    for batch in (1, 3, 0):
+        with _ctx(cfg) as ms:
+            ms.set_potential(V)
+            ms.make_probe()
+            ms.set_batch(batch)
+            res.append(ms.run_stem(ipos, want_exit=True))
+    for r in res[1:]:
+        # detector sums use LDS float atomics inside a block: order-dependent rounding only
+        assert rel_l2(r["det"], res[0]["det"]) < 1e-6
+        assert np.array_equal(r["exit"], res[0]["exit"])
+
+
+def test_continue_equals_one_run(oracle):
+    """Two calls of 3 slices with start=1 equal one call of 6 slices (slab continuation, cellDiv > 1)."""
+    cfg = small_config(nx=128, ny=128, pad=16, slices=6)
+    V = smooth_potential(6, cfg.potNx, cfg.potNy, seed=8)
+    ipos = probe_positions(cfg, 2, seed=3)
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        ms.make_probe()
+        full = ms.run_stem(ipos, want_exit=True)["exit"]
+    with _ctx(cfg) as ms:
+        ms.set_potential(V[:3])
+        ms.make_probe()
+        ms.run_stem(ipos, nslice=3, want_det=False)
+        ms.set_potential(V[3:])
+        part = ms.run_stem(ipos, nslice=3, start=1, want_exit=True, want_det=False)["exit"]
+    assert rel_l2(part, full) < 1e-6
