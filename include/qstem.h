@@ -116,6 +116,14 @@ int qs_get_batch(const qs_ctx *ctx);
 /* Device time of the last qs_run_stem in milliseconds (HIP events on the context stream). */
 float qs_last_run_ms(const qs_ctx *ctx);
 
+/* Per-launch timing of the row sweep (transmit + row transforms) and the
+ * column sweep (column transforms + propagate + detect): when on, each launch
+ * is bracketed by HIP events on the context stream and the durations are
+ * accumulated until the next qs_set_profiling call.  No reference counterpart
+ * (the reference only prints coarse CPU timers, stem3.cpp:2937-2945). */
+int qs_set_profiling(qs_ctx *ctx, int on);
+int qs_kernel_times(const qs_ctx *ctx, double *ms_rows, double *ms_cols, long *n_rows, long *n_cols);
+
 #ifdef __cplusplus
 }
 #endif

@@ -69,6 +69,9 @@ SIGNATURES = {
     "qs_set_batch": (_I, [_P, _I]),
     "qs_get_batch": (_I, [_P]),
     "qs_last_run_ms": (_F, [_P]),
+    "qs_set_profiling": (_I, [_P, _I]),
+    "qs_kernel_times": (_I, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_long), ctypes.POINTER(ctypes.c_long)]),
 }
 
 _lib = None
@@ -81,6 +84,14 @@ def load():
         return _lib
     if not os.path.exists(LIB_PATH):
         raise ImportError("qstem_amd: %s is missing; build it with `python -m qstem_amd.build`" % LIB_PATH)
+    # PyTorch-ROCm ships its own libamdhip64.so.7 / libhsa-runtime64.so.1.  Two
+    # HSA runtimes in one process cannot share the device, so when torch is
+    # installed it is loaded first and this library binds to the same runtime
+    # (the dynamic loader matches the already-loaded SONAMEs).
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     lib = ctypes.CDLL(LIB_PATH)
     for name, (res, args) in SIGNATURES.items():
         fn = getattr(lib, name)

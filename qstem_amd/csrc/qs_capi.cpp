@@ -113,6 +113,11 @@ struct qs_ctx {
   bool have_probe = false;
   int state_probes = 0;
   bool state_valid = false;
+  // optional per-launch timing of the two sweeps (HIP events on the context stream)
+  bool profiling = false;
+  std::vector<hipEvent_t> evpool;
+  double ms_rows = 0, ms_cols = 0;
+  long n_rows = 0, n_cols = 0;
 };
 
 static int fail(qs_ctx* c, int code, const std::string& m) {
@@ -307,6 +312,7 @@ void qs_destroy(qs_ctx* c) {
                     &c->probe_partial, &c->rx.tw, &c->rx.chirp_f, &c->rx.vhat_f, &c->rx.chirp_b, &c->rx.vhat_b,
                     &c->ry.tw, &c->ry.chirp_f, &c->ry.vhat_f, &c->ry.chirp_b, &c->ry.vhat_b};
   for (DevBuf* b : bufs) b->release();
+  for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
   if (c->stream) hipStreamDestroy(c->stream);
@@ -470,6 +476,23 @@ int qs_get_batch(const qs_ctx* c) { return c ? c->batch : -1; }
 
 float qs_last_run_ms(const qs_ctx* c) { return c ? c->last_ms : 0.f; }
 
+int qs_set_profiling(qs_ctx* c, int on) {
+  if (!c) return QS_ERR_ARG;
+  c->profiling = on != 0;
+  c->ms_rows = c->ms_cols = 0;
+  c->n_rows = c->n_cols = 0;
+  return QS_OK;
+}
+
+int qs_kernel_times(const qs_ctx* c, double* ms_rows, double* ms_cols, long* n_rows, long* n_cols) {
+  if (!c || !ms_rows || !ms_cols || !n_rows || !n_cols) return QS_ERR_ARG;
+  *ms_rows = c->ms_rows;
+  *ms_cols = c->ms_cols;
+  *n_rows = c->n_rows;
+  *n_cols = c->n_cols;
+  return QS_OK;
+}
+
 int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bin_of_slice, int nslice, int nbins,
                 float* det, float* diff, float* exit_wave, float* int_intensity) {
   if (!c) return QS_ERR_ARG;
@@ -541,6 +564,20 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
   a.partial = c->partial.as<float>();
   a.diff_scale = (float)pow((double)nx * ny, 1.5);
   const size_t slice_elems = (size_t)p.potNx * p.potNy;
+  // profiling: event pairs around every sweep launch, accumulated after the run
+  std::vector<std::pair<int, int>> marks;  // (kind 0=rows 1=cols, first event index)
+  size_t nev = 0;
+  auto mark = [&](int kind, bool begin) -> hipError_t {
+    if (!c->profiling) return hipSuccess;
+    if (nev >= c->evpool.size()) {
+      hipEvent_t e;
+      hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      c->evpool.push_back(e);
+    }
+    if (begin) marks.push_back({kind, (int)nev});
+    return hipEventRecord(c->evpool[nev++], c->stream);
+  };
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   for (int p0 = 0; p0 < nprobe; p0 += B) {
     const int nb = (nprobe - p0) < B ? (nprobe - p0) : B;
@@ -549,12 +586,16 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
     for (int s = 0; s < nslice; s++) {
       a.trans = c->trans.as<float2>() + (size_t)s * slice_elems;
       const bool last = (s == nslice - 1);
+      HIPCHK(c, mark(0, true));
       qs::launch_sweep_rows(a, (s == 0 && start == 0) ? 0 : 1, c->stream);
+      HIPCHK(c, mark(0, false));
       const int bin = (want_det && bin_of_slice) ? bin_of_slice[s] : -1;
       const bool tot = last && int_intensity;
       a.rec_bin = (bin >= 0 || tot) ? (bin >= 0 ? bin : 0) : -1;
       a.diff = (last && diff) ? c->diff.as<float>() : nullptr;
+      HIPCHK(c, mark(1, true));
       qs::launch_sweep_cols(a, a.rec_bin >= 0, a.diff != nullptr, c->stream);
+      HIPCHK(c, mark(1, false));
       if (bin >= 0 || tot) {
         float* dbin = bin >= 0 ? c->det.as<float>() + (size_t)bin * p.ndet * nprobe : nullptr;
         qs::launch_reduce_partials(c->partial.as<float>(), nb, c->ntiles, bin >= 0 ? p.ndet : 0, p.electronScale, dbin,
@@ -577,6 +618,12 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
     HIPCHK(c, hipMemcpyAsync(int_intensity, c->total.p, sizeof(float) * nprobe, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  for (auto& m : marks) {
+    float ms = 0.f;
+    HIPCHK(c, hipEventElapsedTime(&ms, c->evpool[m.second], c->evpool[m.second + 1]));
+    if (m.first == 0) { c->ms_rows += ms; c->n_rows++; }
+    else { c->ms_cols += ms; c->n_cols++; }
+  }
   c->state_valid = true;
   c->state_probes = nprobe;
   return QS_OK;

@@ -144,11 +144,45 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Per-lane twiddles of the two twiddled passes.  For small plans they are
+// loaded once into registers (init) and reused by every transform the lane
+// performs; larger plans read the table on every pass.
+template <int N>
+struct Twiddles {
+  using P = Plan<N>;
+  static constexpr int T = N / P::E;
+  static constexpr int NS1 = P::R0, NS2 = P::R0 * P::R1;
+  static constexpr int NB1 = P::E / P::R1, NB2 = P::E / P::R2;
+  static constexpr int C1 = NB1 * (P::R1 - 1), C2 = NB2 * (P::R2 - 1);
+  static constexpr bool REG = (C1 + C2) <= 16;
+  float2 w1[REG ? C1 : 1], w2[REG ? C2 : 1];
+  const float2* tab;
+  __device__ __forceinline__ void init(const float2* __restrict__ tw, int t) {
+    tab = tw;
+    if constexpr (REG) {
+#pragma unroll
+      for (int b = 0; b < NB1; b++) {
+        const int jm = (t + T * b) % NS1;
+#pragma unroll
+        for (int k = 1; k < P::R1; k++) w1[b * (P::R1 - 1) + k - 1] = tw[k * jm * (N / (NS1 * P::R1))];
+      }
+#pragma unroll
+      for (int b = 0; b < NB2; b++) {
+        const int jm = (t + T * b) % NS2;
+#pragma unroll
+        for (int k = 1; k < P::R2; k++) w2[b * (P::R2 - 1) + k - 1] = tw[k * jm * (N / (NS2 * P::R2))];
+      }
+    }
+  }
+};
+
 // One Stockham pass.  FIRST: input from registers (natural layout).  LAST:
 // output to registers (natural layout).  Otherwise through the LDS slot.
-// tw is the forward twiddle table W_N^m = exp(-2 pi i m/N), m < N.
+// wr: this pass's register twiddles (or nullptr: read tab, the forward table
+// W_N^m = exp(-2 pi i m/N), m < N).
 template <int N, int R, int NS, int DIR, bool FIRST, bool LAST>
-__device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const float2* __restrict__ tw, int t) {
+__device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const float2* wr,
+                                              const float2* __restrict__ tab, int t) {
   constexpr int E = Plan<N>::E;
   constexpr int T = N / E;
   constexpr int NB = E / R;
@@ -170,7 +204,7 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
       const int jm = (t + T * b) % NS;
 #pragma unroll
       for (int k = 1; k < R; k++) {
-        float2 w = tw[k * jm * (N / (NS * R))];
+        const float2 w = wr ? wr[b * (R - 1) + k - 1] : tab[k * jm * (N / (NS * R))];
         a[b][k] = DIR < 0 ? cmul(a[b][k], w) : cmulc(a[b][k], w);
       }
     }
@@ -198,13 +232,14 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
 // Full length-N transform of the lane-distributed vector v (natural layout in
 // and out).  slot: this transform's LDS region of lds_slot<N>() float2.
 template <int N, int DIR>
-__device__ __forceinline__ void fft_wave(float2* v, float2* slot, const float2* __restrict__ tw, int t) {
+__device__ __forceinline__ void fft_wave(float2* v, float2* slot, const Twiddles<N>& tw, int t) {
   using P = Plan<N>;
   constexpr int S1 = P::R0;
   constexpr int S2 = P::R0 * P::R1;
-  stockham_pass<N, P::R0, 1, DIR, true, false>(v, slot, tw, t);
-  stockham_pass<N, P::R1, S1, DIR, false, false>(v, slot, tw, t);
-  stockham_pass<N, P::R2, S2, DIR, false, true>(v, slot, tw, t);
+  constexpr bool REG = Twiddles<N>::REG;
+  stockham_pass<N, P::R0, 1, DIR, true, false>(v, slot, nullptr, tw.tab, t);
+  stockham_pass<N, P::R1, S1, DIR, false, false>(v, slot, REG ? tw.w1 : nullptr, tw.tab, t);
+  stockham_pass<N, P::R2, S2, DIR, false, true>(v, slot, REG ? tw.w2 : nullptr, tw.tab, t);
 }
 
 }  // namespace qs

@@ -44,6 +44,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_rows(SweepArgs a) {
   float2* slot = lds + slot_id * lds_slot<NY>();
   float2 v[E];
   const int pg = a.probe0 + p;  // global probe index (ipos, state)
+  Twiddles<NY> tw;
+  tw.init(a.twy, t);
   if constexpr (MODE == 0) {
     const float2* src = a.probe + (size_t)x * NY;
 #pragma unroll
@@ -56,7 +58,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_rows(SweepArgs a) {
       const int c = iy <= a.band_a ? iy : (iy >= NY - a.band_b ? iy - (NY - a.band_b) + a.band_a + 1 : -1);
       v[m] = c >= 0 ? src[c] : make_float2(0.f, 0.f);
     }
-    fft_wave<NY, +1>(v, slot, a.twy, t);
+    fft_wave<NY, +1>(v, slot, tw, t);
   }
   if constexpr (MODE == 2) {
     float2* dst = a.out_wave + ((size_t)pg * a.nx + x) * NY;
@@ -68,7 +70,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_rows(SweepArgs a) {
     const float2* tr = a.trans + (size_t)(x + ip.x) * a.potNy + ip.y;
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = cmul(v[m], tr[t + T * m]);
-    fft_wave<NY, -1>(v, slot, a.twy, t);
+    fft_wave<NY, -1>(v, slot, tw, t);
     float2* dst = a.state + ((size_t)pg * a.nx + x) * a.kcp;
 #pragma unroll
     for (int m = 0; m < E; m++) {
@@ -82,8 +84,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_rows(SweepArgs a) {
 // ============================================================================
 // Sweep B: ky columns of the band, TC columns per block staged through LDS.
 template <int NX> struct ColCfg {
-  static constexpr int TC = NX <= 512 ? 16 : (NX == 1024 ? 8 : (NX == 2048 ? 4 : 2));
-  static constexpr int STRIDE = ((lds_slot<NX>() + 31) / 32) * 32 + 4;  // float2 per LDS column
+  static constexpr int TC = NX <= 512 ? 8 : (NX == 1024 ? 8 : (NX == 2048 ? 4 : 2));
+  static constexpr int STRIDE = ((lds_slot<NX>() + 31) / 32) * 32 + 2;  // float2 per LDS column
 };
 
 template <int NX, bool REC, bool DIFF>
@@ -115,6 +117,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
 
   const int slot_id = threadIdx.x / T;
   const int t = threadIdx.x % T;
+  Twiddles<NX> tw;
+  tw.init(a.twx, t);
   for (int r = 0; r < ROUNDS; r++) {
     const int cl = r * G + slot_id;
     if (cl >= TC) break;  // only when G does not divide TC (uniform per transform)
@@ -124,7 +128,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = col[lpad(t + T * m)];
-    fft_wave<NX, -1>(v, col, a.twx, t);
+    fft_wave<NX, -1>(v, col, tw, t);
     // propagate: P = px[ix]*py[iy] (1/(nx*ny) folded into px), zero outside k2max
     if (iy >= 0) {
       const float ky2 = a.ky2[iy];
@@ -170,7 +174,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
 #pragma unroll
       for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
     }
-    fft_wave<NX, +1>(v, col, a.twx, t);
+    fft_wave<NX, +1>(v, col, tw, t);
     wave_lds_sync();
 #pragma unroll
     for (int m = 0; m < E; m++) col[lpad(t + T * m)] = v[m];
@@ -201,7 +205,9 @@ __global__ __launch_bounds__(kBlock) void k_rows_fft(float2* __restrict__ data, 
   float2 v[E];
 #pragma unroll
   for (int m = 0; m < E; m++) v[m] = r[t + T * m];
-  fft_wave<L, DIR>(v, lds + slot_id * lds_slot<L>(), tw, t);
+  Twiddles<L> twc;
+  twc.init(tw, t);
+  fft_wave<L, DIR>(v, lds + slot_id * lds_slot<L>(), twc, t);
 #pragma unroll
   for (int m = 0; m < E; m++) r[t + T * m] = cscale(v[m], scale);
 }
@@ -228,10 +234,12 @@ __global__ __launch_bounds__(kBlock) void k_rows_bluestein(float2* __restrict__ 
     const int k = t + T * m;
     v[m] = k < n ? cmul(r[k], chirp[k]) : make_float2(0.f, 0.f);
   }
-  fft_wave<M, -1>(v, slot, tw, t);
+  Twiddles<M> twc;
+  twc.init(tw, t);
+  fft_wave<M, -1>(v, slot, twc, t);
 #pragma unroll
   for (int m = 0; m < E; m++) v[m] = cmul(v[m], vhat[t + T * m]);
-  fft_wave<M, +1>(v, slot, tw, t);
+  fft_wave<M, +1>(v, slot, twc, t);
   const float s = scale / (float)M;
 #pragma unroll
   for (int m = 0; m < E; m++) {

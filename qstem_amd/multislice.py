@@ -190,6 +190,17 @@ class Multislice:
     def set_batch(self, batch: int):
         self._check(self._lib.qs_set_batch(self._h, int(batch)), "qs_set_batch")
 
+    def set_profiling(self, on: bool = True):
+        self._check(self._lib.qs_set_profiling(self._h, 1 if on else 0), "qs_set_profiling")
+
+    def kernel_times(self):
+        """(ms_rows, ms_cols, n_rows, n_cols): summed launch durations of the two sweeps."""
+        a, b = ctypes.c_double(), ctypes.c_double()
+        na, nb = ctypes.c_long(), ctypes.c_long()
+        self._check(self._lib.qs_kernel_times(self._h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(na),
+                                              ctypes.byref(nb)), "qs_kernel_times")
+        return a.value, b.value, na.value, nb.value
+
     @property
     def last_run_ms(self) -> float:
         return float(self._lib.qs_last_run_ms(self._h))
