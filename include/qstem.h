@@ -86,6 +86,38 @@ int qs_set_potential(qs_ctx *ctx, const float *V, int nslice);
 int qs_set_trans(qs_ctx *ctx, const float *trans, int nslice);
 int qs_get_trans(qs_ctx *ctx, float *trans_out);
 
+/* One atom as make3DSlices consumes it (libs/stemtypes_fftw3.h:69-77): Cartesian
+ * position in Angstrom, Debye-Waller factor B (A^2), occupancy, charge, Z (0 = vacancy). */
+typedef struct qs_atom {
+  float x, y, z, dw, occ, q;
+  int Znum;
+} qs_atom;
+
+/* The make3DSlices inputs beyond qs_params (stem3.cpp:827-1394). */
+typedef struct qs_pot_params {
+  float atomRadius;             /* "atom radius:" (default 5 A) */
+  float potOffsetX, potOffsetY; /* scan_x_start - nx/2*resolutionX, likewise y (stem3.cpp:1436-1437) */
+  float czOffset;               /* "zOffset:" */
+  int centerSlices;             /* "center slices:" */
+  int periodicXY;               /* "periodicXY:" */
+  int tds;                      /* frozen phonons: B := 0 in the LUT (stemlib.cpp:914) */
+  int compat_q6;                /* cache the LUT by Z only, first B in z order wins (stemlib.cpp:1422) */
+  int lut_dr_fix;               /* 0: sample the LUT with dr = resolutionX/2 as make3DSlices does
+                                 * (stemlib.cpp:701) although its radial spacing is resolutionX
+                                 * (dkx, stemlib.cpp:1383), i.e. atoms radially compressed 2x (Q10);
+                                 * 1: sample at the LUT's own spacing */
+} qs_pot_params;
+
+/* Rez et al. electron scattering factors: 93 rows x 123 doubles, row 0 = s grid,
+ * row Z = f_e(s) (stem3/scatfactsRez.h:22-25).  Required before qs_build_potential. */
+int qs_set_scattering_table(qs_ctx *ctx, const double *table);
+
+/* make3DSlices + getAtomPotential3D + initSTEMSlices: builds the projected potential of
+ * the given atoms into qs_params.slices slices on the device, then the transmission
+ * functions.  Neutral atoms and the 3D potential only (QS_ERR_UNSUPPORTED otherwise).
+ * V_out (nullable) receives V [slices][potNx][potNy]. */
+int qs_build_potential(qs_ctx *ctx, const qs_atom *atoms, int natom, const qs_pot_params *pp, float *V_out);
+
 /* Forms the probe at the window centre (the only position the driver uses,
  * stem3.cpp:2838); keeps it on the device; copies it to out (nx*ny complex64) when non-NULL. */
 int qs_make_probe(qs_ctx *ctx, float *out);

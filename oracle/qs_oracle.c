@@ -559,7 +559,7 @@ void qo_make_slices(const qo_params *p, const qo_lut_geom *g, const float *const
     for (int iax = iax0; iax <= iax1; iax++) {
       double x2 = (float)(iax * dx - atomX);
       x2 *= x2;
-      const int ixw = periodic ? ((iax + 2 * nx) % nx) : iax;
+      const int ixw = periodic ? (((iax % nx) + nx) % nx) : iax;
       for (int iay = iay0; iay <= iay1; iay++) {
         double y2 = (float)(iay * dy - atomY);
         y2 *= y2;
@@ -568,7 +568,7 @@ void qo_make_slices(const qo_params *p, const qo_lut_geom *g, const float *const
         int ir = (int)floor(ddr);
         if (ir >= Nr - 1) continue;
         ddr -= ir;
-        const int iyw = periodic ? ((iay + 2 * ny) % ny) : iay;
+        const int iyw = periodic ? (((iay % ny) + ny) % ny) : iay;
         int iOffsZ = iOffsZ0;
         for (int iaz = iaz0; iaz <= iaz1; iaz++) {
           double pv = 0;

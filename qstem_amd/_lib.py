@@ -52,6 +52,15 @@ class QsParams(ctypes.Structure):
     ]
 
 
+class QsAtom(ctypes.Structure):
+    _fields_ = [("x", _F), ("y", _F), ("z", _F), ("dw", _F), ("occ", _F), ("q", _F), ("Znum", _I)]
+
+
+class QsPotParams(ctypes.Structure):
+    _fields_ = [("atomRadius", _F), ("potOffsetX", _F), ("potOffsetY", _F), ("czOffset", _F),
+                ("centerSlices", _I), ("periodicXY", _I), ("tds", _I), ("compat_q6", _I), ("lut_dr_fix", _I)]
+
+
 # Symbols declared in include/qstem.h, with their ctypes signatures.
 _P = ctypes.c_void_p
 _FP = ctypes.POINTER(ctypes.c_float)
@@ -64,6 +73,8 @@ SIGNATURES = {
     "qs_set_trans": (_I, [_P, _FP, _I]),
     "qs_get_trans": (_I, [_P, _FP]),
     "qs_make_probe": (_I, [_P, _FP]),
+    "qs_set_scattering_table": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "qs_build_potential": (_I, [_P, ctypes.POINTER(QsAtom), _I, ctypes.POINTER(QsPotParams), _FP]),
     "qs_record_bins": (_I, [_I, _I, _I, _I, _IP]),
     "qs_run_stem": (_I, [_P, _IP, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
     "qs_set_batch": (_I, [_P, _I]),

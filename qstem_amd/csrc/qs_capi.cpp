@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cmath>
+#include <algorithm>
 #include <complex>
 #include <cstdio>
 #include <cstring>
@@ -89,6 +90,11 @@ struct RowXform {
   DevBuf tw, chirp_f, vhat_f, chirp_b, vhat_b;
 };
 
+struct LutArgsHost {
+  int nx = 0, nz = 0, nzp = 0;
+  double dkx = 0, dkz = 0;
+};
+
 }  // namespace
 
 struct qs_ctx {
@@ -109,6 +115,11 @@ struct qs_ctx {
   DevBuf state, ipos, partial, det, diff, exitw, total;
   DevBuf pkx2, pky2, probe_partial;
   RowXform rx, ry;  // potNx and potNy row transforms
+  // potential builder
+  std::vector<double> scat;  // 93 x 123 Rez table
+  DevBuf atoms, kinds, lutptrs, spl, luttmp, lutscr;
+  std::vector<DevBuf> luts;
+  RowXform lx, lz;  // LUT transform lengths
   int trans_slices = 0;
   bool have_probe = false;
   int state_probes = 0;
@@ -310,8 +321,12 @@ void qs_destroy(qs_ctx* c) {
   DevBuf* bufs[] = {&c->twx, &c->twy, &c->px, &c->py, &c->dkx2, &c->dky2, &c->probe, &c->trans, &c->scratch, &c->V,
                     &c->state, &c->ipos, &c->partial, &c->det, &c->diff, &c->exitw, &c->total, &c->pkx2, &c->pky2,
                     &c->probe_partial, &c->rx.tw, &c->rx.chirp_f, &c->rx.vhat_f, &c->rx.chirp_b, &c->rx.vhat_b,
-                    &c->ry.tw, &c->ry.chirp_f, &c->ry.vhat_f, &c->ry.chirp_b, &c->ry.vhat_b};
+                    &c->ry.tw, &c->ry.chirp_f, &c->ry.vhat_f, &c->ry.chirp_b, &c->ry.vhat_b,
+                    &c->atoms, &c->kinds, &c->lutptrs, &c->spl, &c->luttmp, &c->lutscr,
+                    &c->lx.tw, &c->lx.chirp_f, &c->lx.vhat_f, &c->lx.chirp_b, &c->lx.vhat_b,
+                    &c->lz.tw, &c->lz.chirp_f, &c->lz.vhat_f, &c->lz.chirp_b, &c->lz.vhat_b};
   for (DevBuf* b : bufs) b->release();
+  for (DevBuf& b : c->luts) b.release();
   for (hipEvent_t e : c->evpool) hipEventDestroy(e);
   if (c->ev0) hipEventDestroy(c->ev0);
   if (c->ev1) hipEventDestroy(c->ev1);
@@ -341,13 +356,11 @@ static int bandlimit_trans(qs_ctx* c, int nslice) {
   return QS_OK;
 }
 
-int qs_set_potential(qs_ctx* c, const float* V, int nslice) {
-  if (!c || !V || nslice < 1) return fail(c, QS_ERR_ARG, "qs_set_potential: bad arguments");
+// initSTEMSlices on the device-resident potential c->V.
+static int build_trans(qs_ctx* c, int nslice) {
   const qs_params& p = c->p;
   const size_t n = (size_t)nslice * p.potNx * p.potNy;
-  HIPCHK(c, c->V.ensure(sizeof(float) * n));
   HIPCHK(c, c->trans.ensure(sizeof(float2) * n));
-  HIPCHK(c, hipMemcpyAsync(c->V.p, V, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
   // sigma = gamma*lambda with gamma = 1 + v0/511 in float (stemlib.cpp:2271-2274)
   const float mm0 = 1.0F + p.v0 / 511.0F;
   const double scale = mm0 * wavelength(p.v0);
@@ -361,6 +374,151 @@ int qs_set_potential(qs_ctx* c, const float* V, int nslice) {
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->trans_slices = nslice;
   return QS_OK;
+}
+
+int qs_set_potential(qs_ctx* c, const float* V, int nslice) {
+  if (!c || !V || nslice < 1) return fail(c, QS_ERR_ARG, "qs_set_potential: bad arguments");
+  const qs_params& p = c->p;
+  const size_t n = (size_t)nslice * p.potNx * p.potNy;
+  HIPCHK(c, c->V.ensure(sizeof(float) * n));
+  HIPCHK(c, hipMemcpyAsync(c->V.p, V, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+  return build_trans(c, nslice);
+}
+
+// Akima spline coefficients (the fit of stem3/stemutil.cpp:975-1041), host set-up of 123 nodes.
+static void akima(const double* x, const double* y, double* b, double* cc, double* d, int n) {
+  const double small = 1.0e-25;
+  double m5 = (y[3] - y[2]) / (x[3] - x[2]), m4 = (y[2] - y[1]) / (x[2] - x[1]), m3 = (y[1] - y[0]) / (x[1] - x[0]);
+  double m2 = 2 * m3 - m4, m1 = 2 * m2 - m3;
+  double m54 = fabs(m5 - m4), m43 = fabs(m4 - m3), m32 = fabs(m3 - m2), m21 = fabs(m2 - m1);
+  double t1 = (m43 + m21 > small) ? (m43 * m2 + m21 * m3) / (m43 + m21) : 0.5 * (m2 + m3);
+  for (int i = 0; i < n - 1; i++) {
+    const double t2 = (m54 + m32 > small) ? (m54 * m3 + m32 * m4) / (m54 + m32) : 0.5 * (m3 + m4);
+    const double h = x[i + 1] - x[i];
+    b[i] = t1;
+    cc[i] = (3 * m3 - 2 * t1 - t2) / h;
+    d[i] = (t1 + t2 - 2 * m3) / (h * h);
+    m1 = m2; m2 = m3; m3 = m4; m4 = m5;
+    m5 = (i < n - 4) ? (y[i + 4] - y[i + 3]) / (x[i + 4] - x[i + 3]) : 2 * m4 - m3;
+    m21 = m32; m32 = m43; m43 = m54; m54 = fabs(m5 - m4);
+    t1 = t2;
+  }
+  b[n - 1] = cc[n - 1] = d[n - 1] = 0;
+}
+
+int qs_set_scattering_table(qs_ctx* c, const double* table) {
+  if (!c || !table) return fail(c, QS_ERR_ARG, "qs_set_scattering_table: bad arguments");
+  c->scat.assign(table, table + 93 * 123);
+  return QS_OK;
+}
+
+int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_params* pp, float* V_out) {
+  if (!c || !pp || natom < 0 || (natom > 0 && !atoms)) return fail(c, QS_ERR_ARG, "qs_build_potential: bad arguments");
+  if (c->scat.size() != 93 * 123) return fail(c, QS_ERR_STATE, "qs_build_potential: scattering table not set");
+  const qs_params& p = c->p;
+  for (int i = 0; i < natom; i++) {
+    if (atoms[i].Znum < 0 || atoms[i].Znum > 92) return fail(c, QS_ERR_ARG, "qs_build_potential: Z out of range");
+    if (atoms[i].Znum > 0 && atoms[i].q != 0.f)
+      return fail(c, QS_ERR_UNSUPPORTED, "qs_build_potential: charged atoms (ion offsets) are not supported");
+  }
+  // LUT geometry (stemlib.cpp:1366-1388), float products as the reference
+  const float R = pp->atomRadius, dx = p.resolutionX, dz = p.sliceThickness;
+  LutArgsHost g;
+  g.nx = 2 * 2 * (int)ceil(R / dx);
+  int nzp = (int)floor(2 * dz / dx);
+  if (2.0 * (nzp >> 1) == nzp) nzp += 1;
+  g.nzp = nzp;
+  g.nz = (2 * (int)ceil(R / dz)) * nzp;
+  g.dkx = 0.5 * 2 / (double)(float)(g.nx * dx);
+  g.dkz = nzp / (double)(float)(g.nz * dz);
+  const double smax = 0.5 * g.nx * g.dkx / 2.0;
+  if (g.nx > 2048 || g.nz > 2048) return fail(c, QS_ERR_UNSUPPORTED, "qs_build_potential: LUT grid above 2048");
+  // taper of the shared s grid (stemlib.cpp:1390-1413)
+  std::vector<double> tab = c->scat;
+  double* sgrid = tab.data();
+  sgrid[122] = 1.2 * smax;
+  sgrid[121] = 1.1 * smax;
+  sgrid[120] = smax;
+  if (sgrid[119] > sgrid[120]) {
+    for (int ix = 0; ix < 123 - 10; ix++) {
+      if (sgrid[119 - ix] < sgrid[120] - 0.001 * (ix + 1)) break;
+      sgrid[119 - ix] = sgrid[120] - 0.001 * (ix + 1);
+      for (int r = 1; r < 93; r++) tab[(size_t)r * 123 + 119 - ix] = 0;
+    }
+  }
+  // distinct LUT kinds: (Z, B) keys, or Z only with the B of the lowest atom in z (compat Q6)
+  std::vector<std::pair<int, float>> keys;
+  std::vector<int> kind(natom, 0);
+  std::vector<int> order(natom);
+  for (int i = 0; i < natom; i++) order[i] = i;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return atoms[a].z < atoms[b].z; });
+  for (int oi = 0; oi < natom; oi++) {
+    const int i = order[oi];
+    if (atoms[i].Znum == 0) continue;
+    const float B = pp->tds ? 0.f : atoms[i].dw;
+    int k = -1;
+    for (size_t j = 0; j < keys.size(); j++)
+      if (keys[j].first == atoms[i].Znum && (pp->compat_q6 || keys[j].second == B)) { k = (int)j; break; }
+    if (k < 0) { keys.push_back({atoms[i].Znum, B}); k = (int)keys.size() - 1; }
+    kind[i] = k;
+  }
+  int rc = setup_rowxform(c, c->lx, g.nx);
+  if (rc == QS_OK) rc = setup_rowxform(c, c->lz, g.nz);
+  if (rc != QS_OK) return rc;
+  qs::LutArgs la{};
+  la.nx = g.nx; la.nz = g.nz; la.nzPerSlice = nzp;
+  la.dkx = g.dkx; la.dkz = g.dkz; la.smax2 = smax * smax;
+  la.zPos = -2.0 * M_PI * ((double)dz / nzp * ((nzp - 1) / 2));
+  const int Nr = g.nx / 2, Nz = g.nz / 2;
+  c->luts.resize(keys.size());
+  HIPCHK(c, c->spl.ensure(sizeof(double) * 5 * 123));
+  HIPCHK(c, c->luttmp.ensure(sizeof(float2) * (size_t)g.nx * g.nz));
+  HIPCHK(c, c->lutscr.ensure(sizeof(float2) * (size_t)g.nx * g.nz));
+  std::vector<const float*> lutp(keys.size());
+  std::vector<double> spl(5 * 123);
+  for (size_t k = 0; k < keys.size(); k++) {
+    std::copy(sgrid, sgrid + 123, spl.begin());
+    std::copy(tab.begin() + (size_t)keys[k].first * 123, tab.begin() + (size_t)keys[k].first * 123 + 123,
+              spl.begin() + 123);
+    akima(&spl[0], &spl[123], &spl[246], &spl[369], &spl[492], 123);
+    HIPCHK(c, hipMemcpyAsync(c->spl.p, spl.data(), sizeof(double) * spl.size(), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, c->luts[k].ensure(sizeof(float) * (size_t)Nr * Nz));
+    float2* tmp = c->luttmp.as<float2>();
+    float2* scr = c->lutscr.as<float2>();
+    qs::launch_lut_fk(c->spl.as<double>(), la, (double)keys[k].second, tmp, c->stream);
+    run_rowxform(c, c->lx, tmp, g.nz, +1, 1.0f);
+    qs::launch_transpose(tmp, scr, 1, g.nz, g.nx, c->stream);
+    run_rowxform(c, c->lz, scr, g.nx, +1, 1.0f);
+    qs::launch_transpose(scr, tmp, 1, g.nx, g.nz, c->stream);
+    qs::launch_lut_extract(tmp, la, c->luts[k].as<float>(), c->stream);
+    lutp[k] = c->luts[k].as<float>();
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // spl is reused by the next kind
+  }
+  // atoms, kinds, LUT pointers
+  const size_t nV = (size_t)p.slices * p.potNx * p.potNy;
+  HIPCHK(c, c->V.ensure(sizeof(float) * nV));
+  HIPCHK(c, hipMemsetAsync(c->V.p, 0, sizeof(float) * nV, c->stream));
+  if (natom > 0 && !keys.empty()) {
+    HIPCHK(c, c->atoms.ensure(sizeof(qs_atom) * natom));
+    HIPCHK(c, c->kinds.ensure(sizeof(int) * natom));
+    HIPCHK(c, c->lutptrs.ensure(sizeof(float*) * keys.size()));
+    HIPCHK(c, hipMemcpyAsync(c->atoms.p, atoms, sizeof(qs_atom) * natom, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->kinds.p, kind.data(), sizeof(int) * natom, hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->lutptrs.p, lutp.data(), sizeof(float*) * keys.size(), hipMemcpyHostToDevice, c->stream));
+    qs::ScatterArgs sa{};
+    sa.potNx = p.potNx; sa.potNy = p.potNy; sa.slices = p.slices;
+    sa.dx = p.resolutionX; sa.dy = p.resolutionY; sa.sliceThickness = dz;
+    sa.potOffsetX = pp->potOffsetX; sa.potOffsetY = pp->potOffsetY; sa.czOffset = pp->czOffset;
+    sa.atomRadius = R; sa.centerSlices = pp->centerSlices; sa.periodic = pp->periodicXY;
+    sa.iRadX = (int)ceil(R / p.resolutionX); sa.iRadY = (int)ceil(R / p.resolutionY); sa.iRadZ = (int)ceil(R / dz);
+    sa.Nr = Nr; sa.Nz = Nz; sa.nzSub = nzp;
+    sa.dr = pp->lut_dr_fix ? p.resolutionX : p.resolutionX / 2;
+    qs::launch_scatter(c->atoms.as<qs::QsAtomDev>(), natom, c->lutptrs.as<const float*>(), c->kinds.as<int>(), sa,
+                       c->V.as<float>(), c->stream);
+  }
+  HIPCHK(c, hipGetLastError());
+  if (V_out) HIPCHK(c, hipMemcpyAsync(V_out, c->V.p, sizeof(float) * nV, hipMemcpyDeviceToHost, c->stream));
+  return build_trans(c, p.slices);
 }
 
 int qs_set_trans(qs_ctx* c, const float* trans, int nslice) {

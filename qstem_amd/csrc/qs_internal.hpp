@@ -51,6 +51,34 @@ struct ProbeArgs {
   double gaussScale, aAIS, edge, resX, resY;
 };
 
+// LUT of one element (getAtomPotential3D, stem3/stemlib.cpp:1342-1541).
+struct LutArgs {
+  int nx, nz, nzPerSlice;
+  double dkx, dkz, smax2;  // smax2 already squared
+  double zPos;             // -2 pi dz/nzPerSlice * izOffset
+};
+
+struct QsAtomDev {
+  float x, y, z, dw, occ, q;
+  int Znum;
+};
+
+// Scatter of atoms into slices (make3DSlices, stem3/stemlib.cpp:883-1213).
+struct ScatterArgs {
+  int potNx, potNy, slices;
+  float dx, dy, sliceThickness;
+  float potOffsetX, potOffsetY, czOffset, atomRadius;
+  int centerSlices, periodic;
+  int iRadX, iRadY, iRadZ;
+  int Nr, Nz, nzSub;
+  float dr;  // radial LUT step used for the lookup
+};
+
+void launch_lut_fk(const double* spl, const LutArgs& g, double B, float2* out, hipStream_t st);
+void launch_lut_extract(const float2* tmp, const LutArgs& g, float* lut, hipStream_t st);
+void launch_scatter(const QsAtomDev* atoms, int natom, const float* const* luts, const int* kind,
+                    const ScatterArgs& s, float* V, hipStream_t st);
+
 bool supported_size(int n);
 int col_tile(int nx);
 

@@ -1,0 +1,177 @@
+// qs_potential.hip -- projected-potential slices on the GPU.
+//
+// Restates make3DSlices (stem3/stemlib.cpp:383-1334, fftpotential +
+// potential3D branches, Z_INTERPOLATION 0) and getAtomPotential3D
+// (stem3/stemlib.cpp:1342-1541):
+//   1. per element, f_e(k) from an Akima spline of the Rez table times the
+//      Debye-Waller factor, summed over q_y (k_lut_fk, FP64),
+//   2. a 2D inverse DFT of the (kz, kx) map (Stockham/Bluestein row kernels),
+//   3. z-integration over nzPerSlice sub-samples, clamp >= 0, scale (k_lut_extract),
+//   4. one block per atom adds r-linear / z-nearest LUT samples into the slices
+//      it reaches (k_scatter), with float atomics.
+#include <hip/hip_runtime.h>
+
+#include "qs_internal.hpp"
+
+namespace qs {
+
+constexpr int kNsf = 123;  // nodes of the Rez s-grid (stem3/scatfactsRez.h:23)
+
+__device__ double seval_dev(const double* __restrict__ x, const double* __restrict__ y, const double* __restrict__ b,
+                            const double* __restrict__ c, const double* __restrict__ d, double x0) {
+  // stem3/stemutil.cpp:1069-1092: binary search, end intervals clamp
+  int i;
+  if (x0 <= x[0]) i = 0;
+  else if (x0 >= x[kNsf - 2]) i = kNsf - 2;
+  else {
+    int lo = 0, hi = kNsf;
+    while (hi - lo > 1) {
+      const int k = (lo + hi) / 2;
+      if (x0 < x[k]) hi = k; else lo = k;
+    }
+    i = lo;
+  }
+  const double z = x0 - x[i];
+  return y[i] + (b[i] + (c[i] + d[i] * z) * z) * z;
+}
+
+// f(kx,kz) with the q_y sum (stemlib.cpp:1447-1480); spl = [x, y, b, c, d] each kNsf doubles.
+__global__ void k_lut_fk(const double* __restrict__ spl, LutArgs g, double B, float2* __restrict__ out) {
+  const int n = g.nx * g.nz;
+  const double* sx = spl;
+  const double* sy = spl + kNsf;
+  const double* sb = spl + 2 * kNsf;
+  const double* sc = spl + 3 * kNsf;
+  const double* sd = spl + 4 * kNsf;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    const int iz = i / g.nx, ix = i % g.nx;
+    const double kz = g.dkz * (iz < g.nz / 2 ? iz : iz - g.nz);
+    const double kx = g.dkx * (ix < g.nx / 2 ? ix : ix - g.nx);
+    const double s2 = kx * kx + kz * kz;
+    float2 v = make_float2(0.f, 0.f);
+    if (s2 < g.smax2) {
+      double f = seval_dev(sx, sy, sb, sc, sd, sqrt(s2)) * exp(-s2 * B * 0.25);
+      for (int iy = 1; iy < g.nx; iy++) {
+        double s3 = g.dkx * iy;
+        s3 = s3 * s3 + s2;
+        if (s3 < g.smax2) f += 2.0 * seval_dev(sx, sy, sb, sc, sd, sqrt(s3)) * exp(-s3 * B * 0.25);
+        else break;
+      }
+      f *= g.dkx;
+      const double phase = kz * g.zPos;
+      double sn, cs;
+      sincos(phase, &sn, &cs);
+      v = make_float2((float)(f * cs), (float)(f * sn));
+    }
+    out[i] = v;
+  }
+}
+
+// z-integration and scale (stemlib.cpp:1503-1522).  tmp is the real-space map [nz][nx].
+__global__ void k_lut_extract(const float2* __restrict__ tmp, LutArgs g, float* __restrict__ lut) {
+  const int Nr = g.nx / 2, Nz = g.nz / 2;
+  const int izOff = (g.nzPerSlice - 1) / 2;
+  const double scale = 47.8658 * g.dkx * g.dkz / g.nz;
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < Nr * Nz; i += gridDim.x * blockDim.x) {
+    const int iz = i / Nr, ix = i % Nr;
+    double zs = 0;
+    for (int iiz = -izOff; iiz <= izOff; iiz++)
+      if (iz + izOff + iiz < Nz) zs += tmp[ix + (size_t)(iz + izOff + iiz) * g.nx].x;
+    if (zs < 0) zs = 0;
+    lut[ix + (size_t)iz * Nr] = (float)(scale * zs);
+  }
+}
+
+// One block per atom (stemlib.cpp:883-1213).  Float arithmetic follows the
+// reference statement by statement where it decides sample indices.
+__global__ __launch_bounds__(256) void k_scatter(const QsAtomDev* __restrict__ atoms, int natom,
+                                                 const float* const* __restrict__ luts,
+                                                 const int* __restrict__ kind, ScatterArgs s, float* __restrict__ V) {
+  const int ia = blockIdx.x;
+  if (ia >= natom) return;
+  const QsAtomDev at = atoms[ia];
+  if (at.Znum == 0) return;  // vacancy
+  const float dzs = s.sliceThickness;
+  const float c = dzs * s.slices;
+  // :886-889, then :945 (the -dz and +dz are kept to reproduce float rounding)
+  float atomZ = (float)((double)(at.z + s.czOffset) - (0.5 * (double)dzs * (double)(1 - s.centerSlices)));
+  atomZ -= dzs;
+  if (atomZ - s.atomRadius > c) return;
+  if (atomZ + s.atomRadius + dzs < 0) return;
+  atomZ += dzs;
+  const float atomX = at.x - s.potOffsetX, atomY = at.y - s.potOffsetY;
+  const int iAtomX = (int)floor(atomX / s.dx), iAtomY = (int)floor(atomY / s.dy);
+  const int iAtomZ = (int)floor(atomZ / dzs + 0.5);
+  int iax0, iax1, iay0, iay1;
+  if (!s.periodic) {
+    iax0 = iAtomX - s.iRadX < 0 ? 0 : iAtomX - s.iRadX;
+    iax1 = iAtomX + s.iRadX >= s.potNx ? s.potNx - 1 : iAtomX + s.iRadX;
+    iay0 = iAtomY - s.iRadY < 0 ? 0 : iAtomY - s.iRadY;
+    iay1 = iAtomY + s.iRadY >= s.potNy ? s.potNy - 1 : iAtomY + s.iRadY;
+    if (!(iax0 < s.potNx && iax1 >= 0 && iay0 < s.potNy && iay1 >= 0)) return;
+  } else {  // :1106-1109, upper bounds exclusive
+    iax0 = iAtomX - s.iRadX; iax1 = iAtomX + s.iRadX - 1;
+    iay0 = iAtomY - s.iRadY; iay1 = iAtomY + s.iRadY - 1;
+  }
+  const int iaz0 = iAtomZ - s.iRadZ < 0 ? -iAtomZ : -s.iRadZ;
+  const int iaz1 = iAtomZ + s.iRadZ >= s.slices ? s.slices - iAtomZ - 1 : s.iRadZ;
+  if (!(iAtomZ + iaz0 < s.slices && iAtomZ + iaz1 >= 0)) return;
+  const float* lut = luts[kind[ia]];
+  const float zrel = atomZ / dzs;
+  const double dOffsZ = (double)((float)((float)(iAtomZ + iaz0) - zrel) * (float)s.nzSub);
+  const int iOffsZ0 = (int)(dOffsZ + 0.5) * s.Nr;
+  const int limHi = s.Nr * (s.Nz - 1), limLo = -s.Nr * (s.Nz - 1), step = s.nzSub * s.Nr;
+  const int nbx = iax1 - iax0 + 1, nby = iay1 - iay0 + 1;
+  const double dr = (double)s.dr;
+  const size_t slice = (size_t)s.potNx * s.potNy;
+  for (int pidx = threadIdx.x; pidx < nbx * nby; pidx += blockDim.x) {
+    const int iax = iax0 + pidx / nby, iay = iay0 + pidx % nby;
+    double x2 = (float)(iax * s.dx - atomX);
+    x2 *= x2;
+    double y2 = (float)(iay * s.dy - atomY);
+    y2 *= y2;
+    const double r = sqrt(x2 + y2);
+    double ddr = r / dr;
+    const int ir = (int)floor(ddr);
+    if (ir >= s.Nr - 1) continue;
+    ddr -= ir;
+    // periodic wrap (the reference adds 2*potN before the modulo; a full
+    // modulo keeps atoms further outside the array in bounds as well)
+    const int ixw = s.periodic ? (((iax % s.potNx) + s.potNx) % s.potNx) : iax;
+    const int iyw = s.periodic ? (((iay % s.potNy) + s.potNy) % s.potNy) : iay;
+    float* cell = V + (size_t)(iAtomZ + iaz0) * slice + (size_t)ixw * s.potNy + iyw;
+    int iOffsZ = iOffsZ0;
+    for (int iaz = iaz0; iaz <= iaz1; iaz++) {
+      double pv = 0;
+      if (iOffsZ < 0) {
+        if (iOffsZ > limLo) pv = (1 - ddr) * lut[ir - iOffsZ + s.Nr] + ddr * lut[ir + 1 - iOffsZ + s.Nr];
+      } else if (iOffsZ < limHi) {
+        pv = (1 - ddr) * lut[ir + iOffsZ] + ddr * lut[ir + 1 + iOffsZ];
+      }
+      if (pv != 0) atomicAdd(cell, (float)pv);
+      cell += slice;
+      iOffsZ += step;
+    }
+  }
+}
+
+static unsigned gridn(size_t n) {
+  size_t g = (n + 255) / 256;
+  if (g > 65536) g = 65536;
+  return (unsigned)(g ? g : 1);
+}
+
+void launch_lut_fk(const double* spl, const LutArgs& g, double B, float2* out, hipStream_t st) {
+  hipLaunchKernelGGL(k_lut_fk, dim3(gridn((size_t)g.nx * g.nz)), dim3(256), 0, st, spl, g, B, out);
+}
+
+void launch_lut_extract(const float2* tmp, const LutArgs& g, float* lut, hipStream_t st) {
+  hipLaunchKernelGGL(k_lut_extract, dim3(gridn((size_t)(g.nx / 2) * (g.nz / 2))), dim3(256), 0, st, tmp, g, lut);
+}
+
+void launch_scatter(const QsAtomDev* atoms, int natom, const float* const* luts, const int* kind,
+                    const ScatterArgs& s, float* V, hipStream_t st) {
+  if (natom > 0) hipLaunchKernelGGL(k_scatter, dim3(natom), dim3(256), 0, st, atoms, natom, luts, kind, s, V);
+}
+
+}  // namespace qs

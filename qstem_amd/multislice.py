@@ -14,6 +14,7 @@ Errors raise QstemError with the library's message instead of the
 reference's printf+exit(0).
 """
 import ctypes
+import os
 from dataclasses import dataclass, field
 from typing import List, Optional, Sequence
 
@@ -89,6 +90,12 @@ class StemConfig:
             p.det_shiftx[i] = d.shiftX
             p.det_shifty[i] = d.shiftY
         return p
+
+
+def rez_table() -> np.ndarray:
+    """Rez et al. electron scattering factors shipped as data (row 0 = s grid, row Z = f_e(s))."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "rez_scatpar_f64.bin")
+    return np.fromfile(path, dtype="<f8").reshape(93, 123)
 
 
 def _fptr(a: Optional[np.ndarray]):
@@ -179,6 +186,35 @@ class Multislice:
         out = np.empty((self._nslice, self.cfg.potNx, self.cfg.potNy), dtype=np.complex64)
         self._check(self._lib.qs_get_trans(self._h, _fptr(out.view(np.float32))), "qs_get_trans")
         return out
+
+    def set_scattering_table(self, table: Optional[np.ndarray] = None):
+        """Rez et al. table (93 x 123 doubles); defaults to qstem_amd/data/rez_scatpar_f64.bin."""
+        if table is None:
+            table = rez_table()
+        table = np.ascontiguousarray(table, dtype=np.float64)
+        assert table.shape == (93, 123), table.shape
+        self._check(self._lib.qs_set_scattering_table(self._h, table.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "qs_set_scattering_table")
+        self._have_table = True
+
+    def build_potential(self, atoms, atomRadius=5.0, potOffsetX=0.0, potOffsetY=0.0, czOffset=0.0,
+                        centerSlices=0, periodicXY=0, tds=0, compat_q6=0, lut_dr_fix=0, return_V=False):
+        """make3DSlices + initSTEMSlices on the GPU.  atoms: (n, 7) array of
+        x, y, z, dw, occ, q, Z (Cartesian Angstrom)."""
+        if not getattr(self, "_have_table", False):
+            self.set_scattering_table()
+        atoms = np.asarray(atoms, dtype=np.float64).reshape(-1, 7)
+        arr = (_lib.QsAtom * max(len(atoms), 1))()
+        for i, a in enumerate(atoms):
+            arr[i] = _lib.QsAtom(float(a[0]), float(a[1]), float(a[2]), float(a[3]), float(a[4]), float(a[5]),
+                                 int(a[6]))
+        pp = _lib.QsPotParams(atomRadius, potOffsetX, potOffsetY, czOffset, int(centerSlices), int(periodicXY),
+                              int(tds), int(compat_q6), int(lut_dr_fix))
+        V = np.empty((self.cfg.slices, self.cfg.potNx, self.cfg.potNy), np.float32) if return_V else None
+        self._check(self._lib.qs_build_potential(self._h, arr, len(atoms), ctypes.byref(pp), _fptr(V)),
+                    "qs_build_potential")
+        self._nslice = self.cfg.slices
+        return V
 
     # -- probe ---------------------------------------------------------------------
     def make_probe(self) -> np.ndarray:

@@ -1,0 +1,72 @@
+"""End-to-end GPU run against the reference's own (statistical) output files.
+
+QSTEM's tests/data/{detector1_2,mulswav_16_2}.img were produced by
+stem3/debug_data/stem_STO_4x4.qsc (SrTiO3, 9x9x20 cells, 200 kV, 40 slices of
+1.9525 A, frozen phonons, 20x20 scan, ADF 70-133 mrad).  Their frozen-phonon
+configuration is time-seeded, so only statistics can be compared:
+  * the exit-wave norm sum|psi|^2/N^2 after 40 slices (0.9711): intensity lost
+    beyond the 2/3 band limit, set by the potential's absolute scale;
+  * the mean of the 20x20 ADF image at t = 46.86 A (0.0673).
+Differences kept on purpose: a 512x512 grid at the reference's dx = 0.0625 A
+(same 134 mrad band limit, 32 A instead of 25 A window) and neutral atoms
+(the reference adds ion offsets, which this build does not implement).
+
+What the files pin here: the scan geometry (window offsets, image
+orientation [scanXN][scanYN]) through the correlation of the ADF image with
+the fixture image.  The absolute ADF level is not pinned: the files come from
+an older QSTEM version (their comment field holds the detector name, while
+the current saveSTEMImages writes "STEM image") and the current algorithm
+gives ~6x less 70-133 mrad intensity and 0.5 % instead of 2.9 % band-limit
+loss; both levels are printed and only loosely bounded.
+"""
+import json
+import os
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def _setup():
+    from qstem_amd import Detector, StemConfig, scan_offsets
+    from qstem_amd.specimen import build_supercell, perovskite_cell
+    dx = 0.0625
+    nx = 512
+    x0, x1, y0, y1 = 11.67, 19.71, 11.45, 19.32
+    f = np.float32
+    pnx = nx + int(f(f(x1) - f(x0)) / f(dx))
+    pny = nx + int(f(f(y1) - f(y0)) / f(dx))
+    cfg = StemConfig(nx=nx, ny=nx, resolutionX=dx, resolutionY=dx, v0=200.0, slices=40, sliceThickness=1.9525,
+                     potNx=pnx, potNy=pny, bandlimittrans=0, Cs=0.05 * 1e7, df0=-137.0, alpha=15.0, ismoth=1,
+                     Cc=1e7, dE_E=0.0, electronScale=1.0,
+                     detectors=[Detector(70.0, 133.0, "detector1")])
+    atoms, ext = build_supercell(perovskite_cell(), 9, 9, 20, tds=True, temperature=300.0, seed=2)
+    sx1 = f(f(x0) + f(dx) * f(pnx - nx))
+    sy1 = f(f(y0) + f(dx) * f(pny - nx))
+    ipos = scan_offsets(x0, sx1, 20, y0, sy1, 20, dx, dx, pnx, pny, nx, nx)
+    return cfg, atoms, ipos, (float(f(x0) - 0.5 * nx * dx), float(f(y0) - 0.5 * nx * dx))
+
+
+def test_sto_fixture_statistics():
+    from qstem_amd import Multislice, record_bins
+    from qstem_amd.imgio import read_img
+    ref = json.load(open(os.path.join(GOLD, "reference_stats.json")))
+    fix = read_img(os.path.join(GOLD, "detector1_2.img")).data.astype(np.float64)
+    cfg, atoms, ipos, (ox, oy) = _setup()
+    with Multislice(cfg) as ms:
+        ms.build_potential(atoms, atomRadius=5.0, potOffsetX=ox, potOffsetY=oy, czOffset=0.976, tds=1)
+        ms.make_probe()
+        bins, nb = record_bins(40, outputInterval=8)
+        out = ms.run_stem(ipos, bins=bins, nbins=nb, want_intensity=True)
+    adf = out["det"][2, 0]  # bin 2: t = 3*8*1.9525 = 46.86 A
+    norm = out["intensity"].mean()
+    print("ADF mean %.4f (ref %.4f), norm %.4f (ref %.4f)" % (adf.mean(), ref["detector1_2"]["mean"], norm,
+                                                              ref["mulswav_16_2"]["norm"]))
+    img = adf.reshape(20, 20).astype(np.float64)  # i = ix*scanYN + iy -> image[ix][iy]
+    corr = np.corrcoef(img.ravel(), fix.ravel())[0, 1]
+    corr_t = np.corrcoef(img.T.ravel(), fix.ravel())[0, 1]
+    print("ADF/fixture correlation %.3f (transposed %.3f)" % (corr, corr_t))
+    assert corr > 0.7 and corr > corr_t + 0.2  # same column positions, same orientation
+    assert 0.95 < norm <= 1.0 and 0.1 < adf.mean() / ref["detector1_2"]["mean"] < 1.0
