@@ -118,6 +118,11 @@ int qs_set_scattering_table(qs_ctx *ctx, const double *table);
  * V_out (nullable) receives V [slices][potNx][potNy]. */
 int qs_build_potential(qs_ctx *ctx, const qs_atom *atoms, int natom, const qs_pot_params *pp, float *V_out);
 
+/* Replaces the probe optics of the context (Cs, C5, df0, astigmatism, a_nm/phi_nm,
+ * alpha, aAIS, Cc, dE_E, smooth, gaussian) by those of p; other fields are ignored.
+ * doSTEM changes dE_E between frozen-phonon runs (stem3.cpp:2750). */
+int qs_set_optics(qs_ctx *ctx, const qs_params *p);
+
 /* Forms the probe at the window centre (the only position the driver uses,
  * stem3.cpp:2838); keeps it on the device; copies it to out (nx*ny complex64) when non-NULL. */
 int qs_make_probe(qs_ctx *ctx, float *out);

@@ -73,6 +73,7 @@ SIGNATURES = {
     "qs_set_trans": (_I, [_P, _FP, _I]),
     "qs_get_trans": (_I, [_P, _FP]),
     "qs_make_probe": (_I, [_P, _FP]),
+    "qs_set_optics": (_I, [_P, ctypes.POINTER(QsParams)]),
     "qs_set_scattering_table": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "qs_build_potential": (_I, [_P, ctypes.POINTER(QsAtom), _I, ctypes.POINTER(QsPotParams), _FP]),
     "qs_record_bins": (_I, [_I, _I, _I, _I, _IP]),

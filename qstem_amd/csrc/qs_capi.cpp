@@ -604,6 +604,21 @@ int qs_make_probe(qs_ctx* c, float* out) {
   return QS_OK;
 }
 
+int qs_set_optics(qs_ctx* c, const qs_params* p) {
+  if (!c || !p) return fail(c, QS_ERR_ARG, "qs_set_optics: bad arguments");
+  qs_params& q = c->p;
+  q.Cs = p->Cs; q.C5 = p->C5; q.df0 = p->df0; q.astigMag = p->astigMag; q.astigAngle = p->astigAngle;
+  q.a33 = p->a33; q.a31 = p->a31; q.a44 = p->a44; q.a42 = p->a42; q.a55 = p->a55;
+  q.a53 = p->a53; q.a51 = p->a51; q.a66 = p->a66; q.a64 = p->a64; q.a62 = p->a62;
+  q.phi33 = p->phi33; q.phi31 = p->phi31; q.phi44 = p->phi44; q.phi42 = p->phi42; q.phi55 = p->phi55;
+  q.phi53 = p->phi53; q.phi51 = p->phi51; q.phi66 = p->phi66; q.phi64 = p->phi64; q.phi62 = p->phi62;
+  q.alpha = p->alpha; q.aAIS = p->aAIS; q.Cc = p->Cc; q.dE_E = p->dE_E;
+  q.ismoth = p->ismoth; q.gaussFlag = p->gaussFlag; q.gaussScale = p->gaussScale;
+  q.compat = (q.compat & ~QS_COMPAT_Q2) | (p->compat & QS_COMPAT_Q2);
+  c->have_probe = false;
+  return QS_OK;
+}
+
 int qs_record_bins(int slices, int cellDiv, int outputInterval, int totalSliceCount, int* bin_of_slice) {
   if (slices < 1 || cellDiv < 1 || outputInterval < 0 || !bin_of_slice) return -1;
   const int total = slices * cellDiv;
