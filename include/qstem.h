@@ -145,6 +145,20 @@ int qs_record_bins(int slices, int cellDiv, int outputInterval, int totalSliceCo
 int qs_run_stem(qs_ctx *ctx, const int *ipos, int nprobe, int start, const int *bin_of_slice, int nslice, int nbins,
                 float *det, float *diff, float *exit_wave, float *int_intensity);
 
+/* Adjoint of qs_run_stem for the loss L = sum_p sum_k (I_p(k) - target_p(k))^2, where
+ * I_p(k) = |Psi_p(k) P(k)|^2/(nx ny)^2 is the diffraction intensity after the last slice
+ * (fraction of the incident beam per pixel, FFTW order [nprobe][nx][ny]).  All
+ * transmission slices are used, starting from the probe.  The forward waves are kept
+ * every ckpt_every slices (0: ceil(sqrt(nslice))) and recomputed inside each segment.
+ * Outputs (nullable): loss; grad_V [nslice][potNx][potNy] = dL/dV (needs the potential
+ * set by qs_set_potential or qs_build_potential); grad_probe[2] = dL/d(df0), dL/d(Cs) in
+ * the units of qs_params (Angstrom).  No reference counterpart (QSTEM has no adjoint). */
+int qs_adjoint(qs_ctx *ctx, const int *ipos, int nprobe, const float *target, int ckpt_every, float *loss,
+               float *grad_V, float *grad_probe);
+/* dL/d(x, y, z) of the atoms of the last qs_build_potential (same atoms, same pp), from the
+ * grad_V of the last qs_adjoint; z is 0 (the potential samples z by nearest slice). */
+int qs_atom_gradients(qs_ctx *ctx, const qs_atom *atoms, int natom, const qs_pot_params *pp, float *grad_xyz);
+
 /* Probes advanced together through all slices before the next group starts
  * (0 = library default).  batch = 1 is the per-probe launch sequence. */
 int qs_set_batch(qs_ctx *ctx, int batch);

@@ -190,3 +190,41 @@ def make_slices(cfg, g, luts, kinds, atoms, potOffsetX, potOffsetY, czOffset=0.0
                           _F(potOffsetX), _F(potOffsetY), _F(czOffset), centerSlices, _F(atomRadius), periodic,
                           _p(V, _F))
     return V
+
+
+def probe_d(cfg, df0, Cs):
+    out = np.empty((cfg.nx, cfg.ny), dtype=np.complex128)
+    p = params(cfg)
+    load().qo_probe_d(ctypes.byref(p), _D(df0), _D(Cs), _p(out))
+    return out
+
+
+def make_trans_d(cfg, V):
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    out = np.empty(V.shape, dtype=np.complex128)
+    p = params(cfg)
+    load().qo_make_trans_d(ctypes.byref(p), _p(V), V.shape[0], _p(out))
+    return out
+
+
+def make_slices_d(cfg, g, luts, kinds, xyz, Z, potOffsetX, potOffsetY, czOffset=0.0, centerSlices=0,
+                  atomRadius=5.0, dr=None):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    Z = np.ascontiguousarray(Z, dtype=np.int32)
+    kinds = np.ascontiguousarray(kinds, dtype=np.int32)
+    lut_ptrs = (ctypes.POINTER(_F) * len(luts))(*[_p(l, _F) for l in luts])
+    V = np.zeros((cfg.slices, cfg.potNx, cfg.potNy), np.float64)
+    dr = float(np.float32(cfg.resolutionX) / 2) if dr is None else dr
+    p = params(cfg)
+    load().qo_make_slices_d(ctypes.byref(p), ctypes.byref(g), lut_ptrs, _p(kinds, _I), _p(xyz), _p(Z, _I), len(Z),
+                            _D(potOffsetX), _D(potOffsetY), _D(czOffset), centerSlices, _D(atomRadius), _D(dr),
+                            _p(V))
+    return V
+
+
+def diffraction_loss(cfg, trans, ipos, psi0, target):
+    """L = sum_p sum_k (|FFT(psi_exit)/(nx ny)|^2 - target)^2 with the oracle's FP64 multislice."""
+    ex, _, _ = multislice(cfg, trans, ipos, psi0)
+    n2 = cfg.nx * cfg.ny
+    I = np.abs(np.fft.fft2(ex, axes=(1, 2)) / n2) ** 2
+    return float(np.sum((I - target) ** 2)), I

@@ -159,7 +159,10 @@ double qo_seval(const double *x, const double *y, const double *b, const double 
 /* probe(), stem3/stemlib.cpp:1952-2191.  The wave is built at the window
  * centre as the driver does (stem3.cpp:2838): dx = nx/2*resX.  Output is the
  * normalised real-space probe [nx][ny] with sum |psi|^2 = nx*ny. */
-void qo_probe(const qo_params *p, double complex *psi) {
+void qo_probe(const qo_params *p, double complex *psi) { qo_probe_d(p, p->df0, p->Cs, psi); }
+
+/* probe() with defocus and Cs given in double precision (finite-difference checks). */
+void qo_probe_d(const qo_params *p, double df0, double Cs, double complex *psi) {
   const int nx = p->nx, ny = p->ny;
   const double ax = nx * (double)p->resolutionX, by = ny * (double)p->resolutionY;
   /* stemlib.cpp:1975-1976: dx = ax - (nx/2)*resX with the argument float-valued */
@@ -184,13 +187,13 @@ void qo_probe(const qo_params *p, double complex *psi) {
       double kt2 = k2 * wavlen * wavlen;
       double kt = sqrt(kt2);
       double phi = atan2(ry * ky, rx * kx);
-      double chi = kt2 * ((double)p->df0 + delta + (double)p->astigMag * cos(2.0 * (phi - (double)p->astigAngle))) / 2.0;
+      double chi = kt2 * (df0 + delta + (double)p->astigMag * cos(2.0 * (phi - (double)p->astigAngle))) / 2.0;
       double ktp = kt2 * kt; /* theta^3 */
       if (p->a33 > 0 || p->a31 > 0)
         chi += ktp * (p->a33 * cos(3.0 * (phi - p->phi33)) + p->a31 * cos(phi - p->phi31)) / 3.0;
       ktp *= kt; /* theta^4 */
-      if (p->a44 > 0 || p->a42 > 0 || p->Cs != 0)
-        chi += ktp * (p->a44 * cos(4.0 * (phi - p->phi44)) + p->a42 * cos(2.0 * (phi - p->phi42)) + p->Cs) / 4.0;
+      if (p->a44 > 0 || p->a42 > 0 || Cs != 0)
+        chi += ktp * (p->a44 * cos(4.0 * (phi - p->phi44)) + p->a42 * cos(2.0 * (phi - p->phi42)) + Cs) / 4.0;
       ktp *= kt; /* theta^5 */
       if (p->a55 > 0 || p->a53 > 0 || p->a51 > 0)
         chi += ktp * (p->a55 * cos(5.0 * (phi - p->phi55)) + p->a53 * cos(3.0 * (phi - p->phi53)) +
@@ -273,6 +276,15 @@ void qo_det_k2(const qo_params *p, float *k2in, float *k2out) {
  * potential [nslice][potNx][potNy]; trans receives exp(i*gamma*lambda*V),
  * band limited to 2/3 of Nyquist when bandlimittrans is set. */
 void qo_make_trans(const qo_params *p, const float *V, int nslice, double complex *trans) {
+  const size_t n = (size_t)nslice * p->potNx * p->potNy;
+  double *Vd = (double *)malloc(sizeof(double) * n);
+  for (size_t i = 0; i < n; i++) Vd[i] = V[i];
+  qo_make_trans_d(p, Vd, nslice, trans);
+  free(Vd);
+}
+
+/* Same with a double-precision potential (finite-difference checks). */
+void qo_make_trans_d(const qo_params *p, const double *V, int nslice, double complex *trans) {
   const int nx = p->potNx, ny = p->potNy;
   const size_t n2 = (size_t)nx * ny;
   float mm0 = 1.0F + p->v0 / 511.0F; /* :2271, float arithmetic */
@@ -579,6 +591,67 @@ void qo_make_slices(const qo_params *p, const qo_lut_geom *g, const float *const
           }
           float *cell = V + ((size_t)(iAtomZ + iaz) * nx + ixw) * ny + iyw;
           *cell = (float)(*cell + pv);
+          iOffsZ += step;
+        }
+      }
+    }
+  }
+}
+
+/* make3DSlices with double-precision atom positions and position arithmetic
+ * (same discrete sampling as qo_make_slices; used for finite differences of
+ * the potential with respect to atom positions). */
+void qo_make_slices_d(const qo_params *p, const qo_lut_geom *g, const float *const *luts,
+                      const int *kind_of_atom, const double *xyz, const int *Znum, int natom,
+                      double potOffsetX, double potOffsetY, double czOffset, int centerSlices,
+                      double atomRadius, double dr, double *V) {
+  const int nx = p->potNx, ny = p->potNy, nslc = p->slices;
+  const double dx = p->resolutionX, dy = p->resolutionY, dzs = p->sliceThickness;
+  const int iRadX = (int)ceil(atomRadius / dx), iRadY = (int)ceil(atomRadius / dy);
+  const int iRadZ = (int)ceil(atomRadius / dzs);
+  const double c = dzs * nslc;
+  const int Nr = g->Nr, Nz = g->Nz_lut, nzSub = g->nzPerSlice;
+  const int limHi = Nr * (Nz - 1), limLo = -Nr * (Nz - 1), step = nzSub * Nr;
+  memset(V, 0, sizeof(double) * (size_t)nslc * nx * ny);
+  for (int ia = 0; ia < natom; ia++) {
+    if (Znum[ia] == 0) continue;
+    double atomZ = xyz[3 * ia + 2] + czOffset - 0.5 * dzs * (1 - centerSlices);
+    if (atomZ - dzs - atomRadius > c) continue;
+    if (atomZ + atomRadius < 0) continue;
+    const double atomX = xyz[3 * ia] - potOffsetX, atomY = xyz[3 * ia + 1] - potOffsetY;
+    const int iAtomX = (int)floor(atomX / dx), iAtomY = (int)floor(atomY / dy);
+    const int iAtomZ = (int)floor(atomZ / dzs + 0.5);
+    int iax0 = iAtomX - iRadX < 0 ? 0 : iAtomX - iRadX;
+    int iax1 = iAtomX + iRadX >= nx ? nx - 1 : iAtomX + iRadX;
+    int iay0 = iAtomY - iRadY < 0 ? 0 : iAtomY - iRadY;
+    int iay1 = iAtomY + iRadY >= ny ? ny - 1 : iAtomY + iRadY;
+    if (!(iax0 < nx && iax1 >= 0 && iay0 < ny && iay1 >= 0)) continue;
+    const int iaz0 = iAtomZ - iRadZ < 0 ? -iAtomZ : -iRadZ;
+    const int iaz1 = iAtomZ + iRadZ >= nslc ? nslc - iAtomZ - 1 : iRadZ;
+    if (!(iAtomZ + iaz0 < nslc && iAtomZ + iaz1 >= 0)) continue;
+    const float *lut = luts[kind_of_atom[ia]];
+    const double dOffsZ = ((double)(iAtomZ + iaz0) - atomZ / dzs) * nzSub;
+    const int iOffsZ0 = (int)(dOffsZ + 0.5) * Nr;
+    for (int iax = iax0; iax <= iax1; iax++) {
+      double x2 = iax * dx - atomX;
+      x2 *= x2;
+      for (int iay = iay0; iay <= iay1; iay++) {
+        double y2 = iay * dy - atomY;
+        y2 *= y2;
+        double r = sqrt(x2 + y2);
+        double ddr = r / dr;
+        int ir = (int)floor(ddr);
+        if (ir >= Nr - 1) continue;
+        ddr -= ir;
+        int iOffsZ = iOffsZ0;
+        for (int iaz = iaz0; iaz <= iaz1; iaz++) {
+          double pv = 0;
+          if (iOffsZ < 0) {
+            if (iOffsZ > limLo) pv = (1 - ddr) * lut[ir - iOffsZ + Nr] + ddr * lut[ir + 1 - iOffsZ + Nr];
+          } else if (iOffsZ < limHi) {
+            pv = (1 - ddr) * lut[ir + iOffsZ] + ddr * lut[ir + 1 + iOffsZ];
+          }
+          V[((size_t)(iAtomZ + iaz) * nx + iax) * ny + iay] += pv;
           iOffsZ += step;
         }
       }

@@ -67,9 +67,11 @@ double qo_seval(const double *x, const double *y, const double *b, const double 
 
 /* ---- hot path pieces ---- */
 void qo_probe(const qo_params *p, double complex *psi);
+void qo_probe_d(const qo_params *p, double df0, double Cs, double complex *psi);
 void qo_k_tables(const qo_params *p, float *kx2, float *ky2, float *k2max);
 void qo_det_k2(const qo_params *p, float *k2in, float *k2out);
 void qo_make_trans(const qo_params *p, const float *V, int nslice, double complex *trans);
+void qo_make_trans_d(const qo_params *p, const double *V, int nslice, double complex *trans);
 void qo_scan_offsets(int ix, int iy, float scanXStart, float scanXStop, int scanXN,
                      float scanYStart, float scanYStop, int scanYN,
                      float resX, float resY, int potNx, int potNy, int nx, int ny,
@@ -99,6 +101,11 @@ void qo_make_slices(const qo_params *p, const qo_lut_geom *g, const float *const
                     const int *kind_of_atom, const qo_atom *atoms, int natom,
                     float potOffsetX, float potOffsetY, float czOffset, int centerSlices,
                     float atomRadius, int periodic, float *V);
+
+void qo_make_slices_d(const qo_params *p, const qo_lut_geom *g, const float *const *luts,
+                      const int *kind_of_atom, const double *xyz, const int *Znum, int natom,
+                      double potOffsetX, double potOffsetY, double czOffset, int centerSlices,
+                      double atomRadius, double dr, double *V);
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,8 @@ SIGNATURES = {
     "qs_build_potential": (_I, [_P, ctypes.POINTER(QsAtom), _I, ctypes.POINTER(QsPotParams), _FP]),
     "qs_record_bins": (_I, [_I, _I, _I, _I, _IP]),
     "qs_run_stem": (_I, [_P, _IP, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
+    "qs_adjoint": (_I, [_P, _IP, _I, _FP, _I, _FP, _FP, _FP]),
+    "qs_atom_gradients": (_I, [_P, ctypes.POINTER(QsAtom), _I, ctypes.POINTER(QsPotParams), _FP]),
     "qs_set_batch": (_I, [_P, _I]),
     "qs_get_batch": (_I, [_P]),
     "qs_last_run_ms": (_F, [_P]),

@@ -30,7 +30,7 @@ struct DevBuf {
   size_t bytes = 0;
   hipError_t ensure(size_t n) {
     if (n <= bytes && p) return hipSuccess;
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
     hipError_t e = hipMalloc(&p, n > 0 ? n : 16);
@@ -38,7 +38,7 @@ struct DevBuf {
     return e;
   }
   void release() {
-    if (p) hipFree(p);
+    if (p) (void)hipFree(p);
     p = nullptr;
     bytes = 0;
   }
@@ -121,7 +121,13 @@ struct qs_ctx {
   std::vector<DevBuf> luts;
   RowXform lx, lz;  // LUT transform lengths
   int trans_slices = 0;
+  bool v_valid = false;     // c->V holds the potential of the current transmission slices
+  double probe_norm = 1.0;  // c of psi0 = c * IFFT(Psi0)
+  ProbeArgs probe_args{};
   bool have_probe = false;
+  // adjoint
+  std::vector<DevBuf> ckpt;
+  DevBuf gx, work, target, lossp, gT, gprobe, gV, dsum;
   int state_probes = 0;
   bool state_valid = false;
   // optional per-launch timing of the two sweeps (HIP events on the context stream)
@@ -317,27 +323,31 @@ int qs_create(const qs_params* p, qs_ctx** out) {
 
 void qs_destroy(qs_ctx* c) {
   if (!c) return;
-  if (c->stream) hipStreamSynchronize(c->stream);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
   DevBuf* bufs[] = {&c->twx, &c->twy, &c->px, &c->py, &c->dkx2, &c->dky2, &c->probe, &c->trans, &c->scratch, &c->V,
                     &c->state, &c->ipos, &c->partial, &c->det, &c->diff, &c->exitw, &c->total, &c->pkx2, &c->pky2,
                     &c->probe_partial, &c->rx.tw, &c->rx.chirp_f, &c->rx.vhat_f, &c->rx.chirp_b, &c->rx.vhat_b,
                     &c->ry.tw, &c->ry.chirp_f, &c->ry.vhat_f, &c->ry.chirp_b, &c->ry.vhat_b,
                     &c->atoms, &c->kinds, &c->lutptrs, &c->spl, &c->luttmp, &c->lutscr,
                     &c->lx.tw, &c->lx.chirp_f, &c->lx.vhat_f, &c->lx.chirp_b, &c->lx.vhat_b,
-                    &c->lz.tw, &c->lz.chirp_f, &c->lz.vhat_f, &c->lz.chirp_b, &c->lz.vhat_b};
+                    &c->lz.tw, &c->lz.chirp_f, &c->lz.vhat_f, &c->lz.chirp_b, &c->lz.vhat_b,
+                    &c->gx, &c->work, &c->target, &c->lossp, &c->gT, &c->gprobe, &c->gV, &c->dsum};
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : c->luts) b.release();
-  for (hipEvent_t e : c->evpool) hipEventDestroy(e);
-  if (c->ev0) hipEventDestroy(c->ev0);
-  if (c->ev1) hipEventDestroy(c->ev1);
-  if (c->stream) hipStreamDestroy(c->stream);
+  for (DevBuf& b : c->ckpt) b.release();
+  for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
 }
 
-static int bandlimit_trans(qs_ctx* c, int nslice) {
+// Band limit of initSTEMSlices (FFT, 2/3 mask with 1/(potNx potNy), IFFT) applied in
+// place to `tr` [nslice][potNx][potNy]; the operator is self-adjoint, so the adjoint
+// uses it unchanged on the transmission gradient.
+static int bandlimit_buf(qs_ctx* c, float2* tr, int nslice) {
   const qs_params& p = c->p;
   const int nx = p.potNx, ny = p.potNy;
-  float2* tr = c->trans.as<float2>();
   float2* sc = c->scratch.as<float2>();
   run_rowxform(c, c->ry, tr, (long)nslice * nx, -1, 1.0f);
   qs::launch_transpose(tr, sc, nslice, nx, ny, c->stream);
@@ -368,7 +378,7 @@ static int build_trans(qs_ctx* c, int nslice) {
   HIPCHK(c, hipGetLastError());
   if (p.bandlimittrans) {
     HIPCHK(c, c->scratch.ensure(sizeof(float2) * n));
-    int rc = bandlimit_trans(c, nslice);
+    int rc = bandlimit_buf(c, c->trans.as<float2>(), nslice);
     if (rc != QS_OK) return rc;
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
@@ -382,7 +392,9 @@ int qs_set_potential(qs_ctx* c, const float* V, int nslice) {
   const size_t n = (size_t)nslice * p.potNx * p.potNy;
   HIPCHK(c, c->V.ensure(sizeof(float) * n));
   HIPCHK(c, hipMemcpyAsync(c->V.p, V, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
-  return build_trans(c, nslice);
+  const int rc = build_trans(c, nslice);
+  c->v_valid = rc == QS_OK;
+  return rc;
 }
 
 // Akima spline coefficients (the fit of stem3/stemutil.cpp:975-1041), host set-up of 123 nodes.
@@ -518,7 +530,9 @@ int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_
   }
   HIPCHK(c, hipGetLastError());
   if (V_out) HIPCHK(c, hipMemcpyAsync(V_out, c->V.p, sizeof(float) * nV, hipMemcpyDeviceToHost, c->stream));
-  return build_trans(c, p.slices);
+  rc = build_trans(c, p.slices);
+  c->v_valid = rc == QS_OK;
+  return rc;
 }
 
 int qs_set_trans(qs_ctx* c, const float* trans, int nslice) {
@@ -528,6 +542,7 @@ int qs_set_trans(qs_ctx* c, const float* trans, int nslice) {
   HIPCHK(c, hipMemcpyAsync(c->trans.p, trans, sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->trans_slices = nslice;
+  c->v_valid = false;
   return QS_OK;
 }
 
@@ -599,7 +614,14 @@ int qs_make_probe(qs_ctx* c, float* out) {
   qs::launch_probe_window(a, pr, c->probe_partial.as<double>(), c->stream);
   HIPCHK(c, hipGetLastError());
   if (out) HIPCHK(c, hipMemcpyAsync(out, pr, sizeof(float2) * n, hipMemcpyDeviceToHost, c->stream));
+  std::vector<double> parts(qs::probe_window_blocks(nx * ny));
+  HIPCHK(c, hipMemcpyAsync(parts.data(), c->probe_partial.p, sizeof(double) * parts.size(), hipMemcpyDeviceToHost,
+                           c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
+  double tot = 0;
+  for (double v : parts) tot += v;
+  c->probe_norm = sqrt((double)n / tot);
+  c->probe_args = a;
   c->have_probe = true;
   return QS_OK;
 }
@@ -802,4 +824,187 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
   return QS_OK;
 }
 
+
+// ---------------------------------------------------------------------------------
+// Adjoint multislice with slice checkpointing.
+static void sweep_args(qs_ctx* c, SweepArgs& a, int nprobe) {
+  const qs_params& p = c->p;
+  a = SweepArgs{};
+  a.nx = p.nx; a.ny = p.ny; a.nprobe = nprobe; a.probe0 = 0;
+  a.kc = c->kc; a.kcp = c->kcp; a.band_a = c->band_a; a.band_b = c->band_b; a.ntiles = c->ntiles;
+  a.potNy = p.potNy;
+  a.probe = c->probe.as<float2>();
+  a.ipos = c->ipos.as<int2>();
+  a.twx = c->twx.as<float2>(); a.twy = c->twy.as<float2>();
+  a.px = c->px.as<float2>(); a.py = c->py.as<float2>();
+  a.kx2 = c->dkx2.as<float>(); a.ky2 = c->dky2.as<float>();
+  a.k2max = c->k2max;
+  a.rec_bin = -1;
+  a.ndet = 0;
+  a.partial = c->partial.as<float>();
+}
+
+int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int ckpt_every, float* loss_out,
+               float* grad_V, float* grad_probe) {
+  if (!c || !ipos || nprobe < 1 || !target) return fail(c, QS_ERR_ARG, "qs_adjoint: bad arguments");
+  const qs_params& p = c->p;
+  const int S = c->trans_slices;
+  if (S < 1) return fail(c, QS_ERR_STATE, "qs_adjoint: no transmission functions");
+  if (!c->have_probe) return fail(c, QS_ERR_STATE, "qs_adjoint: qs_make_probe not called");
+  if (grad_V && !c->v_valid) return fail(c, QS_ERR_STATE, "qs_adjoint: grad_V needs qs_set_potential/qs_build_potential");
+  if (grad_probe && (p.gaussFlag || p.aAIS > 0))
+    return fail(c, QS_ERR_UNSUPPORTED, "qs_adjoint: probe gradients need gaussian: no and no AIS aperture");
+  for (int i = 0; i < nprobe; i++) {
+    const int x = ipos[2 * i], y = ipos[2 * i + 1];
+    if (x < 0 || y < 0 || x + p.nx > p.potNx || y + p.ny > p.potNy)
+      return fail(c, QS_ERR_ARG, "qs_adjoint: window offset outside the transmission array");
+  }
+  const int K = ckpt_every > 0 ? ckpt_every : (int)ceil(sqrt((double)S));
+  const int nseg = (S + K - 1) / K;
+  const size_t st_elems = (size_t)nprobe * p.nx * c->kcp;
+  const size_t st_bytes = sizeof(float2) * st_elems;
+  const size_t slice_elems = (size_t)p.potNx * p.potNy;
+  HIPCHK(c, c->state.ensure(st_bytes));
+  HIPCHK(c, c->gx.ensure(st_bytes));
+  HIPCHK(c, c->work.ensure(st_bytes));
+  if ((int)c->ckpt.size() < nseg + K) c->ckpt.resize(nseg + K);
+  for (int j = 0; j < nseg + K; j++) HIPCHK(c, c->ckpt[j].ensure(st_bytes));  // [0,nseg): X_{jK-1}; [nseg,nseg+K): segment
+  HIPCHK(c, c->ipos.ensure(sizeof(int) * 2 * nprobe));
+  HIPCHK(c, hipMemcpyAsync(c->ipos.p, ipos, sizeof(int) * 2 * nprobe, hipMemcpyHostToDevice, c->stream));
+  const size_t nt = (size_t)nprobe * p.nx * p.ny;
+  HIPCHK(c, c->target.ensure(sizeof(float) * nt));
+  HIPCHK(c, hipMemcpyAsync(c->target.p, target, sizeof(float) * nt, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, c->partial.ensure(sizeof(float) * (size_t)nprobe * c->ntiles * (qs::kMaxDet + 1)));
+  HIPCHK(c, c->lossp.ensure(sizeof(float) * (size_t)nprobe * c->ntiles));
+  HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * S));
+  HIPCHK(c, c->gprobe.ensure(sizeof(float2) * (size_t)p.nx * p.ny));
+  HIPCHK(c, c->dsum.ensure(sizeof(double) * 4));
+  qs::launch_cplx_fill_zero(c->gT.as<float>(), 2 * slice_elems * S, c->stream);
+  qs::launch_cplx_fill_zero(c->gprobe.as<float>(), 2 * (size_t)p.nx * p.ny, c->stream);
+  qs::launch_cplx_fill_zero(reinterpret_cast<float*>(c->dsum.p), 8, c->stream);
+  SweepArgs a;
+  sweep_args(c, a, nprobe);
+  a.state = c->state.as<float2>();
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  // forward pass, keeping X_{jK-1} for every segment j >= 1
+  qs::launch_fill_zero(a.state, st_elems, c->stream);
+  for (int s = 0; s < S; s++) {
+    a.trans = c->trans.as<float2>() + (size_t)s * slice_elems;
+    qs::launch_sweep_rows(a, s == 0 ? 0 : 1, c->stream);
+    qs::launch_sweep_cols(a, false, false, c->stream);
+    if ((s + 1) % K == 0 && s + 1 < S)
+      HIPCHK(c, hipMemcpyAsync(c->ckpt[(s + 1) / K].p, a.state, st_bytes, hipMemcpyDeviceToDevice, c->stream));
+  }
+  // backward, segment by segment, recomputing the forward states inside each segment
+  qs::AdjArgs A{};
+  A.f = a;
+  A.gx = c->gx.as<float2>();
+  A.target = c->target.as<float>();
+  A.loss_partial = c->lossp.as<float>();
+  A.gprobe = c->gprobe.as<float>();
+  qs::launch_fill_zero(A.gx, st_elems, c->stream);
+  for (int j = nseg - 1; j >= 0; j--) {
+    const int s0 = j * K, s1 = std::min(S, s0 + K) - 1;
+    // recompute X_s, s in [s0, s1-1], into segment slots nseg + (s - s0)
+    SweepArgs w = a;
+    w.state = c->work.as<float2>();
+    if (s0 > 0) HIPCHK(c, hipMemcpyAsync(w.state, c->ckpt[j].p, st_bytes, hipMemcpyDeviceToDevice, c->stream));
+    else qs::launch_fill_zero(w.state, st_elems, c->stream);
+    for (int s = s0; s < s1; s++) {
+      w.trans = c->trans.as<float2>() + (size_t)s * slice_elems;
+      qs::launch_sweep_rows(w, s == 0 ? 0 : 1, c->stream);
+      qs::launch_sweep_cols(w, false, false, c->stream);
+      HIPCHK(c, hipMemcpyAsync(c->ckpt[nseg + (s - s0)].p, w.state, st_bytes, hipMemcpyDeviceToDevice, c->stream));
+    }
+    for (int s = s1; s >= s0; s--) {
+      A.f.trans = c->trans.as<float2>() + (size_t)s * slice_elems;
+      A.gT = c->gT.as<float>() + 2 * (size_t)s * slice_elems;
+      const bool last = s == S - 1;
+      A.xprev = last ? c->state.as<float2>() : nullptr;
+      qs::launch_adj_cols(A, last, c->stream);
+      if (s > 0) A.xprev = (s - 1 >= s0) ? c->ckpt[nseg + (s - 1 - s0)].as<float2>() : c->ckpt[j].as<float2>();
+      qs::launch_adj_rows(A, s == 0, c->stream);
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  // loss = sum over band pixels (partials) + sum of target^2 outside the band
+  qs::launch_target_outside_band(c->target.as<float>(), nprobe, p.nx, p.ny, c->band_a, c->band_b,
+                                 c->dsum.as<double>() + 2, c->stream);
+  // dL/dV: band limit (self-adjoint) of dL/dt, then sigma Im(g conj(u))
+  if (grad_V) {
+    if (p.bandlimittrans) {
+      HIPCHK(c, c->scratch.ensure(sizeof(float2) * slice_elems * S));
+      int rc = bandlimit_buf(c, c->gT.as<float2>(), S);
+      if (rc != QS_OK) return rc;
+    }
+    HIPCHK(c, c->gV.ensure(sizeof(float) * slice_elems * S));
+    const float mm0 = 1.0F + p.v0 / 511.0F;
+    qs::launch_grad_v(c->gT.as<float2>(), c->V.as<float>(), c->gV.as<float>(), slice_elems * S,
+                      mm0 * wavelength(p.v0), c->stream);
+  }
+  if (grad_probe) {  // G = c * FFT(sum_p g_psi0)
+    float2* g = c->gprobe.as<float2>();
+    HIPCHK(c, c->scratch.ensure(std::max(c->scratch.bytes, sizeof(float2) * (size_t)p.nx * p.ny)));
+    float2* sc = c->scratch.as<float2>();
+    qs::launch_rows_fft(g, p.nx, p.ny, -1, c->twy.as<float2>(), 1.0f, c->stream);
+    qs::launch_transpose(g, sc, 1, p.nx, p.ny, c->stream);
+    qs::launch_rows_fft(sc, p.ny, p.nx, -1, c->twx.as<float2>(), 1.0f, c->stream);
+    qs::launch_transpose(sc, g, 1, p.ny, p.nx, c->stream);
+    qs::launch_probe_grad(c->probe_args, g, c->probe_norm, c->probe_args.wavlen, c->dsum.as<double>(), c->stream);
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  std::vector<float> lp((size_t)nprobe * c->ntiles);
+  double ds[4];
+  HIPCHK(c, hipMemcpyAsync(lp.data(), c->lossp.p, sizeof(float) * lp.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ds, c->dsum.p, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
+  if (grad_V)
+    HIPCHK(c, hipMemcpyAsync(grad_V, c->gV.p, sizeof(float) * slice_elems * S, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  double loss = ds[2];
+  for (float v : lp) loss += v;
+  if (loss_out) *loss_out = (float)loss;
+  if (grad_probe) {
+    grad_probe[0] = (float)ds[0];
+    grad_probe[1] = (float)ds[1];
+  }
+  c->state_valid = false;
+  return QS_OK;
+}
+
+int qs_atom_gradients(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_params* pp, float* grad_xyz) {
+  if (!c || !pp || !grad_xyz || natom < 0 || (natom > 0 && !atoms)) return fail(c, QS_ERR_ARG, "qs_atom_gradients: bad arguments");
+  if (c->gV.bytes == 0) return fail(c, QS_ERR_STATE, "qs_atom_gradients: run qs_adjoint with grad_V first");
+  if (c->luts.empty() || natom == 0) {
+    memset(grad_xyz, 0, sizeof(float) * 3 * natom);
+    return QS_OK;
+  }
+  const qs_params& p = c->p;
+  // the kinds and LUTs of the last qs_build_potential call are reused: same atoms required
+  qs::ScatterArgs sa{};
+  const float R = pp->atomRadius, dz = p.sliceThickness;
+  int nzp = (int)floor(2 * dz / p.resolutionX);
+  if (2.0 * (nzp >> 1) == nzp) nzp += 1;
+  const int lnx = 2 * 2 * (int)ceil(R / p.resolutionX);
+  const int lnz = (2 * (int)ceil(R / dz)) * nzp;
+  sa.potNx = p.potNx; sa.potNy = p.potNy; sa.slices = p.slices;
+  sa.dx = p.resolutionX; sa.dy = p.resolutionY; sa.sliceThickness = dz;
+  sa.potOffsetX = pp->potOffsetX; sa.potOffsetY = pp->potOffsetY; sa.czOffset = pp->czOffset;
+  sa.atomRadius = R; sa.centerSlices = pp->centerSlices; sa.periodic = pp->periodicXY;
+  sa.iRadX = (int)ceil(R / p.resolutionX); sa.iRadY = (int)ceil(R / p.resolutionY); sa.iRadZ = (int)ceil(R / dz);
+  sa.Nr = lnx / 2; sa.Nz = lnz / 2; sa.nzSub = nzp;
+  sa.dr = pp->lut_dr_fix ? p.resolutionX : p.resolutionX / 2;
+  HIPCHK(c, c->atoms.ensure(sizeof(qs_atom) * natom));
+  HIPCHK(c, hipMemcpyAsync(c->atoms.p, atoms, sizeof(qs_atom) * natom, hipMemcpyHostToDevice, c->stream));
+  DevBuf g;
+  HIPCHK(c, g.ensure(sizeof(float) * 3 * natom));
+  qs::launch_atom_grad(c->atoms.as<qs::QsAtomDev>(), natom, c->lutptrs.as<const float*>(), c->kinds.as<int>(), sa,
+                       c->gV.as<float>(), g.as<float>(), c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipMemcpyAsync(grad_xyz, g.p, sizeof(float) * 3 * natom, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  g.release();
+  return QS_OK;
+}
 }  // extern "C"

@@ -37,6 +37,17 @@ struct SweepArgs {
   float diff_scale;
 };
 
+// Backward sweeps of the adjoint (reverse of SweepArgs' forward sweeps).
+struct AdjArgs {
+  SweepArgs f;              // geometry, tables, trans slice, ipos, probe, twiddles
+  float2* gx;               // adjoint state, same layout as the forward state
+  const float2* xprev;      // forward state X_{s-1} (rows sweep, s > 0) or X_{S-1} (last column sweep)
+  const float* target;      // [nprobe_total][nx][ny] target intensities, FFTW order (last column sweep)
+  float* loss_partial;      // [nprobe][ntiles] (last column sweep)
+  float* gT;                // [potNx][potNy][2] gradient w.r.t. the transmission slice (atomics)
+  float* gprobe;            // [nx][ny][2] sum over probes of the gradient w.r.t. psi_0 (atomics)
+};
+
 struct ProbeArgs {
   int nx, ny;
   double ax, by, rx, ry, rx2, ry2;
@@ -98,5 +109,15 @@ int probe_window_blocks(int n);
 void launch_probe_window(const ProbeArgs& a, float2* psi, double* partial, hipStream_t st);
 void launch_fill_zero(float2* p, size_t n, hipStream_t st);
 hipError_t set_col_lds_limits();
+void launch_adj_rows(const AdjArgs& a, bool first_slice, hipStream_t st);
+void launch_adj_cols(const AdjArgs& a, bool last_slice, hipStream_t st);
+void launch_grad_v(const float2* gu, const float* V, float* gV, size_t n, double sigma, hipStream_t st);
+void launch_target_outside_band(const float* target, int nprobe, int nx, int ny, int band_a, int band_b,
+                                double* out, hipStream_t st);
+void launch_probe_grad(const ProbeArgs& a, const float2* G, double cscale, double wavlen_p, double* out2,
+                       hipStream_t st);
+void launch_atom_grad(const QsAtomDev* atoms, int natom, const float* const* luts, const int* kind,
+                      const ScatterArgs& s, const float* gV, float* grad, hipStream_t st);
+void launch_cplx_fill_zero(float* p, size_t nfloats, hipStream_t st);
 
 }  // namespace qs

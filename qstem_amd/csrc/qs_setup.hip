@@ -161,7 +161,9 @@ __global__ void k_fill_zero(float2* __restrict__ p, size_t n) {
   hipError_t set_col_lds_##N();                                                                     \
   int col_tile_##N();                                                                               \
   void rows_fft_##N(float2*, long, int, const float2*, float, hipStream_t);                         \
-  void bluestein_##N(float2*, long, int, const float2*, const float2*, const float2*, float, hipStream_t);
+  void bluestein_##N(float2*, long, int, const float2*, const float2*, const float2*, float, hipStream_t); \
+  void adj_rows_##N(const AdjArgs&, bool, hipStream_t);                                             \
+  void adj_cols_##N(const AdjArgs&, bool, hipStream_t);
 QS_DECL(64)
 QS_DECL(128)
 QS_DECL(256)
@@ -223,6 +225,18 @@ void launch_rows_bluestein(float2* d, long nrows, int n, int M, const float2* ch
                            const float2* tw, float scale, hipStream_t st) {
 #define C(N) bluestein_##N(d, nrows, n, chirp, vhat, tw, scale, st)
   QS_SWITCH(M, C)
+#undef C
+}
+
+void launch_adj_rows(const AdjArgs& a, bool first_slice, hipStream_t st) {
+#define C(N) adj_rows_##N(a, first_slice, st)
+  QS_SWITCH(a.f.ny, C)
+#undef C
+}
+
+void launch_adj_cols(const AdjArgs& a, bool last_slice, hipStream_t st) {
+#define C(N) adj_cols_##N(a, last_slice, st)
+  QS_SWITCH(a.f.nx, C)
 #undef C
 }
 

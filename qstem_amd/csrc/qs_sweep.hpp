@@ -307,4 +307,190 @@ void blue_t(float2* d, long nrows, int n, const float2* chirp, const float2* vha
                      chirp, vhat, tw, scale);
 }
 
+
+// ============================================================================
+// Adjoint sweeps (reverse mode of the two forward sweeps).
+//
+// Column sweep (slice s, reverse):  g_Phi = colFFT(g_X)  [last slice: 4 (I - T) Phi,
+// Phi = colFFT(X_{S-1})/nx, I = |Phi|^2]  ->  g_Psi = conj(P) g_Phi  ->  g_R = colIFFT(g_Psi).
+template <int NX, bool LAST>
+__global__ __launch_bounds__(kBlock) void k_adj_cols(AdjArgs A) {
+  const SweepArgs& a = A.f;
+  constexpr int E = Plan<NX>::E;
+  constexpr int T = NX / E;
+  constexpr int TC = ColCfg<NX>::TC;
+  constexpr int ST = ColCfg<NX>::STRIDE;
+  constexpr int G = kBlock / T;
+  constexpr int ROUNDS = (TC + G - 1) / G;
+  extern __shared__ float2 tile[];
+  __shared__ float red;
+  const int p = blockIdx.x / a.ntiles;
+  const int tile_id = blockIdx.x % a.ntiles;
+  const int pg = a.probe0 + p;
+  const int c0 = tile_id * TC;
+  const size_t off = (size_t)pg * a.nx * a.kcp + c0;
+  const float2* src = (LAST ? A.xprev : A.gx) + off;
+  float2* dst = A.gx + off;
+  if (threadIdx.x == 0) red = 0.f;
+  for (int e = threadIdx.x; e < TC * NX; e += kBlock) {
+    const int c = e % TC, x = e / TC;
+    tile[c * ST + lpad(x)] = src[(size_t)x * a.kcp + c];
+  }
+  __syncthreads();
+  const int slot_id = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  Twiddles<NX> tw;
+  tw.init(a.twx, t);
+  float lsum = 0.f;
+  for (int r = 0; r < ROUNDS; r++) {
+    const int cl = r * G + slot_id;
+    if (cl >= TC) break;
+    float2* col = tile + cl * ST;
+    const int c = c0 + cl;
+    const int iy = c <= a.band_a ? c : (c < a.kc ? c - (a.band_a + 1) + (a.ny - a.band_b) : -1);
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) v[m] = col[lpad(t + T * m)];
+    fft_wave<NX, -1>(v, col, tw, t);
+    if (iy >= 0) {
+      const float ky2 = a.ky2[iy];
+      const float2 py = a.py[iy];
+      const float inx = 1.0f / (float)a.nx;
+#pragma unroll
+      for (int m = 0; m < E; m++) {
+        const int ix = t + T * m;
+        const float kx2 = a.kx2[ix];
+        const bool in = kx2 < a.k2max && (kx2 + ky2) < a.k2max;
+        float2 g;
+        if constexpr (LAST) {
+          const float2 phi = in ? cscale(v[m], inx) : make_float2(0.f, 0.f);
+          const float d = cnorm(phi) - A.target[((size_t)pg * a.nx + ix) * a.ny + iy];
+          lsum += d * d;
+          g = cscale(phi, 4.0f * d);
+        } else {
+          g = v[m];
+        }
+        v[m] = in ? cmulc(cmulc(g, py), a.px[ix]) : make_float2(0.f, 0.f);
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
+    }
+    fft_wave<NX, +1>(v, col, tw, t);
+    wave_lds_sync();
+#pragma unroll
+    for (int m = 0; m < E; m++) col[lpad(t + T * m)] = v[m];
+  }
+  if constexpr (LAST) atomicAdd(&red, lsum);
+  __syncthreads();
+  for (int e = threadIdx.x; e < TC * NX; e += kBlock) {
+    const int c = e % TC, x = e / TC;
+    dst[(size_t)x * a.kcp + c] = tile[c * ST + lpad(x)];
+  }
+  if constexpr (LAST) {
+    if (threadIdx.x == 0) A.loss_partial[(size_t)p * a.ntiles + tile_id] = red;
+  }
+}
+
+// Row sweep (slice s, reverse):  g_a = rowIFFT(g_R);  psi_s = rowIFFT(X_{s-1}) (or the
+// probe);  g_t += conj(psi_s) g_a;  g_psi = conj(t_s) g_a;  g_X_{s-1} = rowFFT(g_psi)
+// (s > 0) or sum_p g_psi -> gprobe (s = 0).
+template <int NY, bool FIRST>
+__global__ __launch_bounds__(kBlock) void k_adj_rows(AdjArgs A) {
+  const SweepArgs& a = A.f;
+  constexpr int E = Plan<NY>::E;
+  constexpr int T = NY / E;
+  constexpr int RPB = kBlock / T;
+  __shared__ float2 lds[RPB * lds_slot<NY>()];
+  const int slot_id = threadIdx.x / T;
+  const int t = threadIdx.x % T;
+  const long row = (long)blockIdx.x * RPB + slot_id;
+  if (row >= (long)a.nprobe * a.nx) return;
+  const int p = (int)(row / a.nx);
+  const int x = (int)(row % a.nx);
+  const int pg = a.probe0 + p;
+  float2* slot = lds + slot_id * lds_slot<NY>();
+  Twiddles<NY> tw;
+  tw.init(a.twy, t);
+  float2* grow = A.gx + ((size_t)pg * a.nx + x) * a.kcp;
+  float2 g[E], w[E];
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const int iy = t + T * m;
+    const int c = iy <= a.band_a ? iy : (iy >= NY - a.band_b ? iy - (NY - a.band_b) + a.band_a + 1 : -1);
+    g[m] = c >= 0 ? grow[c] : make_float2(0.f, 0.f);
+  }
+  fft_wave<NY, +1>(g, slot, tw, t);
+  if constexpr (FIRST) {
+    const float2* pr = a.probe + (size_t)x * NY;
+#pragma unroll
+    for (int m = 0; m < E; m++) w[m] = pr[t + T * m];
+  } else {
+    const float2* xr = A.xprev + ((size_t)pg * a.nx + x) * a.kcp;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const int iy = t + T * m;
+      const int c = iy <= a.band_a ? iy : (iy >= NY - a.band_b ? iy - (NY - a.band_b) + a.band_a + 1 : -1);
+      w[m] = c >= 0 ? xr[c] : make_float2(0.f, 0.f);
+    }
+    fft_wave<NY, +1>(w, slot, tw, t);
+  }
+  const int2 ip = a.ipos[pg];
+  const float2* tr = a.trans + (size_t)(x + ip.x) * a.potNy + ip.y;
+  float* gt = A.gT + 2 * ((size_t)(x + ip.x) * a.potNy + ip.y);
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    const int iy = t + T * m;
+    const float2 gtv = cmulc(g[m], w[m]);  // conj(psi) * g_a
+    atomicAdd(gt + 2 * iy, gtv.x);
+    atomicAdd(gt + 2 * iy + 1, gtv.y);
+    g[m] = cmulc(g[m], tr[iy]);  // conj(t) * g_a
+  }
+  if constexpr (FIRST) {
+    float* gp = A.gprobe + 2 * (size_t)x * NY;
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      atomicAdd(gp + 2 * (t + T * m), g[m].x);
+      atomicAdd(gp + 2 * (t + T * m) + 1, g[m].y);
+    }
+  } else {
+    fft_wave<NY, -1>(g, slot, tw, t);
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const int iy = t + T * m;
+      const int c = iy <= a.band_a ? iy : (iy >= NY - a.band_b ? iy - (NY - a.band_b) + a.band_a + 1 : -1);
+      if (c >= 0) grow[c] = g[m];
+    }
+  }
+}
+
+template <int N>
+void launch_adj_rows_t(const AdjArgs& a, bool first, hipStream_t st) {
+  constexpr int T = N / Plan<N>::E;
+  constexpr int RPB = kBlock / T;
+  const long rows = (long)a.f.nprobe * a.f.nx;
+  dim3 grid((unsigned)((rows + RPB - 1) / RPB));
+  if (first) hipLaunchKernelGGL((k_adj_rows<N, true>), grid, dim3(kBlock), 0, st, a);
+  else hipLaunchKernelGGL((k_adj_rows<N, false>), grid, dim3(kBlock), 0, st, a);
+}
+
+template <int N>
+void launch_adj_cols_t(const AdjArgs& a, bool last, hipStream_t st) {
+  const size_t lds = col_lds_bytes_t<N>();
+  dim3 grid((unsigned)(a.f.nprobe * a.f.ntiles));
+  if (last) hipLaunchKernelGGL((k_adj_cols<N, true>), grid, dim3(kBlock), lds, st, a);
+  else hipLaunchKernelGGL((k_adj_cols<N, false>), grid, dim3(kBlock), lds, st, a);
+}
+
+template <int N>
+hipError_t set_adj_lds_t() {
+  const int b = col_lds_bytes_t<N>();
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, true>),
+                                     hipFuncAttributeMaxDynamicSharedMemorySize, b);
+  if (e == hipSuccess)
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, false>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, b);
+  return e;
+}
+
 }  // namespace qs

@@ -222,6 +222,40 @@ class Multislice:
         self._check(self._lib.qs_make_probe(self._h, _fptr(out.view(np.float32))), "qs_make_probe")
         return out
 
+    # -- adjoint ----------------------------------------------------------------------
+    def adjoint(self, ipos: np.ndarray, target: np.ndarray, ckpt_every: int = 0, want_grad_V: bool = True,
+                want_grad_probe: bool = True) -> dict:
+        """Loss and gradients of L = sum (I - target)^2 over all slices (qs_adjoint)."""
+        ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        assert target.shape == (ipos.shape[0], self.cfg.nx, self.cfg.ny), target.shape
+        loss = np.zeros(1, np.float32)
+        gV = np.empty((self._nslice, self.cfg.potNx, self.cfg.potNy), np.float32) if want_grad_V else None
+        gp = np.zeros(2, np.float32) if want_grad_probe else None
+        rc = self._lib.qs_adjoint(self._h, _iptr(ipos), ipos.shape[0], _fptr(target), int(ckpt_every), _fptr(loss),
+                                  _fptr(gV), _fptr(gp))
+        self._check(rc, "qs_adjoint")
+        out = {"loss": float(loss[0])}
+        if gV is not None:
+            out["grad_V"] = gV
+        if gp is not None:
+            out["grad_df0"], out["grad_Cs"] = float(gp[0]), float(gp[1])
+        return out
+
+    def atom_gradients(self, atoms, atomRadius=5.0, potOffsetX=0.0, potOffsetY=0.0, czOffset=0.0, centerSlices=0,
+                       periodicXY=0, tds=0, compat_q6=0, lut_dr_fix=0) -> np.ndarray:
+        """dL/d(x, y, z) per atom from the last adjoint (same atoms as build_potential)."""
+        atoms = np.asarray(atoms, dtype=np.float64).reshape(-1, 7)
+        arr = (_lib.QsAtom * max(len(atoms), 1))()
+        for i, a in enumerate(atoms):
+            arr[i] = _lib.QsAtom(*[float(v) for v in a[:6]], int(a[6]))
+        pp = _lib.QsPotParams(atomRadius, potOffsetX, potOffsetY, czOffset, int(centerSlices), int(periodicXY),
+                              int(tds), int(compat_q6), int(lut_dr_fix))
+        g = np.zeros((len(atoms), 3), np.float32)
+        self._check(self._lib.qs_atom_gradients(self._h, arr, len(atoms), ctypes.byref(pp), _fptr(g)),
+                    "qs_atom_gradients")
+        return g
+
     # -- multislice ------------------------------------------------------------------
     def set_batch(self, batch: int):
         self._check(self._lib.qs_set_batch(self._h, int(batch)), "qs_set_batch")
