@@ -72,6 +72,58 @@ def synthetic_potential(cfg, potoff):
     return np.concatenate([V] * (cfg.slices // 2), axis=0)
 
 
+def adjoint_workload(steps=3):
+    """4D-STEM adjoint workload: 32x32 scan, 256x256 wave (dx = 0.05 A), 200 kV, 20 slices of
+    seeded Gaussian-filtered random potentials (sigma 0.5 A, max phase 0.3 rad); loss = L2
+    between simulated and target diffraction intensities.  Returns the timing record."""
+    import torch
+    from qstem_amd import Multislice, StemConfig
+    n, dx, S = 256, 0.05, 20
+    step_px = 4
+    pot = n + 32 * step_px
+    cfg = StemConfig(nx=n, ny=n, resolutionX=dx, resolutionY=dx, v0=200.0, slices=S, sliceThickness=2.0,
+                     potNx=pot, potNy=pot, bandlimittrans=1, Cs=1.0e4, df0=-50.0, alpha=20.0, electronScale=1.0)
+    rng = np.random.default_rng(7)
+    noise = rng.standard_normal((S, pot, pot))
+    k = np.fft.fftfreq(pot, d=dx)
+    g = np.exp(-2 * (np.pi * 0.5) ** 2 * (k[:, None] ** 2 + k[None, :] ** 2))
+    V = np.real(np.fft.ifft2(np.fft.fft2(noise, axes=(1, 2)) * g, axes=(1, 2)))
+    sigma = (1 + 200.0 / 511.0) * 12.3984244 / np.sqrt(200.0 * (2 * 510.99906 + 200.0))
+    V = (V * (0.3 / sigma) / np.abs(V).max()).astype(np.float32)
+    ix, iy = np.meshgrid(np.arange(32) * step_px, np.arange(32) * step_px, indexing="ij")
+    ipos = np.stack([ix.ravel(), iy.ravel()], 1).astype(np.int32)
+    ms = Multislice(cfg)
+    ms.set_potential(1.05 * V)
+    ms.make_probe()
+    d = ms.run_stem(ipos, want_det=False, want_diff=True)["diff"]
+    target = (np.fft.ifftshift(d, axes=(1, 2)) / float(n * n) ** 1.5).astype(np.float32)
+    ms.set_potential(V)
+    ms.make_probe()
+    ms.run_stem(ipos, want_det=False, want_diff=True)
+    ms.adjoint(ipos, target)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ms.run_stem(ipos, want_det=False, want_diff=True)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - t0) / steps
+    res = {}
+    for k_ck in (0, 1):
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            out = ms.adjoint(ipos, target, ckpt_every=k_ck)
+        torch.cuda.synchronize()
+        res[k_ck] = (time.perf_counter() - t0) / steps
+    ms.close()
+    return {"workload": "4D-STEM 32x32 scan, 256x256, 20 slices, L2 on diffraction intensities",
+            "forward_ms": round(fwd * 1e3, 2), "adjoint_ms": round(res[0] * 1e3, 2),
+            "adjoint_over_forward": round(res[0] / fwd, 3), "ckpt_every": int(np.ceil(np.sqrt(S))),
+            "adjoint_ms_no_recompute": round(res[1] * 1e3, 2),
+            "adjoint_over_forward_no_recompute": round(res[1] / fwd, 3),
+            "note": "adjoint = one call returning loss, dL/dV, dL/d(df, Cs): forward with checkpoints, "
+                    "segment recompute, reverse sweeps"}
+
+
 def _oracle_worker(args):
     cfg_d, trans_path, probe_path, ipos, bins, nbins = args
     sys.path.insert(0, ROOT)
@@ -206,6 +258,9 @@ def main():
                      "kernel": "slice step = k_sweep_rows<512,1> + k_sweep_cols<512>",
                      "bytes_per_probe_slice": bytes_per_probe_slice},
     }
+
+    if not a.no_adjoint:
+        result["adjoint"] = adjoint_workload()
 
     if a.cpu_sample > 0:
         ns = min(a.cpu_sample, len(os.sched_getaffinity(0)), 16)
