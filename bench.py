@@ -162,7 +162,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=16)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--batch", type=int, default=256, help="probes per launch group")
+    ap.add_argument("--batch", type=int, default=256, help="probes per qs_run_stem call (one step)")
+    ap.add_argument("--group", type=int, default=0, help="probes advanced together inside the library (0 = auto)")
     ap.add_argument("--loop-probes", type=int, default=8, help="probes for the per-probe loop baseline")
     ap.add_argument("--cpu-sample", type=int, default=8, help="probes for the oracle CPU baseline (0 = skip)")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -186,7 +187,7 @@ def main():
     t0 = time.perf_counter()
     psi0 = ms.make_probe()
     t_probe = time.perf_counter() - t0
-    ms.set_batch(P)
+    ms.set_batch(a.group)
 
     nbatches = (ipos_all.shape[0] + P - 1) // P
 
@@ -218,15 +219,17 @@ def main():
     loop_out = ms.run_stem(sl, bins=bins, nbins=nbins)
     torch.cuda.synchronize()
     loop_rate = a.loop_probes * nslice / (time.perf_counter() - t0)
-    ms.set_batch(P)
+    ms.set_batch(a.group)
 
     # roofline of the slice step (row sweep + column sweep, one logical kernel in two launches)
     N2 = cfg.nx * cfg.ny
     W = cfg.potNx * cfg.potNy
     ndet = len(cfg.detectors)
-    bytes_per_probe_slice = 16.0 * N2 + 8.0 * W / P + 4.0 * ndet
     t_step_ms = ms_rows / max(n_rows, 1) + ms_cols / max(n_cols, 1)
-    achieved = bytes_per_probe_slice * P / (t_step_ms * 1e-3) / 1e9
+    G = n_rows and round(P * a.steps * nslice / n_rows)  # probes per kernel launch inside the library
+    # algorithmic bytes: wave in + out, the union transmission window read once per launch, detector sums
+    bytes_per_probe_slice = 16.0 * N2 + 8.0 * W / G + 4.0 * ndet
+    achieved = bytes_per_probe_slice * G / (t_step_ms * 1e-3) / 1e9
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic_sweeps_512.json")
     if os.path.exists(tp):
@@ -248,7 +251,8 @@ def main():
         "data": "synthetic: perovskite-window projected potential (Gaussian atom columns), seeded positions",
         "config": {"workload": "batched plain STEM, perovskite 12x12x40 cells, 80 slices, 512x512 wave, "
                                "dx=0.030 A, 300 kV, 25 mrad, Cs=1 um, df=-5 nm, 64x64 scan, BF/ABF/ADF",
-                   "probes_per_launch": P, "slices": nslice, "nx": cfg.nx, "potN": cfg.potNx},
+                   "probes_per_launch": P, "probes_per_kernel_launch": G, "slices": nslice, "nx": cfg.nx,
+                   "potN": cfg.potNx},
         "per_probe_loop": {"value": round(loop_rate, 1), "unit": "probe-slices/s", "probes": a.loop_probes},
         "batched_over_per_probe": round(value / loop_rate, 2),
         "kernel_ms": {"rows_sweep": round(ms_rows / max(n_rows, 1), 4), "cols_sweep": round(ms_cols / max(n_cols, 1), 4)},
@@ -272,7 +276,7 @@ def main():
             det_o, wall, per = cpu_baseline(cfg, trans, psi0, ipos_all[:ns], bins, nbins, ns, tmpdir)
         finally:
             shutil.rmtree(tmpdir, ignore_errors=True)
-        ms.set_batch(P)
+        ms.set_batch(a.group)
         det_g = ms.run_stem(ipos_all[:ns], bins=bins, nbins=nbins)["det"]
         errs = []
         for b in range(nbins):

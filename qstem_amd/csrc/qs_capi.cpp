@@ -709,7 +709,13 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
       if (bin_of_slice[s] >= nbins) return fail(c, QS_ERR_ARG, "qs_run_stem: bin index >= nbins");
   }
   const int nx = p.nx, ny = p.ny;
-  const int B = c->batch > 0 ? (c->batch < nprobe ? c->batch : nprobe) : (nprobe < 256 ? nprobe : 256);
+  // probe group advanced through all slices together: the library default keeps the
+  // group's wave state (two sweeps per slice read and write it) inside the 256 MB
+  // Infinity Cache, measured best at ~180 MB for 512x512 waves
+  const size_t per_probe = sizeof(float2) * (size_t)nx * c->kcp;
+  int Bdef = (int)std::max<size_t>(32, std::min<size_t>(1024, (size_t)(180u << 20) / per_probe));
+  Bdef = std::max(32, (Bdef / 16) * 16);
+  const int B = c->batch > 0 ? std::min(c->batch, nprobe) : std::min(Bdef, nprobe);
   const size_t state_elems = (size_t)nprobe * nx * c->kcp;
   HIPCHK(c, c->state.ensure(sizeof(float2) * state_elems));
   if (start == 0) qs::launch_fill_zero(c->state.as<float2>(), state_elems, c->stream);  // padding columns stay zero

@@ -88,6 +88,29 @@ template <int NX> struct ColCfg {
   static constexpr int STRIDE = ((lds_slot<NX>() + 31) / 32) * 32 + 2;  // float2 per LDS column
 };
 
+// Tile staging between the row-major state (TC contiguous complex per row) and
+// the column-major LDS tile; 16-byte accesses (two columns per lane).
+template <int NX>
+__device__ __forceinline__ void stage_tile_in(float2* tile, const float2* __restrict__ base, int kcp) {
+  constexpr int TC = ColCfg<NX>::TC, ST = ColCfg<NX>::STRIDE, H = TC / 2;
+  for (int e = threadIdx.x; e < H * NX; e += kBlock) {
+    const int c = 2 * (e % H), x = e / H;
+    const float4 q = *reinterpret_cast<const float4*>(base + (size_t)x * kcp + c);
+    tile[c * ST + lpad(x)] = make_float2(q.x, q.y);
+    tile[(c + 1) * ST + lpad(x)] = make_float2(q.z, q.w);
+  }
+}
+
+template <int NX>
+__device__ __forceinline__ void stage_tile_out(const float2* tile, float2* __restrict__ base, int kcp) {
+  constexpr int TC = ColCfg<NX>::TC, ST = ColCfg<NX>::STRIDE, H = TC / 2;
+  for (int e = threadIdx.x; e < H * NX; e += kBlock) {
+    const int c = 2 * (e % H), x = e / H;
+    const float2 u = tile[c * ST + lpad(x)], w = tile[(c + 1) * ST + lpad(x)];
+    *reinterpret_cast<float4*>(base + (size_t)x * kcp + c) = make_float4(u.x, u.y, w.x, w.y);
+  }
+}
+
 template <int NX, bool REC, bool DIFF>
 __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
   constexpr int E = Plan<NX>::E;
@@ -108,17 +131,21 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
   if constexpr (REC) {
     if (threadIdx.x <= kMaxDet) red[threadIdx.x] = 0.f;
   }
-  // stage the tile: rows of TC contiguous complex values
-  for (int e = threadIdx.x; e < TC * NX; e += kBlock) {
-    const int c = e % TC, x = e / TC;
-    tile[c * ST + lpad(x)] = base[(size_t)x * a.kcp + c];
-  }
+  stage_tile_in<NX>(tile, base, a.kcp);
   __syncthreads();
 
   const int slot_id = threadIdx.x / T;
   const int t = threadIdx.x % T;
   Twiddles<NX> tw;
   tw.init(a.twx, t);
+  // per-lane propagator factors and k^2 (independent of the column)
+  float2 pxr[E];
+  float kx2r[E];
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    pxr[m] = a.px[t + T * m];
+    kx2r[m] = a.kx2[t + T * m];
+  }
   for (int r = 0; r < ROUNDS; r++) {
     const int cl = r * G + slot_id;
     if (cl >= TC) break;  // only when G does not divide TC (uniform per transform)
@@ -135,9 +162,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
       const float2 py = a.py[iy];
 #pragma unroll
       for (int m = 0; m < E; m++) {
-        const int ix = t + T * m;
-        const float kx2 = a.kx2[ix];
-        if (kx2 < a.k2max && (kx2 + ky2) < a.k2max) v[m] = cmul(cmul(v[m], py), a.px[ix]);
+        const float kx2 = kx2r[m];
+        if (kx2 < a.k2max && (kx2 + ky2) < a.k2max) v[m] = cmul(cmul(v[m], py), pxr[m]);
         else v[m] = make_float2(0.f, 0.f);
       }
       if constexpr (REC) {
@@ -180,10 +206,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
     for (int m = 0; m < E; m++) col[lpad(t + T * m)] = v[m];
   }
   __syncthreads();
-  for (int e = threadIdx.x; e < TC * NX; e += kBlock) {
-    const int c = e % TC, x = e / TC;
-    base[(size_t)x * a.kcp + c] = tile[c * ST + lpad(x)];
-  }
+  stage_tile_out<NX>(tile, base, a.kcp);
   if constexpr (REC) {
     if (threadIdx.x <= kMaxDet) a.partial[((size_t)p * a.ntiles + tile_id) * (kMaxDet + 1) + threadIdx.x] = red[threadIdx.x];
   }
@@ -332,10 +355,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cols(AdjArgs A) {
   const float2* src = (LAST ? A.xprev : A.gx) + off;
   float2* dst = A.gx + off;
   if (threadIdx.x == 0) red = 0.f;
-  for (int e = threadIdx.x; e < TC * NX; e += kBlock) {
-    const int c = e % TC, x = e / TC;
-    tile[c * ST + lpad(x)] = src[(size_t)x * a.kcp + c];
-  }
+  stage_tile_in<NX>(tile, src, a.kcp);
   __syncthreads();
   const int slot_id = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -383,10 +403,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cols(AdjArgs A) {
   }
   if constexpr (LAST) atomicAdd(&red, lsum);
   __syncthreads();
-  for (int e = threadIdx.x; e < TC * NX; e += kBlock) {
-    const int c = e % TC, x = e / TC;
-    dst[(size_t)x * a.kcp + c] = tile[c * ST + lpad(x)];
-  }
+  stage_tile_out<NX>(tile, dst, a.kcp);
   if constexpr (LAST) {
     if (threadIdx.x == 0) A.loss_partial[(size_t)p * a.ntiles + tile_id] = red;
   }
