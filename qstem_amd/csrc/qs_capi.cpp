@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <complex>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -104,6 +105,8 @@ struct qs_ctx {
   float last_ms = 0.f;
   std::string err;
   int batch = 0;
+  int rows_per_wave = 0;  // cap on row-sweep steps per wavefront (0: kernel default)
+  int tiles_per_block = 0;  // column-sweep tiles per block (0: launcher default)
   // derived tables
   int band_a = 0, band_b = 0, kc = 0, kcp = 0, ntiles = 0, tc = 0;
   float k2max = 0.f;
@@ -228,6 +231,8 @@ int qs_create(const qs_params* p, qs_ctx** out) {
     c->err = "hipStreamCreate/hipEventCreate failed (no HIP device?)";
     return bad(QS_ERR_HIP);
   }
+  if (const char* e = getenv("QS_ROWS_PER_WAVE")) c->rows_per_wave = atoi(e);  // tuning overrides
+  if (const char* e = getenv("QS_TILES_PER_BLOCK")) c->tiles_per_block = atoi(e);
   if (qs::set_col_lds_limits() != hipSuccess) {
     c->err = "hipFuncSetAttribute for the column sweep LDS size failed";
     return bad(QS_ERR_HIP);
@@ -267,7 +272,9 @@ int qs_create(const qs_params* p, qs_ctx** out) {
   c->kc = a + 1 + b;
   c->tc = qs::col_tile(nx);
   c->ntiles = (c->kc + c->tc - 1) / c->tc;
-  c->kcp = c->ntiles * c->tc;
+  // row pitch of the band-compact state: whole 128-byte lines, so that the two
+  // halves of a line are adjacent column tiles (one block advances both)
+  c->kcp = (c->ntiles * c->tc + 15) / 16 * 16;
   // propagator e^{-i pi dz lambda k^2}, double -> float, 1/(nx ny) folded into px
   const double wl = wavelength(p->v0), dz = p->sliceThickness;
   const double axd = (double)p->resolutionX * nx, byd = (double)p->resolutionY * ny;
@@ -744,6 +751,8 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
   a.band_b = c->band_b;
   a.ntiles = c->ntiles;
   a.potNy = p.potNy;
+  a.rw = c->rows_per_wave;
+  a.tpb = c->tiles_per_block;
   a.probe = c->probe.as<float2>();
   a.ipos = c->ipos.as<int2>();
   a.state = c->state.as<float2>();
@@ -839,6 +848,8 @@ static void sweep_args(qs_ctx* c, SweepArgs& a, int nprobe) {
   a.nx = p.nx; a.ny = p.ny; a.nprobe = nprobe; a.probe0 = 0;
   a.kc = c->kc; a.kcp = c->kcp; a.band_a = c->band_a; a.band_b = c->band_b; a.ntiles = c->ntiles;
   a.potNy = p.potNy;
+  a.rw = c->rows_per_wave;
+  a.tpb = c->tiles_per_block;
   a.probe = c->probe.as<float2>();
   a.ipos = c->ipos.as<int2>();
   a.twx = c->twx.as<float2>(); a.twy = c->twy.as<float2>();

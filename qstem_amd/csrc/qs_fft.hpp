@@ -1,11 +1,12 @@
 // qs_fft.hpp -- wavefront-resident Stockham FFT building blocks for gfx950.
 //
-// A length-N transform is owned by T = N/E consecutive lanes of one 64-wide
-// wavefront; each lane holds E complex values in registers.  In the natural
-// layout lane t holds elements t + T*m, m = 0..E-1, which is also the layout
-// coalesced global loads and stores produce, so the first pass takes its
-// input straight from registers and the last pass leaves its output there.
-// Between passes the values are exchanged through a per-transform LDS slot.
+// A length-N transform is owned by T = N/E consecutive threads: lanes of one
+// 64-wide wavefront up to N = 1024, a whole 256-thread block beyond; each
+// thread holds E complex values in registers.  In the natural layout thread t
+// holds elements t + T*m, m = 0..E-1, which is also the layout coalesced
+// global loads and stores produce, so the first pass takes its input straight
+// from registers and the last pass leaves its output there.  Between passes
+// the values are exchanged through a per-transform LDS slot.
 //
 // Pass structure (Stockham autosort): for radix R and span Ns, butterfly
 // j = t + T*b (b < E/R) reads x[j + k*N/R], applies twiddles
@@ -121,20 +122,41 @@ __device__ __forceinline__ void dft(float2* a) {
 }
 
 // ---- radix plans -------------------------------------------------------------
-// E: values per lane; R0,R1,R2: radices of the three passes (each divides E).
+// E: values per thread; T = N/E threads per transform (one wavefront up to
+// N = 1024, the whole 256-thread block for 2048 and 4096); NP passes of radix
+// R0..R3 (each divides E).
 template <int N> struct Plan;
-template <> struct Plan<64>   { static constexpr int E = 4,  R0 = 4,  R1 = 4,  R2 = 4; };
-template <> struct Plan<128>  { static constexpr int E = 8,  R0 = 8,  R1 = 8,  R2 = 2; };
-template <> struct Plan<256>  { static constexpr int E = 8,  R0 = 8,  R1 = 8,  R2 = 4; };
-template <> struct Plan<512>  { static constexpr int E = 8,  R0 = 8,  R1 = 8,  R2 = 8; };
-template <> struct Plan<1024> { static constexpr int E = 16, R0 = 16, R1 = 8,  R2 = 8; };
-template <> struct Plan<2048> { static constexpr int E = 32, R0 = 16, R1 = 16, R2 = 8; };
-template <> struct Plan<4096> { static constexpr int E = 64, R0 = 16, R1 = 16, R2 = 16; };
+template <> struct Plan<64>   { static constexpr int E = 4,  NP = 3, R0 = 4,  R1 = 4,  R2 = 4,  R3 = 1; };
+template <> struct Plan<128>  { static constexpr int E = 8,  NP = 3, R0 = 8,  R1 = 8,  R2 = 2,  R3 = 1; };
+template <> struct Plan<256>  { static constexpr int E = 8,  NP = 3, R0 = 8,  R1 = 8,  R2 = 4,  R3 = 1; };
+template <> struct Plan<512>  { static constexpr int E = 8,  NP = 3, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 1; };
+template <> struct Plan<1024> { static constexpr int E = 16, NP = 3, R0 = 16, R1 = 8,  R2 = 8,  R3 = 1; };
+template <> struct Plan<2048> { static constexpr int E = 8,  NP = 4, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 4; };
+template <> struct Plan<4096> { static constexpr int E = 16, NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1; };
 
-// LDS slot padding: one float2 every 16 breaks the power-of-two strides of the
-// Stockham scatter across banks.
-__device__ __forceinline__ int lpad(int i) { return i + (i >> 4); }
-template <int N> constexpr int lds_slot() { return N + N / 16; }
+// LDS placement of element i of a length-N transform.  On gfx950 a
+// ds_read_b64 serves 32 lanes at a time over 32 eight-byte bank pairs and a
+// ds_write_b64 16 lanes over 16.  Up to N = 512 the low four index bits are
+// XORed with a function of bits 4.. (i ^ f_N(i >> 4)): every read and write
+// pattern of fft_wave below, and the natural-order column accesses of the
+// sweeps, then hit distinct bank pairs (sizes 64 and 128 also need the slot
+// strides of lds_slot, since several transforms share a wavefront there).
+// Beyond 512 one float2 of padding every 16 (i + (i >> 4)) keeps every
+// address an immediate offset from a per-thread base (the XOR form would hold
+// one address register per access), at a residual 1.3-1.5-way conflict rate.
+template <int N>
+__device__ __forceinline__ int lidx(int i) {
+  if constexpr (N >= 1024) {
+    return i + (i >> 4);
+  } else {
+    const int j = i >> 4;
+    int f;
+    if constexpr (N == 64) f = ((j & 1) ? 9 : 0) ^ (((j ^ (j >> 1)) & 1) ? 6 : 0);
+    else f = (j & 7) | (((j >> 2) & 1) << 3);
+    return i ^ f;
+  }
+}
+template <int N> constexpr int lds_slot() { return N == 64 ? 80 : (N == 128 ? 144 : (N >= 1024 ? N + N / 16 : N)); }
 
 // Orders this lane's LDS reads before later LDS writes of other lanes of the
 // same wavefront and vice versa (all lanes of a transform live in one wave).
@@ -144,9 +166,18 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-// Per-lane twiddles of the two twiddled passes.  For small plans they are
-// loaded once into registers (init) and reused by every transform the lane
-// performs; larger plans read the table on every pass.
+// LDS ordering between the threads of one length-N transform: a wavefront
+// barrier when the transform lives in one wavefront, else a block barrier
+// (the block is the transform; every thread of it reaches every call).
+template <int N>
+__device__ __forceinline__ void xform_sync() {
+  if constexpr (N / Plan<N>::E <= 64) wave_lds_sync();
+  else __syncthreads();
+}
+
+// Per-thread twiddles of the twiddled passes.  For small three-pass plans they
+// are loaded once into registers (init) and reused by every transform the
+// thread performs; larger plans read the table on every pass.
 template <int N>
 struct Twiddles {
   using P = Plan<N>;
@@ -154,7 +185,7 @@ struct Twiddles {
   static constexpr int NS1 = P::R0, NS2 = P::R0 * P::R1;
   static constexpr int NB1 = P::E / P::R1, NB2 = P::E / P::R2;
   static constexpr int C1 = NB1 * (P::R1 - 1), C2 = NB2 * (P::R2 - 1);
-  static constexpr bool REG = (C1 + C2) <= 16;
+  static constexpr bool REG = P::NP == 3 && (C1 + C2) <= 16;
   float2 w1[REG ? C1 : 1], w2[REG ? C2 : 1];
   const float2* tab;
   __device__ __forceinline__ void init(const float2* __restrict__ tw, int t) {
@@ -196,7 +227,7 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
 #pragma unroll
     for (int b = 0; b < NB; b++)
 #pragma unroll
-      for (int k = 0; k < R; k++) a[b][k] = slot[lpad(t + T * b + k * (N / R))];
+      for (int k = 0; k < R; k++) a[b][k] = slot[lidx<N>(t + T * b + k * (N / R))];
   }
   if constexpr (NS > 1) {
 #pragma unroll
@@ -217,20 +248,20 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
 #pragma unroll
       for (int k = 0; k < R; k++) v[b + k * NB] = a[b][k];
   } else {
-    wave_lds_sync();  // all earlier reads of the slot precede these writes
+    xform_sync<N>();  // all earlier reads of the slot precede these writes
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int j = t + T * b;
       const int base = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-      for (int k = 0; k < R; k++) slot[lpad(base + k * NS)] = a[b][k];
+      for (int k = 0; k < R; k++) slot[lidx<N>(base + k * NS)] = a[b][k];
     }
-    wave_lds_sync();
+    xform_sync<N>();
   }
 }
 
-// Full length-N transform of the lane-distributed vector v (natural layout in
-// and out).  slot: this transform's LDS region of lds_slot<N>() float2.
+// Full length-N transform of the thread-distributed vector v (natural layout
+// in and out).  slot: this transform's LDS region of lds_slot<N>() float2.
 template <int N, int DIR>
 __device__ __forceinline__ void fft_wave(float2* v, float2* slot, const Twiddles<N>& tw, int t) {
   using P = Plan<N>;
@@ -239,7 +270,12 @@ __device__ __forceinline__ void fft_wave(float2* v, float2* slot, const Twiddles
   constexpr bool REG = Twiddles<N>::REG;
   stockham_pass<N, P::R0, 1, DIR, true, false>(v, slot, nullptr, tw.tab, t);
   stockham_pass<N, P::R1, S1, DIR, false, false>(v, slot, REG ? tw.w1 : nullptr, tw.tab, t);
-  stockham_pass<N, P::R2, S2, DIR, false, true>(v, slot, REG ? tw.w2 : nullptr, tw.tab, t);
+  if constexpr (P::NP == 3) {
+    stockham_pass<N, P::R2, S2, DIR, false, true>(v, slot, REG ? tw.w2 : nullptr, tw.tab, t);
+  } else {
+    stockham_pass<N, P::R2, S2, DIR, false, false>(v, slot, nullptr, tw.tab, t);
+    stockham_pass<N, P::R3, S2 * P::R2, DIR, false, true>(v, slot, nullptr, tw.tab, t);
+  }
 }
 
 }  // namespace qs

@@ -16,6 +16,8 @@ struct SweepArgs {
   int band_a, band_b;  // band = iy in [0, band_a] U [ny-band_b, ny-1]
   int ntiles;          // column tiles per probe in sweep B
   int potNy;           // row pitch of the transmission slice
+  int rw;              // sweep A: transform steps per wavefront (set by the launcher)
+  int tpb;             // sweep B: column tiles per block (0: launcher default)
   const float2* probe;     // shared real-space probe [nx][ny] (sweep A mode 0)
   const float2* trans;     // transmission slice [potNx][potNy]
   const int2* ipos;        // window offset per global probe

@@ -25,58 +25,143 @@ constexpr int kBlock = 256;
 
 
 // ============================================================================
+// Buffer-resource (MUBUF) access with hardware range checking: a load at an
+// offset at or past the resource's byte size returns zero and a store there is
+// dropped.  kOut is past every resource built here (sizes stay below 2^31).
+using rsrc_t = __amdgpu_buffer_rsrc_t;
+constexpr unsigned kOut = 0x80000000u;
+
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, long bytes) {
+  const int n = bytes < 0x7fffffffL ? (int)bytes : 0x7fffffff;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, n, 0x00020000);
+}
+__device__ __forceinline__ float2 buf_load_c64(rsrc_t r, unsigned off) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+}
+__device__ __forceinline__ void buf_store_c64(float2 v, rsrc_t r, unsigned off) {
+  using U2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0u, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, off, 0, 0);
+}
+
+// Byte offset of ky column iy inside a band-compact row, or kOut when iy lies in
+// the zero gap (band_a, ny-band_b) between the two halves of the band;
+// gap = ny - band_b - band_a - 1.
+__device__ __forceinline__ unsigned band_off(int iy, int band_a, int gap) {
+  const int c = iy > band_a ? iy - gap : iy;
+  return (unsigned)(iy - band_a - 1) < (unsigned)gap ? kOut : (unsigned)c * 8u;
+}
+
+__device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+// ============================================================================
 // Sweep A: rows.
 // mode 0: input is the shared real-space probe (first slice of a fresh run)
 // mode 1: input is the band-compact state; row IFFT first
 // mode 2: exit: row IFFT of the state, write the real-space wave, no transmit
+// Each wavefront owns a chunk of TPW*a.rw consecutive rows of one probe
+// (TPW = 64/T transforms side by side, a.rw steps; the chunk divides nx), so
+// probe, window offset and row bases are wave-uniform scalars.  Twiddles and
+// band offsets are set up once per wavefront, and the loads of step i+1 are
+// in flight while step i is transformed.
 template <int NY, int MODE>
-__global__ __launch_bounds__(kBlock) void k_sweep_rows(SweepArgs a) {
+__global__ __launch_bounds__(kBlock, Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 ? 3 : 1) void k_sweep_rows(SweepArgs a) {
   constexpr int E = Plan<NY>::E;
   constexpr int T = NY / E;
-  constexpr int RPB = kBlock / T;  // rows per block
-  __shared__ float2 lds[RPB * lds_slot<NY>()];
-  const int slot_id = threadIdx.x / T;
-  const int t = threadIdx.x % T;
-  const long row = (long)blockIdx.x * RPB + slot_id;
-  if (row >= (long)a.nprobe * a.nx) return;  // whole transforms only: T divides 64
-  const int p = (int)(row / a.nx);
-  const int x = (int)(row % a.nx);
-  float2* slot = lds + slot_id * lds_slot<NY>();
-  float2 v[E];
-  const int pg = a.probe0 + p;  // global probe index (ipos, state)
+  static_assert(T <= 64 || T == kBlock, "a transform is a wavefront part or the whole block");
+  constexpr int TPW = T <= 64 ? 64 / T : 1;   // transforms side by side per unit
+  constexpr int UPB = T <= 64 ? kBlock / 64 : 1;  // units (wavefronts, or the block) per block
+  constexpr int SPB = kBlock / T;    // transform slots per block
+  constexpr bool BOFF = E <= 16;     // keep the band offsets in registers
+  __shared__ float2 lds[SPB * lds_slot<NY>()];
+  const int lane = T <= 64 ? (threadIdx.x & 63) : threadIdx.x;
+  const int sub = lane / T;
+  const int t = lane % T;
+  const int chunk = TPW * a.rw;
+  const int unit = T <= 64 ? (threadIdx.x >> 6) : 0;
+  const int row0 = wave_uniform((blockIdx.x * UPB + unit) * chunk);
+  const int nrows = a.nprobe * a.nx;
+  if (row0 >= nrows) return;         // whole wavefronts only (chunk divides nprobe*nx)
+  const int p = row0 / a.nx;
+  const int x0 = row0 - p * a.nx;    // first row of the chunk
+  const int pg = a.probe0 + p;       // global probe index (ipos, state)
+  float2* slot = lds + (threadIdx.x / T) * lds_slot<NY>();
+  const size_t srow0 = ((size_t)pg * a.nx + x0) * a.kcp;
+  const rsrc_t rs = make_rsrc(a.state + srow0, (long)(nrows - row0) * a.kcp * 8);
+  const unsigned rowb = (unsigned)a.kcp * 8u;
+  const int gap = NY - a.band_b - a.band_a - 1;
+  // per-lane byte offset of element m within step 0's rows (kOut outside the band);
+  // step i adds the wave-uniform i*TPW*rowb as the scalar offset
+  unsigned boff[BOFF ? E : 1];
+  auto bo = [&](int m) -> unsigned {
+    const unsigned o = band_off(t + T * m, a.band_a, gap);
+    return o == kOut ? kOut : o + (unsigned)sub * rowb;
+  };
+  if constexpr (BOFF) {
+#pragma unroll
+    for (int m = 0; m < E; m++) boff[m] = bo(m);
+  }
+  int2 ip = make_int2(0, 0);
+  if constexpr (MODE != 2) ip = a.ipos[pg];
+  const float2* tr0 = a.trans + (size_t)(x0 + ip.x) * a.potNy + ip.y;
+
+  auto load = [&](int i, float2* dv, float2* dt) {
+    const int xr = i * TPW + sub;  // row within the chunk
+    if constexpr (MODE != 2) {
+      const float2* tr = tr0 + (size_t)xr * a.potNy;
+#pragma unroll
+      for (int m = 0; m < E; m++) dt[m] = tr[t + T * m];
+    }
+    if constexpr (MODE == 0) {
+      const float2* src = a.probe + (size_t)(x0 + xr) * NY;
+#pragma unroll
+      for (int m = 0; m < E; m++) dv[m] = src[t + T * m];
+    } else {
+      const unsigned so = (unsigned)(i * TPW) * rowb;
+#pragma unroll
+      for (int m = 0; m < E; m++)
+        dv[m] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, BOFF ? boff[m] : bo(m), so, 0));
+    }
+  };
   Twiddles<NY> tw;
   tw.init(a.twy, t);
-  if constexpr (MODE == 0) {
-    const float2* src = a.probe + (size_t)x * NY;
+  // one step: finish row i (v: state or probe row, tv: its transmission row)
+  auto step = [&](int i, float2* v, const float2* tv) {
+    if constexpr (MODE != 0) fft_wave<NY, +1>(v, slot, tw, t);
+    const int xr = i * TPW + sub;
+    if constexpr (MODE == 2) {
+      float2* dst = a.out_wave + ((size_t)pg * a.nx + x0 + xr) * NY;
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = src[t + T * m];
-  } else {
-    const float2* src = a.state + ((size_t)pg * a.nx + x) * a.kcp;
+      for (int m = 0; m < E; m++) dst[t + T * m] = v[m];
+    } else {
 #pragma unroll
-    for (int m = 0; m < E; m++) {
-      const int iy = t + T * m;
-      const int c = iy <= a.band_a ? iy : (iy >= NY - a.band_b ? iy - (NY - a.band_b) + a.band_a + 1 : -1);
-      v[m] = c >= 0 ? src[c] : make_float2(0.f, 0.f);
+      for (int m = 0; m < E; m++) v[m] = cmul(v[m], tv[m]);
+      fft_wave<NY, -1>(v, slot, tw, t);
+      const unsigned so = (unsigned)(i * TPW) * rowb;
+      using U2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0u, 0, 0));
+#pragma unroll
+      for (int m = 0; m < E; m++)
+        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v[m]), rs, BOFF ? boff[m] : bo(m), so, 0);
     }
-    fft_wave<NY, +1>(v, slot, tw, t);
-  }
-  if constexpr (MODE == 2) {
-    float2* dst = a.out_wave + ((size_t)pg * a.nx + x) * NY;
-#pragma unroll
-    for (int m = 0; m < E; m++) dst[t + T * m] = v[m];
-    return;
-  } else {
-    const int2 ip = a.ipos[pg];
-    const float2* tr = a.trans + (size_t)(x + ip.x) * a.potNy + ip.y;
-#pragma unroll
-    for (int m = 0; m < E; m++) v[m] = cmul(v[m], tr[t + T * m]);
-    fft_wave<NY, -1>(v, slot, tw, t);
-    float2* dst = a.state + ((size_t)pg * a.nx + x) * a.kcp;
-#pragma unroll
-    for (int m = 0; m < E; m++) {
-      const int iy = t + T * m;
-      const int c = iy <= a.band_a ? iy : (iy >= NY - a.band_b ? iy - (NY - a.band_b) + a.band_a + 1 : -1);
-      if (c >= 0) dst[c] = v[m];
+  };
+  if constexpr (E <= 8) {
+    // two register sets in ping-pong: the loads of step i+1 are in flight during step i
+    float2 va[E], ta[E], vb[E], tb[E];
+    load(0, va, ta);
+    if (a.rw == 1) {
+      step(0, va, ta);
+      return;
+    }
+    for (int i = 0; i < a.rw; i += 2) {  // a.rw is even here
+      load(i + 1, vb, tb);
+      step(i, va, ta);
+      if (i + 2 < a.rw) load(i + 2, va, ta);
+      step(i + 1, vb, tb);
+    }
+  } else {  // long rows: one register set
+    float2 va[E], ta[E];
+    for (int i = 0; i < a.rw; i++) {
+      load(i, va, ta);
+      step(i, va, ta);
     }
   }
 }
@@ -85,32 +170,58 @@ __global__ __launch_bounds__(kBlock) void k_sweep_rows(SweepArgs a) {
 // Sweep B: ky columns of the band, TC columns per block staged through LDS.
 template <int NX> struct ColCfg {
   static constexpr int TC = NX <= 512 ? 8 : (NX == 1024 ? 8 : (NX == 2048 ? 4 : 2));
-  static constexpr int STRIDE = ((lds_slot<NX>() + 31) / 32) * 32 + 2;  // float2 per LDS column
+  // float2 per LDS column: lds_slot for the transforms sharing a wavefront
+  // (64, 128), else the slot rounded so the tile staging is at most 2-way
+  static constexpr int STRIDE =
+      NX == 64 ? 80 : (NX == 128 ? 144 : (NX <= 512 ? NX + 2 : ((lds_slot<NX>() + 31) / 32) * 32 + 2));
 };
 
 // Tile staging between the row-major state (TC contiguous complex per row) and
-// the column-major LDS tile; 16-byte accesses (two columns per lane).
+// the column-major LDS tile; 16-byte accesses (two columns per thread), the
+// global loads issued in batches of up to 8 per thread before their LDS writes.
 template <int NX>
 __device__ __forceinline__ void stage_tile_in(float2* tile, const float2* __restrict__ base, int kcp) {
   constexpr int TC = ColCfg<NX>::TC, ST = ColCfg<NX>::STRIDE, H = TC / 2;
-  for (int e = threadIdx.x; e < H * NX; e += kBlock) {
-    const int c = 2 * (e % H), x = e / H;
-    const float4 q = *reinterpret_cast<const float4*>(base + (size_t)x * kcp + c);
-    tile[c * ST + lpad(x)] = make_float2(q.x, q.y);
-    tile[(c + 1) * ST + lpad(x)] = make_float2(q.z, q.w);
+  constexpr int IT = H * NX / kBlock, IB = IT < 8 ? IT : 8, XS = kBlock / H;
+  static_assert(H * NX % kBlock == 0 && kBlock % H == 0 && IT % IB == 0, "tile staging assumes whole rounds");
+  const int c = 2 * (threadIdx.x % H), x0 = threadIdx.x / H;
+#pragma unroll
+  for (int i0 = 0; i0 < IT; i0 += IB) {
+    float4 q[IB];
+#pragma unroll
+    for (int i = 0; i < IB; i++)
+      q[i] = *reinterpret_cast<const float4*>(base + (size_t)(x0 + XS * (i0 + i)) * kcp + c);
+#pragma unroll
+    for (int i = 0; i < IB; i++) {
+      const int x = x0 + XS * (i0 + i);
+      tile[c * ST + lidx<NX>(x)] = make_float2(q[i].x, q[i].y);
+      tile[(c + 1) * ST + lidx<NX>(x)] = make_float2(q[i].z, q[i].w);
+    }
   }
 }
 
 template <int NX>
 __device__ __forceinline__ void stage_tile_out(const float2* tile, float2* __restrict__ base, int kcp) {
   constexpr int TC = ColCfg<NX>::TC, ST = ColCfg<NX>::STRIDE, H = TC / 2;
-  for (int e = threadIdx.x; e < H * NX; e += kBlock) {
-    const int c = 2 * (e % H), x = e / H;
-    const float2 u = tile[c * ST + lpad(x)], w = tile[(c + 1) * ST + lpad(x)];
+  constexpr int IT = H * NX / kBlock, XS = kBlock / H;
+  const int c = 2 * (threadIdx.x % H), x0 = threadIdx.x / H;
+#pragma unroll
+  for (int i = 0; i < IT; i++) {
+    const int x = x0 + XS * i;
+    const float2 u = tile[c * ST + lidx<NX>(x)], w = tile[(c + 1) * ST + lidx<NX>(x)];
     *reinterpret_cast<float4*>(base + (size_t)x * kcp + c) = make_float4(u.x, u.y, w.x, w.y);
   }
 }
 
+// Sum over the 64 lanes of a wavefront (result valid in every lane).
+__device__ __forceinline__ float wave_sum(float s) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+  return s;
+}
+
+// A block advances a.tpb consecutive tiles of one probe (the two halves of a
+// 128-byte state line are adjacent tiles, so the second read hits L2).
 template <int NX, bool REC, bool DIFF>
 __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
   constexpr int E = Plan<NX>::E;
@@ -119,96 +230,128 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
   constexpr int ST = ColCfg<NX>::STRIDE;
   constexpr int G = kBlock / T;  // transforms in flight per block
   constexpr int ROUNDS = (TC + G - 1) / G;
+  constexpr bool PXR = E <= 8;   // per-thread propagator factors and k^2 kept in registers
   extern __shared__ float2 tile[];  // TC * ST
   __shared__ float red[kMaxDet + 1];
 
-  const int p = blockIdx.x / a.ntiles;
-  const int tile_id = blockIdx.x % a.ntiles;
+  const int bpp = (a.ntiles + a.tpb - 1) / a.tpb;  // blocks per probe
+  const int p = blockIdx.x / bpp;
+  const int tile_first = (blockIdx.x % bpp) * a.tpb;
+  const int tile_end = min(tile_first + a.tpb, a.ntiles);
   const int pg = a.probe0 + p;
-  const int c0 = tile_id * TC;
-  float2* base = a.state + (size_t)pg * a.nx * a.kcp + c0;
+  float2* pbase = a.state + (size_t)pg * a.nx * a.kcp;
 
-  if constexpr (REC) {
-    if (threadIdx.x <= kMaxDet) red[threadIdx.x] = 0.f;
-  }
-  stage_tile_in<NX>(tile, base, a.kcp);
-  __syncthreads();
-
-  const int slot_id = threadIdx.x / T;
+  // one transform per wavefront when T == 64: the column is wave-uniform
+  const int slot_id = T == 64 ? wave_uniform(threadIdx.x >> 6) : threadIdx.x / T;
   const int t = threadIdx.x % T;
   Twiddles<NX> tw;
   tw.init(a.twx, t);
-  // per-lane propagator factors and k^2 (independent of the column)
-  float2 pxr[E];
-  float kx2r[E];
+  // per-thread propagator factors and k^2 (independent of the column)
+  float2 pxr[PXR ? E : 1];
+  float kx2r[PXR ? E : 1];
+  if constexpr (PXR) {
 #pragma unroll
-  for (int m = 0; m < E; m++) {
-    pxr[m] = a.px[t + T * m];
-    kx2r[m] = a.kx2[t + T * m];
+    for (int m = 0; m < E; m++) {
+      pxr[m] = a.px[t + T * m];
+      kx2r[m] = a.kx2[t + T * m];
+    }
   }
-  for (int r = 0; r < ROUNDS; r++) {
-    const int cl = r * G + slot_id;
-    if (cl >= TC) break;  // only when G does not divide TC (uniform per transform)
-    float2* col = tile + cl * ST;
-    const int c = c0 + cl;
-    const int iy = c <= a.band_a ? c : (c < a.kc ? c - (a.band_a + 1) + (a.ny - a.band_b) : -1);
-    float2 v[E];
+  auto kx2_of = [&](int m) -> float { return PXR ? kx2r[m] : a.kx2[t + T * m]; };
+  auto px_of = [&](int m) -> float2 { return PXR ? pxr[m] : a.px[t + T * m]; };
+  for (int tl = tile_first; tl < tile_end; tl++) {
+    if (tl > tile_first) __syncthreads();  // previous tile's LDS reads (stage-out, red) are done
+    stage_tile_in<NX>(tile, pbase + tl * TC, a.kcp);
+    if constexpr (REC) {
+      if (threadIdx.x <= kMaxDet) red[threadIdx.x] = 0.f;
+    }
+    __syncthreads();
+    const int c0 = tl * TC;
+    float dsum[REC ? kMaxDet + 1 : 1];
+    if constexpr (REC) {
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = col[lpad(t + T * m)];
-    fft_wave<NX, -1>(v, col, tw, t);
-    // propagate: P = px[ix]*py[iy] (1/(nx*ny) folded into px), zero outside k2max
-    if (iy >= 0) {
-      const float ky2 = a.ky2[iy];
-      const float2 py = a.py[iy];
+      for (int d = 0; d <= kMaxDet; d++) dsum[d] = 0.f;
+    }
+    for (int r = 0; r < ROUNDS; r++) {
+      const int cl = r * G + slot_id;
+      if (cl >= TC) break;  // only when G does not divide TC (uniform per transform)
+      float2* col = tile + cl * ST;
+      const int c = c0 + cl;
+      const int iy = c <= a.band_a ? c : (c < a.kc ? c - (a.band_a + 1) + (a.ny - a.band_b) : -1);
+      float2 v[E];
 #pragma unroll
-      for (int m = 0; m < E; m++) {
-        const float kx2 = kx2r[m];
-        if (kx2 < a.k2max && (kx2 + ky2) < a.k2max) v[m] = cmul(cmul(v[m], py), pxr[m]);
-        else v[m] = make_float2(0.f, 0.f);
-      }
-      if constexpr (REC) {
-        // collectIntensity: detector d sums |Psi|^2 over pixels whose
-        // (shift-corrected) k^2 lies in [k2in, k2out]; slot kMaxDet is the total.
-        float tot = 0.f;
+      for (int m = 0; m < E; m++) v[m] = col[lidx<NX>(t + T * m)];
+      fft_wave<NX, -1>(v, col, tw, t);
+      // propagate: P = px[ix]*py[iy] (1/(nx*ny) folded into px), zero outside k2max
+      if (iy >= 0) {
+        const float ky2 = a.ky2[iy];
+        const float2 py = a.py[iy];
 #pragma unroll
-        for (int m = 0; m < E; m++) tot += cnorm(v[m]);
-        atomicAdd(&red[kMaxDet], tot);
-        for (int d = 0; d < a.ndet; d++) {
-          const int sx = a.det_sx[d], sy = a.det_sy[d];
-          const float k2y = a.ky2[(iy - sy + 2 * a.ny) % a.ny];
-          const float lo = a.det_k2in[d], hi = a.det_k2out[d];
-          float s = 0.f;
+        for (int m = 0; m < E; m++) {
+          const float kx2 = kx2_of(m);
+          if (kx2 < a.k2max && (kx2 + ky2) < a.k2max) v[m] = cmul(cmul(v[m], py), px_of(m));
+          else v[m] = make_float2(0.f, 0.f);
+        }
+        if constexpr (REC) {
+          // collectIntensity: detector d sums |Psi|^2 over pixels whose
+          // (shift-corrected) k^2 lies in [k2in, k2out]; slot kMaxDet is the total.
+#pragma unroll
+          for (int m = 0; m < E; m++) dsum[kMaxDet] += cnorm(v[m]);
+#pragma unroll
+          for (int d = 0; d < kMaxDet; d++) {  // unrolled: dsum stays in registers
+            if (d >= a.ndet) break;
+            const int sx = a.det_sx[d];
+            const float k2y = a.ky2[(iy - a.det_sy[d]) & (a.ny - 1)];
+            const float lo = a.det_k2in[d], hi = a.det_k2out[d];
+            float s = 0.f;
+            if (sx == 0) {
+#pragma unroll
+              for (int m = 0; m < E; m++) {
+                const float k2 = kx2_of(m) + k2y;
+                s += (k2 >= lo && k2 <= hi) ? cnorm(v[m]) : 0.f;
+              }
+            } else {
+#pragma unroll
+              for (int m = 0; m < E; m++) {
+                const float k2 = a.kx2[(t + T * m - sx) & (a.nx - 1)] + k2y;
+                s += (k2 >= lo && k2 <= hi) ? cnorm(v[m]) : 0.f;
+              }
+            }
+            dsum[d] += s;
+          }
+        }
+        if constexpr (DIFF) {  // fftshifted |Psi|^2/sqrt(nx*ny) of the unnormalised Psi
+          float* dp = a.diff + (size_t)pg * a.nx * a.ny;
+          const int sy = (iy + a.ny / 2) % a.ny;
 #pragma unroll
           for (int m = 0; m < E; m++) {
             const int ix = t + T * m;
-            const float k2 = a.kx2[(ix - sx + 2 * a.nx) % a.nx] + k2y;
-            if (k2 >= lo && k2 <= hi) s += cnorm(v[m]);
+            dp[(size_t)((ix + a.nx / 2) % a.nx) * a.ny + sy] = cnorm(v[m]) * a.diff_scale;
           }
-          atomicAdd(&red[d], s);
         }
-      }
-      if constexpr (DIFF) {  // fftshifted |Psi|^2/sqrt(nx*ny) of the unnormalised Psi
-        float* dp = a.diff + (size_t)pg * a.nx * a.ny;
-        const int sy = (iy + a.ny / 2) % a.ny;
+      } else {
 #pragma unroll
-        for (int m = 0; m < E; m++) {
-          const int ix = t + T * m;
-          dp[(size_t)((ix + a.nx / 2) % a.nx) * a.ny + sy] = cnorm(v[m]) * a.diff_scale;
-        }
+        for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
       }
-    } else {
+      fft_wave<NX, +1>(v, col, tw, t);
+      xform_sync<NX>();
 #pragma unroll
-      for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
+      for (int m = 0; m < E; m++) col[lidx<NX>(t + T * m)] = v[m];
     }
-    fft_wave<NX, +1>(v, col, tw, t);
-    wave_lds_sync();
+    if constexpr (REC) {  // one LDS atomic per wavefront and sum
 #pragma unroll
-    for (int m = 0; m < E; m++) col[lpad(t + T * m)] = v[m];
-  }
-  __syncthreads();
-  stage_tile_out<NX>(tile, base, a.kcp);
-  if constexpr (REC) {
-    if (threadIdx.x <= kMaxDet) a.partial[((size_t)p * a.ntiles + tile_id) * (kMaxDet + 1) + threadIdx.x] = red[threadIdx.x];
+      for (int d = 0; d < kMaxDet; d++) {
+        if (d >= a.ndet) break;
+        const float s = wave_sum(dsum[d]);
+        if ((threadIdx.x & 63) == 0) atomicAdd(&red[d], s);
+      }
+      const float s = wave_sum(dsum[kMaxDet]);
+      if ((threadIdx.x & 63) == 0) atomicAdd(&red[kMaxDet], s);
+    }
+    __syncthreads();
+    stage_tile_out<NX>(tile, pbase + c0, a.kcp);
+    if constexpr (REC) {
+      if (threadIdx.x <= kMaxDet) a.partial[((size_t)p * a.ntiles + tl) * (kMaxDet + 1) + threadIdx.x] = red[threadIdx.x];
+    }
   }
 }
 
@@ -274,12 +417,23 @@ __global__ __launch_bounds__(kBlock) void k_rows_bluestein(float2* __restrict__ 
 
 // ============================================================================
 // Launch templates, instantiated per size in qs_sizes.hip.
+// Steps per unit (a wavefront, or the block for block-wide transforms): enough
+// units to fill every CU several times over, each looping over up to a.rw
+// steps (a power of two dividing nx/TPW).
 template <int N>
-void launch_rows_mode(const SweepArgs& a, int mode, hipStream_t st) {
+void launch_rows_mode(const SweepArgs& a0, int mode, hipStream_t st) {
   constexpr int T = N / Plan<N>::E;
-  constexpr int RPB = kBlock / T;
+  constexpr int TPW = T <= 64 ? 64 / T : 1;
+  constexpr int UPB = T <= 64 ? kBlock / 64 : 1;
+  SweepArgs a = a0;
   const long rows = (long)a.nprobe * a.nx;
-  dim3 grid((unsigned)((rows + RPB - 1) / RPB));
+  const int max_rw = a.rw > 0 ? a.rw : 4;  // caller's cap (0: default, measured best at 512^2)
+  const long target_units = 256L * 8 * UPB;
+  int rw = 1;
+  while (rw < max_rw && rows / ((long)TPW * rw * 2) >= target_units && (a.nx / TPW) % (rw * 2) == 0) rw *= 2;
+  a.rw = rw;
+  const long units = rows / ((long)TPW * rw);
+  dim3 grid((unsigned)((units + UPB - 1) / UPB));
   if (mode == 0) hipLaunchKernelGGL((k_sweep_rows<N, 0>), grid, dim3(kBlock), 0, st, a);
   else if (mode == 1) hipLaunchKernelGGL((k_sweep_rows<N, 1>), grid, dim3(kBlock), 0, st, a);
   else hipLaunchKernelGGL((k_sweep_rows<N, 2>), grid, dim3(kBlock), 0, st, a);
@@ -288,10 +442,17 @@ void launch_rows_mode(const SweepArgs& a, int mode, hipStream_t st) {
 template <int N>
 constexpr int col_lds_bytes_t() { return (int)sizeof(float2) * ColCfg<N>::TC * ColCfg<N>::STRIDE; }
 
+// Tiles per block: a.tpb from the caller when > 0, else 1 when the launch has
+// few blocks, up to 4 when there are enough probes to fill every CU.
 template <int N>
-void launch_cols_t(const SweepArgs& a, bool rec, bool diff, hipStream_t st) {
+void launch_cols_t(const SweepArgs& a0, bool rec, bool diff, hipStream_t st) {
   const size_t lds = col_lds_bytes_t<N>();
-  dim3 grid((unsigned)(a.nprobe * a.ntiles));
+  SweepArgs a = a0;
+  if (a.tpb <= 0) {
+    a.tpb = 1;
+    while (a.tpb < 4 && (long)a.nprobe * ((a.ntiles + 2 * a.tpb - 1) / (2 * a.tpb)) >= 256L * 8) a.tpb *= 2;
+  }
+  dim3 grid((unsigned)(a.nprobe * ((a.ntiles + a.tpb - 1) / a.tpb)));
   if (rec && diff) hipLaunchKernelGGL((k_sweep_cols<N, true, true>), grid, dim3(kBlock), lds, st, a);
   else if (rec) hipLaunchKernelGGL((k_sweep_cols<N, true, false>), grid, dim3(kBlock), lds, st, a);
   else if (diff) hipLaunchKernelGGL((k_sweep_cols<N, false, true>), grid, dim3(kBlock), lds, st, a);
@@ -370,7 +531,7 @@ __global__ __launch_bounds__(kBlock) void k_adj_cols(AdjArgs A) {
     const int iy = c <= a.band_a ? c : (c < a.kc ? c - (a.band_a + 1) + (a.ny - a.band_b) : -1);
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = col[lpad(t + T * m)];
+    for (int m = 0; m < E; m++) v[m] = col[lidx<NX>(t + T * m)];
     fft_wave<NX, -1>(v, col, tw, t);
     if (iy >= 0) {
       const float ky2 = a.ky2[iy];
@@ -397,9 +558,9 @@ __global__ __launch_bounds__(kBlock) void k_adj_cols(AdjArgs A) {
       for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
     }
     fft_wave<NX, +1>(v, col, tw, t);
-    wave_lds_sync();
+    xform_sync<NX>();
 #pragma unroll
-    for (int m = 0; m < E; m++) col[lpad(t + T * m)] = v[m];
+    for (int m = 0; m < E; m++) col[lidx<NX>(t + T * m)] = v[m];
   }
   if constexpr (LAST) atomicAdd(&red, lsum);
   __syncthreads();
