@@ -134,19 +134,22 @@ template <> struct Plan<1024> { static constexpr int E = 16, NP = 3, R0 = 16, R1
 template <> struct Plan<2048> { static constexpr int E = 8,  NP = 4, R0 = 8,  R1 = 8,  R2 = 8,  R3 = 4; };
 template <> struct Plan<4096> { static constexpr int E = 16, NP = 3, R0 = 16, R1 = 16, R2 = 16, R3 = 1; };
 
-// LDS placement of element i of a length-N transform.  On gfx950 a
-// ds_read_b64 serves 32 lanes at a time over 32 eight-byte bank pairs and a
-// ds_write_b64 16 lanes over 16.  Up to N = 512 the low four index bits are
-// XORed with a function of bits 4.. (i ^ f_N(i >> 4)): every read and write
-// pattern of fft_wave below, and the natural-order column accesses of the
-// sweeps, then hit distinct bank pairs (sizes 64 and 128 also need the slot
-// strides of lds_slot, since several transforms share a wavefront there).
-// Beyond 512 one float2 of padding every 16 (i + (i >> 4)) keeps every
-// address an immediate offset from a per-thread base (the XOR form would hold
-// one address register per access), at a residual 1.3-1.5-way conflict rate.
-template <int N>
+// LDS placement of element i of a length-N transform, in one of two layouts.
+// On gfx950 a ds_read_b64 serves 32 lanes at a time over 32 eight-byte bank
+// pairs and a ds_write_b64 16 lanes over 16.
+//  SWZ (default up to N = 512): the low four index bits are XORed with a
+//    function of bits 4.. (i ^ f_N(i >> 4)); every read and write pattern of
+//    fft_wave below then hits distinct bank pairs (sizes 64 and 128 also need
+//    the slot strides of lds_slot, as several transforms share a wavefront).
+//    The write addresses are no longer immediate offsets from one base, so
+//    the compiler keeps one address register per access.
+//  PAD: one float2 of padding every 16 (i + (i >> 4)): immediate offsets from
+//    a per-thread base, at a residual 1.3-1.6-way conflict rate; used beyond
+//    512 and wherever registers are the scarcer resource.
+template <int N> constexpr bool kSwz = N <= 512;
+template <int N, bool SWZ = kSwz<N>>
 __device__ __forceinline__ int lidx(int i) {
-  if constexpr (N >= 1024) {
+  if constexpr (!SWZ) {
     return i + (i >> 4);
   } else {
     const int j = i >> 4;
@@ -156,7 +159,8 @@ __device__ __forceinline__ int lidx(int i) {
     return i ^ f;
   }
 }
-template <int N> constexpr int lds_slot() { return N == 64 ? 80 : (N == 128 ? 144 : (N >= 1024 ? N + N / 16 : N)); }
+template <int N, bool SWZ = kSwz<N>>
+constexpr int lds_slot() { return SWZ ? (N == 64 ? 80 : (N == 128 ? 144 : N)) : N + N / 16; }
 
 // Orders this lane's LDS reads before later LDS writes of other lanes of the
 // same wavefront and vice versa (all lanes of a transform live in one wave).
@@ -211,7 +215,7 @@ struct Twiddles {
 // output to registers (natural layout).  Otherwise through the LDS slot.
 // wr: this pass's register twiddles (or nullptr: read tab, the forward table
 // W_N^m = exp(-2 pi i m/N), m < N).
-template <int N, int R, int NS, int DIR, bool FIRST, bool LAST>
+template <int N, int R, int NS, int DIR, bool FIRST, bool LAST, bool SWZ>
 __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const float2* wr,
                                               const float2* __restrict__ tab, int t) {
   constexpr int E = Plan<N>::E;
@@ -227,15 +231,20 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
 #pragma unroll
     for (int b = 0; b < NB; b++)
 #pragma unroll
-      for (int k = 0; k < R; k++) a[b][k] = slot[lidx<N>(t + T * b + k * (N / R))];
+      for (int k = 0; k < R; k++) a[b][k] = slot[lidx<N, SWZ>(t + T * b + k * (N / R))];
   }
   if constexpr (NS > 1) {
+    // table twiddles are loaded where they are used: the empty asm hides the
+    // pointer's loop invariance, so the loads are not hoisted out of the
+    // callers' loops into registers held across the whole kernel
+    const float2* tp = tab;
+    if (!wr) asm volatile("" : "+s"(tp));
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int jm = (t + T * b) % NS;
 #pragma unroll
       for (int k = 1; k < R; k++) {
-        const float2 w = wr ? wr[b * (R - 1) + k - 1] : tab[k * jm * (N / (NS * R))];
+        const float2 w = wr ? wr[b * (R - 1) + k - 1] : tp[k * jm * (N / (NS * R))];
         a[b][k] = DIR < 0 ? cmul(a[b][k], w) : cmulc(a[b][k], w);
       }
     }
@@ -254,27 +263,27 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
       const int j = t + T * b;
       const int base = (j / NS) * NS * R + (j % NS);
 #pragma unroll
-      for (int k = 0; k < R; k++) slot[lidx<N>(base + k * NS)] = a[b][k];
+      for (int k = 0; k < R; k++) slot[lidx<N, SWZ>(base + k * NS)] = a[b][k];
     }
     xform_sync<N>();
   }
 }
 
 // Full length-N transform of the thread-distributed vector v (natural layout
-// in and out).  slot: this transform's LDS region of lds_slot<N>() float2.
-template <int N, int DIR>
+// in and out).  slot: this transform's LDS region of lds_slot<N, SWZ>() float2.
+template <int N, int DIR, bool SWZ = kSwz<N>>
 __device__ __forceinline__ void fft_wave(float2* v, float2* slot, const Twiddles<N>& tw, int t) {
   using P = Plan<N>;
   constexpr int S1 = P::R0;
   constexpr int S2 = P::R0 * P::R1;
   constexpr bool REG = Twiddles<N>::REG;
-  stockham_pass<N, P::R0, 1, DIR, true, false>(v, slot, nullptr, tw.tab, t);
-  stockham_pass<N, P::R1, S1, DIR, false, false>(v, slot, REG ? tw.w1 : nullptr, tw.tab, t);
+  stockham_pass<N, P::R0, 1, DIR, true, false, SWZ>(v, slot, nullptr, tw.tab, t);
+  stockham_pass<N, P::R1, S1, DIR, false, false, SWZ>(v, slot, REG ? tw.w1 : nullptr, tw.tab, t);
   if constexpr (P::NP == 3) {
-    stockham_pass<N, P::R2, S2, DIR, false, true>(v, slot, REG ? tw.w2 : nullptr, tw.tab, t);
+    stockham_pass<N, P::R2, S2, DIR, false, true, SWZ>(v, slot, REG ? tw.w2 : nullptr, tw.tab, t);
   } else {
-    stockham_pass<N, P::R2, S2, DIR, false, false>(v, slot, nullptr, tw.tab, t);
-    stockham_pass<N, P::R3, S2 * P::R2, DIR, false, true>(v, slot, nullptr, tw.tab, t);
+    stockham_pass<N, P::R2, S2, DIR, false, false, SWZ>(v, slot, nullptr, tw.tab, t);
+    stockham_pass<N, P::R3, S2 * P::R2, DIR, false, true, SWZ>(v, slot, nullptr, tw.tab, t);
   }
 }
 

@@ -170,48 +170,67 @@ __global__ __launch_bounds__(kBlock, Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 
 // Sweep B: ky columns of the band, TC columns per block staged through LDS.
 template <int NX> struct ColCfg {
   static constexpr int TC = NX <= 512 ? 8 : (NX == 1024 ? 8 : (NX == 2048 ? 4 : 2));
-  // float2 per LDS column: lds_slot for the transforms sharing a wavefront
-  // (64, 128), else the slot rounded so the tile staging is at most 2-way
-  static constexpr int STRIDE =
-      NX == 64 ? 80 : (NX == 128 ? 144 : (NX <= 512 ? NX + 2 : ((lds_slot<NX>() + 31) / 32) * 32 + 2));
+  // column slots use the PAD layout (registers are the scarcer resource here);
+  // float2 per LDS column, rounded so the tile staging is at most 2-way
+  static constexpr int STRIDE = ((lds_slot<NX, false>() + 31) / 32) * 32 + 2;
 };
 
 // Tile staging between the row-major state (TC contiguous complex per row) and
-// the column-major LDS tile; 16-byte accesses (two columns per thread), the
-// global loads issued in batches of up to 8 per thread before their LDS writes.
+// the column-major LDS tile (PAD layout); 16-byte accesses (two columns per
+// thread).  Thread (c, x0) moves columns c, c+1 of rows x0 + XS*i: one buffer
+// offset and one LDS base per thread, the row step as the scalar offset and
+// as immediate LDS offsets (XS is a multiple of 16 and x0 < XS, so
+// lidx(x0 + XS*i) = lidx(x0) + XS*i + XS*i/16).  Global loads are issued in
+// batches of up to 8 per thread before their LDS writes.
 template <int NX>
-__device__ __forceinline__ void stage_tile_in(float2* tile, const float2* __restrict__ base, int kcp) {
-  constexpr int TC = ColCfg<NX>::TC, ST = ColCfg<NX>::STRIDE, H = TC / 2;
-  constexpr int IT = H * NX / kBlock, IB = IT < 8 ? IT : 8, XS = kBlock / H;
-  static_assert(H * NX % kBlock == 0 && kBlock % H == 0 && IT % IB == 0, "tile staging assumes whole rounds");
-  const int c = 2 * (threadIdx.x % H), x0 = threadIdx.x / H;
+struct TileStage {
+  static constexpr int TC = ColCfg<NX>::TC, ST = ColCfg<NX>::STRIDE, H = TC / 2;
+  static constexpr int IT = H * NX / kBlock, IB = IT < 8 ? IT : 8, XS = kBlock / H;
+  static_assert(H * NX % kBlock == 0 && kBlock % H == 0 && IT % IB == 0 && XS % 16 == 0,
+                "tile staging assumes whole rounds");
+  static constexpr int lstep(int i) { return XS * i + XS * i / 16; }
+  rsrc_t rs;          // the probe's band-compact state
+  unsigned voff;      // byte offset of (row x0, column c) relative to the tile's first column
+  int lbase;          // LDS index of (column c, row x0)
+  unsigned rstep;     // bytes per XS rows
+  __device__ __forceinline__ TileStage(const float2* pstate, int nx, int kcp) {
+    const int c = 2 * (threadIdx.x % H), x0 = threadIdx.x / H;
+    rs = make_rsrc(pstate, (long)nx * kcp * 8);
+    voff = (unsigned)(x0 * kcp + c) * 8u;
+    lbase = c * ST + lidx<NX, false>(x0);
+    rstep = (unsigned)(XS * kcp) * 8u;
+  }
+  __device__ __forceinline__ void in(float2* tile, int c0) const {
+    const unsigned co = (unsigned)c0 * 8u;
 #pragma unroll
-  for (int i0 = 0; i0 < IT; i0 += IB) {
-    float4 q[IB];
+    for (int i0 = 0; i0 < IT; i0 += IB) {
+      if (i0 > 0) __builtin_amdgcn_sched_barrier(0);  // keep the batches apart (registers)
+      float4 q[IB];
 #pragma unroll
-    for (int i = 0; i < IB; i++)
-      q[i] = *reinterpret_cast<const float4*>(base + (size_t)(x0 + XS * (i0 + i)) * kcp + c);
+      for (int i = 0; i < IB; i++)
+        q[i] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff + co, (i0 + i) * rstep, 0));
 #pragma unroll
-    for (int i = 0; i < IB; i++) {
-      const int x = x0 + XS * (i0 + i);
-      tile[c * ST + lidx<NX>(x)] = make_float2(q[i].x, q[i].y);
-      tile[(c + 1) * ST + lidx<NX>(x)] = make_float2(q[i].z, q[i].w);
+      for (int i = 0; i < IB; i++) {
+        tile[lbase + lstep(i0 + i)] = make_float2(q[i].x, q[i].y);
+        tile[lbase + ST + lstep(i0 + i)] = make_float2(q[i].z, q[i].w);
+      }
     }
   }
-}
-
-template <int NX>
-__device__ __forceinline__ void stage_tile_out(const float2* tile, float2* __restrict__ base, int kcp) {
-  constexpr int TC = ColCfg<NX>::TC, ST = ColCfg<NX>::STRIDE, H = TC / 2;
-  constexpr int IT = H * NX / kBlock, XS = kBlock / H;
-  const int c = 2 * (threadIdx.x % H), x0 = threadIdx.x / H;
+  __device__ __forceinline__ void out(const float2* tile, int c0) const {
+    const unsigned co = (unsigned)c0 * 8u;
+    using U4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(rs, 0u, 0, 0));
 #pragma unroll
-  for (int i = 0; i < IT; i++) {
-    const int x = x0 + XS * i;
-    const float2 u = tile[c * ST + lidx<NX>(x)], w = tile[(c + 1) * ST + lidx<NX>(x)];
-    *reinterpret_cast<float4*>(base + (size_t)x * kcp + c) = make_float4(u.x, u.y, w.x, w.y);
+    for (int i0 = 0; i0 < IT; i0 += IB) {
+      if (i0 > 0) __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int i = i0; i < i0 + IB; i++) {
+        const float2 u = tile[lbase + lstep(i)], w = tile[lbase + ST + lstep(i)];
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, make_float4(u.x, u.y, w.x, w.y)), rs, voff + co,
+                                               i * rstep, 0);
+      }
+    }
   }
-}
+};
 
 // Sum over the 64 lanes of a wavefront (result valid in every lane).
 __device__ __forceinline__ float wave_sum(float s) {
@@ -239,7 +258,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
   const int tile_first = (blockIdx.x % bpp) * a.tpb;
   const int tile_end = min(tile_first + a.tpb, a.ntiles);
   const int pg = a.probe0 + p;
-  float2* pbase = a.state + (size_t)pg * a.nx * a.kcp;
+  const TileStage<NX> stage(a.state + (size_t)pg * a.nx * a.kcp, a.nx, a.kcp);
 
   // one transform per wavefront when T == 64: the column is wave-uniform
   const int slot_id = T == 64 ? wave_uniform(threadIdx.x >> 6) : threadIdx.x / T;
@@ -260,7 +279,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
   auto px_of = [&](int m) -> float2 { return PXR ? pxr[m] : a.px[t + T * m]; };
   for (int tl = tile_first; tl < tile_end; tl++) {
     if (tl > tile_first) __syncthreads();  // previous tile's LDS reads (stage-out, red) are done
-    stage_tile_in<NX>(tile, pbase + tl * TC, a.kcp);
+    stage.in(tile, tl * TC);
     if constexpr (REC) {
       if (threadIdx.x <= kMaxDet) red[threadIdx.x] = 0.f;
     }
@@ -271,6 +290,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
 #pragma unroll
       for (int d = 0; d <= kMaxDet; d++) dsum[d] = 0.f;
     }
+#pragma unroll 1
     for (int r = 0; r < ROUNDS; r++) {
       const int cl = r * G + slot_id;
       if (cl >= TC) break;  // only when G does not divide TC (uniform per transform)
@@ -279,8 +299,8 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
       const int iy = c <= a.band_a ? c : (c < a.kc ? c - (a.band_a + 1) + (a.ny - a.band_b) : -1);
       float2 v[E];
 #pragma unroll
-      for (int m = 0; m < E; m++) v[m] = col[lidx<NX>(t + T * m)];
-      fft_wave<NX, -1>(v, col, tw, t);
+      for (int m = 0; m < E; m++) v[m] = col[lidx<NX, false>(t + T * m)];
+      fft_wave<NX, -1, false>(v, col, tw, t);
       // propagate: P = px[ix]*py[iy] (1/(nx*ny) folded into px), zero outside k2max
       if (iy >= 0) {
         const float ky2 = a.ky2[iy];
@@ -332,10 +352,10 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
 #pragma unroll
         for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
       }
-      fft_wave<NX, +1>(v, col, tw, t);
+      fft_wave<NX, +1, false>(v, col, tw, t);
       xform_sync<NX>();
 #pragma unroll
-      for (int m = 0; m < E; m++) col[lidx<NX>(t + T * m)] = v[m];
+      for (int m = 0; m < E; m++) col[lidx<NX, false>(t + T * m)] = v[m];
     }
     if constexpr (REC) {  // one LDS atomic per wavefront and sum
 #pragma unroll
@@ -348,7 +368,7 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
       if ((threadIdx.x & 63) == 0) atomicAdd(&red[kMaxDet], s);
     }
     __syncthreads();
-    stage_tile_out<NX>(tile, pbase + c0, a.kcp);
+    stage.out(tile, c0);
     if constexpr (REC) {
       if (threadIdx.x <= kMaxDet) a.partial[((size_t)p * a.ntiles + tl) * (kMaxDet + 1) + threadIdx.x] = red[threadIdx.x];
     }
@@ -512,17 +532,16 @@ __global__ __launch_bounds__(kBlock) void k_adj_cols(AdjArgs A) {
   const int tile_id = blockIdx.x % a.ntiles;
   const int pg = a.probe0 + p;
   const int c0 = tile_id * TC;
-  const size_t off = (size_t)pg * a.nx * a.kcp + c0;
-  const float2* src = (LAST ? A.xprev : A.gx) + off;
-  float2* dst = A.gx + off;
+  const size_t off = (size_t)pg * a.nx * a.kcp;
   if (threadIdx.x == 0) red = 0.f;
-  stage_tile_in<NX>(tile, src, a.kcp);
+  TileStage<NX>((LAST ? A.xprev : A.gx) + off, a.nx, a.kcp).in(tile, c0);
   __syncthreads();
   const int slot_id = threadIdx.x / T;
   const int t = threadIdx.x % T;
   Twiddles<NX> tw;
   tw.init(a.twx, t);
   float lsum = 0.f;
+#pragma unroll 1
   for (int r = 0; r < ROUNDS; r++) {
     const int cl = r * G + slot_id;
     if (cl >= TC) break;
@@ -531,8 +550,8 @@ __global__ __launch_bounds__(kBlock) void k_adj_cols(AdjArgs A) {
     const int iy = c <= a.band_a ? c : (c < a.kc ? c - (a.band_a + 1) + (a.ny - a.band_b) : -1);
     float2 v[E];
 #pragma unroll
-    for (int m = 0; m < E; m++) v[m] = col[lidx<NX>(t + T * m)];
-    fft_wave<NX, -1>(v, col, tw, t);
+    for (int m = 0; m < E; m++) v[m] = col[lidx<NX, false>(t + T * m)];
+    fft_wave<NX, -1, false>(v, col, tw, t);
     if (iy >= 0) {
       const float ky2 = a.ky2[iy];
       const float2 py = a.py[iy];
@@ -557,14 +576,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_cols(AdjArgs A) {
 #pragma unroll
       for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
     }
-    fft_wave<NX, +1>(v, col, tw, t);
+    fft_wave<NX, +1, false>(v, col, tw, t);
     xform_sync<NX>();
 #pragma unroll
-    for (int m = 0; m < E; m++) col[lidx<NX>(t + T * m)] = v[m];
+    for (int m = 0; m < E; m++) col[lidx<NX, false>(t + T * m)] = v[m];
   }
   if constexpr (LAST) atomicAdd(&red, lsum);
   __syncthreads();
-  stage_tile_out<NX>(tile, dst, a.kcp);
+  TileStage<NX>(A.gx + off, a.nx, a.kcp).out(tile, c0);
   if constexpr (LAST) {
     if (threadIdx.x == 0) A.loss_partial[(size_t)p * a.ntiles + tile_id] = red;
   }
