@@ -462,16 +462,13 @@ void launch_rows_mode(const SweepArgs& a0, int mode, hipStream_t st) {
 template <int N>
 constexpr int col_lds_bytes_t() { return (int)sizeof(float2) * ColCfg<N>::TC * ColCfg<N>::STRIDE; }
 
-// Tiles per block: a.tpb from the caller when > 0, else 1 when the launch has
-// few blocks, up to 4 when there are enough probes to fill every CU.
+// Tiles per block: a.tpb from the caller when > 0, else 1 (measured best at
+// 512^2: more tiles per block only lengthen each block's serial phases).
 template <int N>
 void launch_cols_t(const SweepArgs& a0, bool rec, bool diff, hipStream_t st) {
   const size_t lds = col_lds_bytes_t<N>();
   SweepArgs a = a0;
-  if (a.tpb <= 0) {
-    a.tpb = 1;
-    while (a.tpb < 4 && (long)a.nprobe * ((a.ntiles + 2 * a.tpb - 1) / (2 * a.tpb)) >= 256L * 8) a.tpb *= 2;
-  }
+  if (a.tpb <= 0) a.tpb = 1;
   dim3 grid((unsigned)(a.nprobe * ((a.ntiles + a.tpb - 1) / a.tpb)));
   if (rec && diff) hipLaunchKernelGGL((k_sweep_cols<N, true, true>), grid, dim3(kBlock), lds, st, a);
   else if (rec) hipLaunchKernelGGL((k_sweep_cols<N, true, false>), grid, dim3(kBlock), lds, st, a);
