@@ -167,6 +167,7 @@ def main():
     ap.add_argument("--loop-probes", type=int, default=8, help="probes for the per-probe loop baseline")
     ap.add_argument("--cpu-sample", type=int, default=8, help="probes for the oracle CPU baseline (0 = skip)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-graph", action="store_true", help="time eager launches only")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); got --gpus %d" % a.gpus,
@@ -195,18 +196,30 @@ def main():
         sl = ipos_all[(i % nbatches) * P:((i % nbatches) + 1) * P]
         return ms.run_stem(sl, bins=bins, nbins=nbins)
 
-    for i in range(a.warmup):
-        step(i)
     import torch
-    torch.cuda.synchronize()
-    ms.set_profiling(True)
-    t0 = time.perf_counter()
-    outs = [step(i) for i in range(a.steps)]
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    ms_rows, ms_cols, n_rows, n_cols = ms.kernel_times()
-    ms.set_profiling(False)
-    elapsed = t1 - t0
+
+    def timed(graph, profile):
+        """K steps between two device synchronisations; eager launches with per-launch
+        HIP events (kernel durations for the roofline), or the captured HIP graph."""
+        ms.set_graph(graph)
+        for i in range(a.warmup if not graph else max(a.warmup, 1)):  # graph: capture on the first call
+            step(i)
+        torch.cuda.synchronize()
+        if profile:
+            ms.set_profiling(True)
+        t0 = time.perf_counter()
+        for i in range(a.steps):
+            step(i)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        kt = ms.kernel_times() if profile else None
+        ms.set_profiling(False)
+        ms.set_graph(False)
+        return t1 - t0, kt
+
+    el_eager, (ms_rows, ms_cols, n_rows, n_cols) = timed(graph=False, profile=True)
+    el_graph, _ = timed(graph=True, profile=False) if not a.no_graph else (float("inf"), None)
+    elapsed = min(el_eager, el_graph)
     probe_slices = a.steps * P * nslice
     value = probe_slices / elapsed
 
@@ -253,6 +266,9 @@ def main():
                                "dx=0.030 A, 300 kV, 25 mrad, Cs=1 um, df=-5 nm, 64x64 scan, BF/ABF/ADF",
                    "probes_per_launch": P, "probes_per_kernel_launch": G, "slices": nslice, "nx": cfg.nx,
                    "potN": cfg.potNx},
+        "launch_modes": {"eager_events": round(probe_slices / el_eager, 1),
+                         "hip_graph": round(probe_slices / el_graph, 1) if el_graph < float("inf") else None,
+                         "value_from": "hip_graph" if el_graph < el_eager else "eager_events"},
         "per_probe_loop": {"value": round(loop_rate, 1), "unit": "probe-slices/s", "probes": a.loop_probes},
         "batched_over_per_probe": round(value / loop_rate, 2),
         "kernel_ms": {"rows_sweep": round(ms_rows / max(n_rows, 1), 4), "cols_sweep": round(ms_cols / max(n_cols, 1), 4)},

@@ -164,6 +164,20 @@ int qs_atom_gradients(qs_ctx *ctx, const qs_atom *atoms, int natom, const qs_pot
 int qs_set_batch(qs_ctx *ctx, int batch);
 int qs_get_batch(const qs_ctx *ctx);
 
+/* Replay the slice loop of qs_run_stem as a HIP graph: captured on the first call
+ * of a given shape (probe count, slices, bins, outputs, buffers) and relaunched
+ * with one hipGraphLaunch while the shape repeats; any setter above drops it.
+ * Per-launch profiling (qs_set_profiling) bypasses the graph.  Replaces the
+ * per-pixel runMulsSTEM calls of the scan loop (stem3/stem3.cpp:2820-2946);
+ * the reference has no launch graph. */
+int qs_set_graph(qs_ctx *ctx, int on);
+
+/* Launch shape of the two sweeps (0 = library default): row-sweep steps per
+ * wavefront (a power of two) and column tiles per block; qs_get_tuning also
+ * reports the probe group size of qs_set_batch.  No reference counterpart. */
+int qs_set_tuning(qs_ctx *ctx, int rows_per_wave, int tiles_per_block);
+int qs_get_tuning(const qs_ctx *ctx, int *rows_per_wave, int *tiles_per_block, int *batch);
+
 /* Device time of the last qs_run_stem in milliseconds (HIP events on the context stream). */
 float qs_last_run_ms(const qs_ctx *ctx);
 

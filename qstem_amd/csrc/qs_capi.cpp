@@ -107,6 +107,10 @@ struct qs_ctx {
   int batch = 0;
   int rows_per_wave = 0;  // cap on row-sweep steps per wavefront (0: kernel default)
   int tiles_per_block = 0;  // column-sweep tiles per block (0: launcher default)
+  // HIP graph of the qs_run_stem slice loop, reused while the call shape matches
+  bool use_graph = false;
+  hipGraphExec_t gexec = nullptr;
+  std::vector<long long> gkey;
   // derived tables
   int band_a = 0, band_b = 0, kc = 0, kcp = 0, ntiles = 0, tc = 0;
   float k2max = 0.f;
@@ -328,9 +332,16 @@ int qs_create(const qs_params* p, qs_ctx** out) {
   return QS_OK;
 }
 
+static void drop_graph(qs_ctx* c) {
+  if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
+  c->gexec = nullptr;
+  c->gkey.clear();
+}
+
 void qs_destroy(qs_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
+  drop_graph(c);
   DevBuf* bufs[] = {&c->twx, &c->twy, &c->px, &c->py, &c->dkx2, &c->dky2, &c->probe, &c->trans, &c->scratch, &c->V,
                     &c->state, &c->ipos, &c->partial, &c->det, &c->diff, &c->exitw, &c->total, &c->pkx2, &c->pky2,
                     &c->probe_partial, &c->rx.tw, &c->rx.chirp_f, &c->rx.vhat_f, &c->rx.chirp_b, &c->rx.vhat_b,
@@ -395,6 +406,7 @@ static int build_trans(qs_ctx* c, int nslice) {
 
 int qs_set_potential(qs_ctx* c, const float* V, int nslice) {
   if (!c || !V || nslice < 1) return fail(c, QS_ERR_ARG, "qs_set_potential: bad arguments");
+  drop_graph(c);  // kernel arguments baked into a captured graph may change
   const qs_params& p = c->p;
   const size_t n = (size_t)nslice * p.potNx * p.potNy;
   HIPCHK(c, c->V.ensure(sizeof(float) * n));
@@ -433,6 +445,7 @@ int qs_set_scattering_table(qs_ctx* c, const double* table) {
 
 int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_params* pp, float* V_out) {
   if (!c || !pp || natom < 0 || (natom > 0 && !atoms)) return fail(c, QS_ERR_ARG, "qs_build_potential: bad arguments");
+  drop_graph(c);  // kernel arguments baked into a captured graph may change
   if (c->scat.size() != 93 * 123) return fail(c, QS_ERR_STATE, "qs_build_potential: scattering table not set");
   const qs_params& p = c->p;
   for (int i = 0; i < natom; i++) {
@@ -544,6 +557,7 @@ int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_
 
 int qs_set_trans(qs_ctx* c, const float* trans, int nslice) {
   if (!c || !trans || nslice < 1) return fail(c, QS_ERR_ARG, "qs_set_trans: bad arguments");
+  drop_graph(c);  // kernel arguments baked into a captured graph may change
   const size_t n = (size_t)nslice * c->p.potNx * c->p.potNy;
   HIPCHK(c, c->trans.ensure(sizeof(float2) * n));
   HIPCHK(c, hipMemcpyAsync(c->trans.p, trans, sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
@@ -564,6 +578,7 @@ int qs_get_trans(qs_ctx* c, float* out) {
 
 int qs_make_probe(qs_ctx* c, float* out) {
   if (!c) return QS_ERR_ARG;
+  drop_graph(c);  // kernel arguments baked into a captured graph may change
   const qs_params& p = c->p;
   const int nx = p.nx, ny = p.ny;
   ProbeArgs a{};
@@ -635,6 +650,7 @@ int qs_make_probe(qs_ctx* c, float* out) {
 
 int qs_set_optics(qs_ctx* c, const qs_params* p) {
   if (!c || !p) return fail(c, QS_ERR_ARG, "qs_set_optics: bad arguments");
+  drop_graph(c);  // kernel arguments baked into a captured graph may change
   qs_params& q = c->p;
   q.Cs = p->Cs; q.C5 = p->C5; q.df0 = p->df0; q.astigMag = p->astigMag; q.astigAngle = p->astigAngle;
   q.a33 = p->a33; q.a31 = p->a31; q.a44 = p->a44; q.a42 = p->a42; q.a55 = p->a55;
@@ -671,6 +687,29 @@ int qs_record_bins(int slices, int cellDiv, int outputInterval, int totalSliceCo
 int qs_set_batch(qs_ctx* c, int batch) {
   if (!c || batch < 0) return fail(c, QS_ERR_ARG, "qs_set_batch: bad arguments");
   c->batch = batch;
+  return QS_OK;
+}
+
+int qs_set_graph(qs_ctx* c, int on) {
+  if (!c) return QS_ERR_ARG;
+  c->use_graph = on != 0;
+  if (!c->use_graph) drop_graph(c);
+  return QS_OK;
+}
+
+int qs_set_tuning(qs_ctx* c, int rows_per_wave, int tiles_per_block) {
+  if (!c || rows_per_wave < 0 || tiles_per_block < 0 || (rows_per_wave & (rows_per_wave - 1)) != 0)
+    return fail(c, QS_ERR_ARG, "qs_set_tuning: rows_per_wave must be 0 or a power of two, tiles_per_block >= 0");
+  c->rows_per_wave = rows_per_wave;
+  c->tiles_per_block = tiles_per_block;
+  return QS_OK;
+}
+
+int qs_get_tuning(const qs_ctx* c, int* rows_per_wave, int* tiles_per_block, int* batch) {
+  if (!c || !rows_per_wave || !tiles_per_block || !batch) return QS_ERR_ARG;
+  *rows_per_wave = c->rows_per_wave;
+  *tiles_per_block = c->tiles_per_block;
+  *batch = c->batch;
   return QS_OK;
 }
 
@@ -789,32 +828,66 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
     return hipEventRecord(c->evpool[nev++], c->stream);
   };
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  for (int p0 = 0; p0 < nprobe; p0 += B) {
-    const int nb = (nprobe - p0) < B ? (nprobe - p0) : B;
-    a.nprobe = nb;
-    a.probe0 = p0;
-    for (int s = 0; s < nslice; s++) {
-      a.trans = c->trans.as<float2>() + (size_t)s * slice_elems;
-      const bool last = (s == nslice - 1);
-      HIPCHK(c, mark(0, true));
-      qs::launch_sweep_rows(a, (s == 0 && start == 0) ? 0 : 1, c->stream);
-      HIPCHK(c, mark(0, false));
-      const int bin = (want_det && bin_of_slice) ? bin_of_slice[s] : -1;
-      const bool tot = last && int_intensity;
-      a.rec_bin = (bin >= 0 || tot) ? (bin >= 0 ? bin : 0) : -1;
-      a.diff = (last && diff) ? c->diff.as<float>() : nullptr;
-      HIPCHK(c, mark(1, true));
-      qs::launch_sweep_cols(a, a.rec_bin >= 0, a.diff != nullptr, c->stream);
-      HIPCHK(c, mark(1, false));
-      if (bin >= 0 || tot) {
-        float* dbin = bin >= 0 ? c->det.as<float>() + (size_t)bin * p.ndet * nprobe : nullptr;
-        qs::launch_reduce_partials(c->partial.as<float>(), nb, c->ntiles, bin >= 0 ? p.ndet : 0, p.electronScale, dbin,
-                                   nprobe, tot ? c->total.as<float>() : nullptr, p0, c->stream);
+  auto enqueue = [&]() -> hipError_t {
+    for (int p0 = 0; p0 < nprobe; p0 += B) {
+      const int nb = (nprobe - p0) < B ? (nprobe - p0) : B;
+      a.nprobe = nb;
+      a.probe0 = p0;
+      for (int s = 0; s < nslice; s++) {
+        a.trans = c->trans.as<float2>() + (size_t)s * slice_elems;
+        const bool last = (s == nslice - 1);
+        hipError_t e = mark(0, true);
+        if (e != hipSuccess) return e;
+        qs::launch_sweep_rows(a, (s == 0 && start == 0) ? 0 : 1, c->stream);
+        if ((e = mark(0, false)) != hipSuccess) return e;
+        const int bin = (want_det && bin_of_slice) ? bin_of_slice[s] : -1;
+        const bool tot = last && int_intensity;
+        a.rec_bin = (bin >= 0 || tot) ? (bin >= 0 ? bin : 0) : -1;
+        a.diff = (last && diff) ? c->diff.as<float>() : nullptr;
+        if ((e = mark(1, true)) != hipSuccess) return e;
+        qs::launch_sweep_cols(a, a.rec_bin >= 0, a.diff != nullptr, c->stream);
+        if ((e = mark(1, false)) != hipSuccess) return e;
+        if (bin >= 0 || tot) {
+          float* dbin = bin >= 0 ? c->det.as<float>() + (size_t)bin * p.ndet * nprobe : nullptr;
+          qs::launch_reduce_partials(c->partial.as<float>(), nb, c->ntiles, bin >= 0 ? p.ndet : 0, p.electronScale,
+                                     dbin, nprobe, tot ? c->total.as<float>() : nullptr, p0, c->stream);
+        }
       }
+      if (exit_wave) qs::launch_sweep_rows(a, 2, c->stream);
     }
-    if (exit_wave) qs::launch_sweep_rows(a, 2, c->stream);
+    return hipGetLastError();
+  };
+  if (c->use_graph && !c->profiling) {
+    // the graph bakes in every kernel argument: key on the call shape and the buffers
+    std::vector<long long> key = {nprobe, B, nslice, start, nbins, want_det, diff != nullptr, exit_wave != nullptr,
+                                  int_intensity != nullptr, (long long)c->trans.p, (long long)c->state.p,
+                                  (long long)c->det.p, (long long)c->diff.p, (long long)c->exitw.p,
+                                  (long long)c->total.p, (long long)c->partial.p, (long long)c->ipos.p,
+                                  (long long)c->probe.p, c->rows_per_wave, c->tiles_per_block};
+    for (int s = 0; s < nslice; s++) key.push_back(bin_of_slice ? bin_of_slice[s] : -1);
+    if (!c->gexec || key != c->gkey) {
+      drop_graph(c);
+      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+      const hipError_t le = enqueue();
+      hipGraph_t g = nullptr;
+      const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+      if (le != hipSuccess || ce != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return fail(c, QS_ERR_HIP, std::string("qs_run_stem: graph capture failed: ") +
+                                       hipGetErrorString(le != hipSuccess ? le : ce));
+      }
+      const hipError_t ie = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(g);
+      if (ie != hipSuccess) {
+        c->gexec = nullptr;
+        return fail(c, QS_ERR_HIP, std::string("qs_run_stem: hipGraphInstantiate: ") + hipGetErrorString(ie));
+      }
+      c->gkey = key;
+    }
+    HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+  } else {
+    HIPCHK(c, enqueue());
   }
-  HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   if (want_det)
     HIPCHK(c, hipMemcpyAsync(det, c->det.p, sizeof(float) * (size_t)nbins * p.ndet * nprobe, hipMemcpyDeviceToHost,

@@ -260,6 +260,21 @@ class Multislice:
     def set_batch(self, batch: int):
         self._check(self._lib.qs_set_batch(self._h, int(batch)), "qs_set_batch")
 
+    def set_graph(self, on: bool = True):
+        """Replay run_stem's slice loop as a HIP graph (captured once per call shape)."""
+        self._check(self._lib.qs_set_graph(self._h, 1 if on else 0), "qs_set_graph")
+
+    def set_tuning(self, rows_per_wave: int = 0, tiles_per_block: int = 0):
+        """Launch shape of the two sweeps (0 = library default)."""
+        self._check(self._lib.qs_set_tuning(self._h, int(rows_per_wave), int(tiles_per_block)), "qs_set_tuning")
+
+    def get_tuning(self):
+        """(rows_per_wave, tiles_per_block, batch) currently set (0 = library default)."""
+        r, t, b = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self._lib.qs_get_tuning(self._h, ctypes.byref(r), ctypes.byref(t), ctypes.byref(b)),
+                    "qs_get_tuning")
+        return r.value, t.value, b.value
+
     def set_profiling(self, on: bool = True):
         self._check(self._lib.qs_set_profiling(self._h, 1 if on else 0), "qs_set_profiling")
 

@@ -127,3 +127,33 @@ def test_continue_equals_one_run(oracle):
         ms.set_potential(V[3:])
         part = ms.run_stem(ipos, nslice=3, start=1, want_exit=True, want_det=False)["exit"]
     assert rel_l2(part, full) < 1e-6
+
+
+def test_graph_replay_equals_eager():
+    """The HIP-graph replay of the slice loop equals eager launches, on repeated
+    replays with eager work in between and with new window offsets each call."""
+    cfg = small_config(nx=256, ny=256, pad=24, slices=5, dx=0.03)
+    V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=12)
+    ipos_a = probe_positions(cfg, 6, seed=2)
+    ipos_b = probe_positions(cfg, 6, seed=3)
+    bins = [-1, 0, -1, -1, 1]
+    kw = dict(bins=np.array(bins, np.int32), nbins=2, want_diff=True, want_exit=True, want_intensity=True)
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        ms.make_probe()
+        ms.set_batch(4)  # two probe groups per call
+        ref_a = ms.run_stem(ipos_a, **kw)
+        ref_b = ms.run_stem(ipos_b, **kw)
+        ms.set_graph(True)
+        outs = []
+        for ip in (ipos_a, ipos_b, ipos_a, ipos_b):
+            outs.append(ms.run_stem(ip, **kw))
+            ms.get_trans()  # eager work between replays
+        ms.set_graph(False)
+        again = ms.run_stem(ipos_a, **kw)
+    for out, ref in zip(outs, (ref_a, ref_b, ref_a, ref_b)):
+        assert np.array_equal(out["exit"], ref["exit"])
+        assert rel_l2(out["det"], ref["det"]) < 1e-6
+        assert rel_l2(out["diff"], ref["diff"]) < 1e-6
+        assert np.allclose(out["intensity"], ref["intensity"], rtol=1e-6)
+    assert np.array_equal(again["exit"], ref_a["exit"])
