@@ -189,6 +189,11 @@ def main():
     psi0 = ms.make_probe()
     t_probe = time.perf_counter() - t0
     ms.set_batch(a.group)
+    tuning = None
+    if a.group == 0:  # autotuner picks group size and sweep launch shapes (not timed)
+        t0 = time.perf_counter()
+        tuning = ms.autotune(ipos_all[:512])
+        tuning["seconds"] = round(time.perf_counter() - t0, 3)
 
     nbatches = (ipos_all.shape[0] + P - 1) // P
 
@@ -224,7 +229,10 @@ def main():
     value = probe_slices / elapsed
 
     # per-probe GPU loop: same kernels, one probe per launch sequence
+    # (library default launch shapes, which suit one probe per launch)
+    tuned = ms.get_tuning()
     ms.set_batch(1)
+    ms.set_tuning(0, 0)
     sl = ipos_all[:a.loop_probes]
     ms.run_stem(sl[:1], bins=bins, nbins=nbins)
     torch.cuda.synchronize()
@@ -232,7 +240,8 @@ def main():
     loop_out = ms.run_stem(sl, bins=bins, nbins=nbins)
     torch.cuda.synchronize()
     loop_rate = a.loop_probes * nslice / (time.perf_counter() - t0)
-    ms.set_batch(a.group)
+    ms.set_tuning(tuned[0], tuned[1])
+    ms.set_batch(tuned[2])
 
     # roofline of the slice step (row sweep + column sweep, one logical kernel in two launches)
     N2 = cfg.nx * cfg.ny
@@ -273,6 +282,7 @@ def main():
         "batched_over_per_probe": round(value / loop_rate, 2),
         "kernel_ms": {"rows_sweep": round(ms_rows / max(n_rows, 1), 4), "cols_sweep": round(ms_cols / max(n_cols, 1), 4)},
         "setup_s": {"transmission": round(t_pot, 3), "probe": round(t_probe, 4)},
+        "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "slice step = k_sweep_rows<512,1> + k_sweep_cols<512>",
@@ -292,7 +302,6 @@ def main():
             det_o, wall, per = cpu_baseline(cfg, trans, psi0, ipos_all[:ns], bins, nbins, ns, tmpdir)
         finally:
             shutil.rmtree(tmpdir, ignore_errors=True)
-        ms.set_batch(a.group)
         det_g = ms.run_stem(ipos_all[:ns], bins=bins, nbins=nbins)["det"]
         errs = []
         for b in range(nbins):

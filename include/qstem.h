@@ -178,6 +178,15 @@ int qs_set_graph(qs_ctx *ctx, int on);
 int qs_set_tuning(qs_ctx *ctx, int rows_per_wave, int tiles_per_block);
 int qs_get_tuning(const qs_ctx *ctx, int *rows_per_wave, int *tiles_per_block, int *batch);
 
+/* Autotuner: times the slice loop on this context's transmission function and
+ * probe (first 4 slices, up to 512 of the given window offsets) over probe group
+ * sizes, row-sweep steps per wavefront and column tiles per block, applies the
+ * fastest (as qs_set_batch + qs_set_tuning) and reports it (outputs nullable;
+ * rate in probe-slices/s).  Results are cached per device and wave geometry for
+ * the life of the process.  No reference counterpart. */
+int qs_autotune(qs_ctx *ctx, const int *ipos, int nprobe, int *batch, int *rows_per_wave, int *tiles_per_block,
+                float *rate);
+
 /* Device time of the last qs_run_stem in milliseconds (HIP events on the context stream). */
 float qs_last_run_ms(const qs_ctx *ctx);
 

@@ -85,6 +85,7 @@ SIGNATURES = {
     "qs_set_graph": (_I, [_P, _I]),
     "qs_set_tuning": (_I, [_P, _I, _I]),
     "qs_get_tuning": (_I, [_P, _IP, _IP, _IP]),
+    "qs_autotune": (_I, [_P, _IP, _I, _IP, _IP, _IP, _FP]),
     "qs_last_run_ms": (_F, [_P]),
     "qs_set_profiling": (_I, [_P, _I]),
     "qs_kernel_times": (_I, [_P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_double),

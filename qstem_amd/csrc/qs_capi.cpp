@@ -13,7 +13,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
+#include <mutex>
 #include <string>
+#include <tuple>
 #include <vector>
 
 #include "../../include/qstem.h"
@@ -1098,3 +1101,96 @@ int qs_atom_gradients(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_p
   return QS_OK;
 }
 }  // extern "C"
+
+// ---------------------------------------------------------------------------------
+// Autotuner: picks the probe group size (batch) and the launch shape of the two
+// sweeps by timing the slice loop on the context's own transmission function and
+// probe, over the first few slices and up to 512 of the caller's window offsets.
+// Coordinate descent: group size first (library launch defaults), then row-sweep
+// steps per wavefront, then column tiles per block.  Results are cached per
+// device and wave geometry for the life of the process.
+namespace {
+struct TuneKey {
+  int dev, nx, ny, kcp, potNy;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(dev, nx, ny, kcp, potNy) < std::tie(o.dev, o.nx, o.ny, o.kcp, o.potNy);
+  }
+};
+struct TuneVal {
+  int batch, rw, tpb;
+  float rate;
+};
+std::mutex g_tune_mu;
+std::map<TuneKey, TuneVal> g_tune_cache;
+}  // namespace
+
+int qs_autotune(qs_ctx* c, const int* ipos, int nprobe, int* batch, int* rows_per_wave, int* tiles_per_block,
+                float* rate) {
+  if (!c || !ipos || nprobe < 1) return fail(c, QS_ERR_ARG, "qs_autotune: bad arguments");
+  if (c->trans_slices < 1 || !c->have_probe)
+    return fail(c, QS_ERR_STATE, "qs_autotune: set the potential and the probe first");
+  int dev = 0;
+  HIPCHK(c, hipGetDevice(&dev));
+  const TuneKey key{dev, c->p.nx, c->p.ny, c->kcp, c->p.potNy};
+  TuneVal best{0, 0, 0, 0.f};
+  bool cached = false;
+  {
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    auto it = g_tune_cache.find(key);
+    if (it != g_tune_cache.end()) {
+      best = it->second;
+      cached = true;
+    }
+  }
+  if (!cached) {
+    const int n = std::min(nprobe, 512);
+    const int S = std::min(4, c->trans_slices);
+    const bool graph = c->use_graph, prof = c->profiling;
+    c->use_graph = false;
+    c->profiling = false;
+    auto measure = [&](int B, int rw, int tpb, float* out) -> int {
+      c->batch = B;
+      c->rows_per_wave = rw;
+      c->tiles_per_block = tpb;
+      float tmin = 1e30f;
+      for (int r = 0; r < 3; r++) {  // first run also pays first-touch costs
+        const int rc = qs_run_stem(c, ipos, n, 0, nullptr, S, 0, nullptr, nullptr, nullptr, nullptr);
+        if (rc != QS_OK) return rc;
+        if (r > 0) tmin = std::min(tmin, c->last_ms);
+      }
+      *out = (float)((double)n * S / (tmin * 1e-3));
+      return QS_OK;
+    };
+    auto consider = [&](int B, int rw, int tpb) -> int {
+      float r = 0.f;
+      const int rc = measure(B, rw, tpb, &r);
+      if (rc != QS_OK) return rc;
+      if (r > best.rate) best = TuneVal{B, rw, tpb, r};
+      return QS_OK;
+    };
+    int rc = QS_OK;
+    const int Bs[] = {16, 32, 64, 96, 128, 192, 256, 384, 512};
+    for (int B : Bs)
+      if (rc == QS_OK && (B <= n || B == Bs[0])) rc = consider(std::min(B, n), 0, 0);
+    const int B0 = best.batch;
+    for (int rw : {1, 2, 4, 8})
+      if (rc == QS_OK) rc = consider(B0, rw, best.tpb);
+    const int rw0 = best.rw;
+    for (int tpb : {1, 2, 4})
+      if (rc == QS_OK) rc = consider(B0, rw0, tpb);
+    c->use_graph = graph;
+    c->profiling = prof;
+    if (rc != QS_OK) return rc;
+    std::lock_guard<std::mutex> lk(g_tune_mu);
+    g_tune_cache[key] = best;
+  }
+  c->batch = best.batch;
+  c->rows_per_wave = best.rw;
+  c->tiles_per_block = best.tpb;
+  drop_graph(c);
+  if (batch) *batch = best.batch;
+  if (rows_per_wave) *rows_per_wave = best.rw;
+  if (tiles_per_block) *tiles_per_block = best.tpb;
+  if (rate) *rate = best.rate;
+  return QS_OK;
+}

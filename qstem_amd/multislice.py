@@ -275,6 +275,16 @@ class Multislice:
                     "qs_get_tuning")
         return r.value, t.value, b.value
 
+    def autotune(self, ipos) -> dict:
+        """Time group sizes and sweep launch shapes on this context's potential and probe,
+        apply the fastest and return it: {'batch', 'rows_per_wave', 'tiles_per_block', 'rate'}."""
+        ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
+        b, r, t = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        rate = ctypes.c_float()
+        self._check(self._lib.qs_autotune(self._h, _iptr(ipos), ipos.shape[0], ctypes.byref(b), ctypes.byref(r),
+                                          ctypes.byref(t), ctypes.byref(rate)), "qs_autotune")
+        return {"batch": b.value, "rows_per_wave": r.value, "tiles_per_block": t.value, "rate": rate.value}
+
     def set_profiling(self, on: bool = True):
         self._check(self._lib.qs_set_profiling(self._h, 1 if on else 0), "qs_set_profiling")
 

@@ -157,3 +157,21 @@ def test_graph_replay_equals_eager():
         assert rel_l2(out["diff"], ref["diff"]) < 1e-6
         assert np.allclose(out["intensity"], ref["intensity"], rtol=1e-6)
     assert np.array_equal(again["exit"], ref_a["exit"])
+
+
+def test_autotune_keeps_results():
+    """The autotuner only changes launch shapes: results after tuning equal the defaults."""
+    cfg = small_config(nx=256, ny=256, pad=24, slices=4, dx=0.03)
+    V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=13)
+    ipos = probe_positions(cfg, 40, seed=4)
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        ms.make_probe()
+        ref = ms.run_stem(ipos, want_exit=True)
+        best = ms.autotune(ipos)
+        assert best["batch"] in (16, 32, 40) and best["rate"] > 0
+        assert best["rows_per_wave"] in (1, 2, 4, 8) and best["tiles_per_block"] in (1, 2, 4)
+        assert ms.get_tuning() == (best["rows_per_wave"], best["tiles_per_block"], best["batch"])
+        out = ms.run_stem(ipos, want_exit=True)
+    assert np.array_equal(out["exit"], ref["exit"])
+    assert rel_l2(out["det"], ref["det"]) < 1e-6
