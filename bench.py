@@ -168,6 +168,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=8, help="probes for the oracle CPU baseline (0 = skip)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches only")
+    ap.add_argument("--rows-per-wave", type=int, default=0, help="with --group: row-sweep steps per wavefront")
+    ap.add_argument("--tiles-per-block", type=int, default=0, help="with --group: column tiles per block")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: this build runs on one GPU only (no multi-GPU path); got --gpus %d" % a.gpus,
@@ -189,6 +191,7 @@ def main():
     psi0 = ms.make_probe()
     t_probe = time.perf_counter() - t0
     ms.set_batch(a.group)
+    ms.set_tuning(a.rows_per_wave, a.tiles_per_block)
     tuning = None
     if a.group == 0:  # autotuner picks group size and sweep launch shapes (not timed)
         t0 = time.perf_counter()

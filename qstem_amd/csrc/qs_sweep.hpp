@@ -232,6 +232,17 @@ struct TileStage {
   }
 };
 
+// Work item of block blockIdx.x such that work items 2j and 2j+1 (the two
+// 64-byte halves of the state's 128-byte lines when a block stages one tile)
+// run back to back on the same XCD: blocks are dispatched round-robin over the
+// 8 XCDs, each with its own L2, so blocks b and b+8 share an L2.  The last
+// partial group of 16 blocks keeps the identity mapping.
+__device__ __forceinline__ int xcd_pair_block(int blk, int nblk) {
+  if (blk >= (nblk & ~15)) return blk;
+  const int xcd = blk & 7, slot = blk >> 3;
+  return (slot >> 1) * 16 + xcd * 2 + (slot & 1);
+}
+
 // Sum over the 64 lanes of a wavefront (result valid in every lane).
 __device__ __forceinline__ float wave_sum(float s) {
 #pragma unroll
@@ -254,8 +265,9 @@ __global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
   __shared__ float red[kMaxDet + 1];
 
   const int bpp = (a.ntiles + a.tpb - 1) / a.tpb;  // blocks per probe
-  const int p = blockIdx.x / bpp;
-  const int tile_first = (blockIdx.x % bpp) * a.tpb;
+  const int b = xcd_pair_block(blockIdx.x, gridDim.x);
+  const int p = b / bpp;
+  const int tile_first = (b % bpp) * a.tpb;
   const int tile_end = min(tile_first + a.tpb, a.ntiles);
   const int pg = a.probe0 + p;
   const TileStage<NX> stage(a.state + (size_t)pg * a.nx * a.kcp, a.nx, a.kcp);
