@@ -145,6 +145,49 @@ int qs_record_bins(int slices, int cellDiv, int outputInterval, int totalSliceCo
 int qs_run_stem(qs_ctx *ctx, const int *ipos, int nprobe, int start, const int *bin_of_slice, int nslice, int nbins,
                 float *det, float *diff, float *exit_wave, float *int_intensity);
 
+/* Frozen-phonon configurations as a batch axis.  The context holds nconfig
+ * transmission functions (one per phonon configuration, same slices); the
+ * potential setters (qs_set_potential, qs_set_trans, qs_build_potential) fill
+ * the configuration chosen by qs_select_config, configuration 0 first (it fixes
+ * the slice count).  qs_set_configs drops all loaded configurations.  Replaces
+ * the reference's serial TDS loop of doSTEM (stem3/stem3.cpp:2769-2950), which
+ * rebuilds one potential per run. */
+int qs_set_configs(qs_ctx *ctx, int nconfig);
+int qs_select_config(qs_ctx *ctx, int config);
+
+/* qs_run_stem with item i = (window offset ipos[i], configuration config[i]);
+ * config NULL means configuration 0 for every item. */
+int qs_run_stem_cfg(qs_ctx *ctx, const int *ipos, const int *config, int nprobe, int start, const int *bin_of_slice,
+                    int nslice, int nbins, float *det, float *diff, float *exit_wave, float *int_intensity);
+
+/* Use src's transmission functions (all configurations) instead of dst's own:
+ * contexts of different window sizes over one potential grid (same potNx/potNy,
+ * resolution, voltage and band limit).  src must outlive dst's runs; src NULL
+ * returns dst to its own.  No reference counterpart (one window size per run). */
+int qs_share_trans(qs_ctx *dst, qs_ctx *src);
+
+/* One window size of an uneven batch: a context (its nx, ny, probe, detectors)
+ * and its items.  det [nbins][ndet][nitem] and int_intensity [nitem] as in
+ * qs_run_stem (nullable). */
+typedef struct qs_group {
+  qs_ctx *ctx;
+  const int *ipos;   /* [nitem][2] */
+  const int *config; /* [nitem] or NULL */
+  int nitem;
+  float *det;
+  float *int_intensity;
+} qs_group;
+
+/* Uneven batch: every group starts from its probe and advances through nslice
+ * slices of the shared transmission functions; within each slice the groups
+ * follow each other on the first group's stream, so a slice is read by every
+ * window size while it is cache resident.  All contexts must share one set of
+ * transmission functions (qs_share_trans); each group uses its own batch size
+ * and launch tuning.  qs_last_run_ms of the first context reports the device
+ * time.  No reference counterpart (the reference runs one window size per
+ * program invocation). */
+int qs_run_groups(const qs_group *groups, int ngroup, const int *bin_of_slice, int nslice, int nbins);
+
 /* Adjoint of qs_run_stem for the loss L = sum_p sum_k (I_p(k) - target_p(k))^2, where
  * I_p(k) = |Psi_p(k) P(k)|^2/(nx ny)^2 is the diffraction intensity after the last slice
  * (fraction of the incident beam per pixel, FFTW order [nprobe][nx][ny]).  All
@@ -179,7 +222,7 @@ int qs_set_tuning(qs_ctx *ctx, int rows_per_wave, int tiles_per_block);
 int qs_get_tuning(const qs_ctx *ctx, int *rows_per_wave, int *tiles_per_block, int *batch);
 
 /* Autotuner: times the slice loop on this context's transmission function and
- * probe (first 4 slices, up to 512 of the given window offsets) over probe group
+ * probe (first 8 slices, up to 512 of the given window offsets) over probe group
  * sizes, row-sweep steps per wavefront and column tiles per block, applies the
  * fastest (as qs_set_batch + qs_set_tuning) and reports it (outputs nullable;
  * rate in probe-slices/s).  Results are cached per device and wave geometry for

@@ -10,5 +10,6 @@ from .multislice import (  # noqa: F401
     StemConfig,
     MILLISEC_PICOAMP,
     record_bins,
+    run_groups,
     scan_offsets,
 )

@@ -61,6 +61,13 @@ class QsPotParams(ctypes.Structure):
                 ("centerSlices", _I), ("periodicXY", _I), ("tds", _I), ("compat_q6", _I), ("lut_dr_fix", _I)]
 
 
+class QsGroup(ctypes.Structure):
+    """struct qs_group of include/qstem.h."""
+    _fields_ = [("ctx", ctypes.c_void_p), ("ipos", ctypes.POINTER(ctypes.c_int)), ("config", ctypes.POINTER(ctypes.c_int)),
+                ("nitem", ctypes.c_int), ("det", ctypes.POINTER(ctypes.c_float)),
+                ("int_intensity", ctypes.POINTER(ctypes.c_float))]
+
+
 # Symbols declared in include/qstem.h, with their ctypes signatures.
 _P = ctypes.c_void_p
 _FP = ctypes.POINTER(ctypes.c_float)
@@ -78,6 +85,11 @@ SIGNATURES = {
     "qs_build_potential": (_I, [_P, ctypes.POINTER(QsAtom), _I, ctypes.POINTER(QsPotParams), _FP]),
     "qs_record_bins": (_I, [_I, _I, _I, _I, _IP]),
     "qs_run_stem": (_I, [_P, _IP, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
+    "qs_run_stem_cfg": (_I, [_P, _IP, _IP, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
+    "qs_set_configs": (_I, [_P, _I]),
+    "qs_select_config": (_I, [_P, _I]),
+    "qs_share_trans": (_I, [_P, _P]),
+    "qs_run_groups": (_I, [ctypes.POINTER(QsGroup), _I, _IP, _I, _I]),
     "qs_adjoint": (_I, [_P, _IP, _I, _FP, _I, _FP, _FP, _FP]),
     "qs_atom_gradients": (_I, [_P, ctypes.POINTER(QsAtom), _I, ctypes.POINTER(QsPotParams), _FP]),
     "qs_set_batch": (_I, [_P, _I]),

@@ -110,6 +110,12 @@ struct qs_ctx {
   int batch = 0;
   int rows_per_wave = 0;  // cap on row-sweep steps per wavefront (0: kernel default)
   int tiles_per_block = 0;  // column-sweep tiles per block (0: launcher default)
+  // frozen-phonon configurations: nconfig transmission functions of trans_slices
+  // slices each, back to back in trans; the potential setters fill cur_config
+  int nconfig = 1;
+  int cur_config = 0;
+  qs_ctx* trans_src = nullptr;  // non-owning: the context whose transmission functions this one uses
+  DevBuf cfgidx;                // per-probe configuration of the current run
   // HIP graph of the qs_run_stem slice loop, reused while the call shape matches
   bool use_graph = false;
   hipGraphExec_t gexec = nullptr;
@@ -345,7 +351,7 @@ void qs_destroy(qs_ctx* c) {
   if (!c) return;
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
-  DevBuf* bufs[] = {&c->twx, &c->twy, &c->px, &c->py, &c->dkx2, &c->dky2, &c->probe, &c->trans, &c->scratch, &c->V,
+  DevBuf* bufs[] = {&c->cfgidx, &c->twx, &c->twy, &c->px, &c->py, &c->dkx2, &c->dky2, &c->probe, &c->trans, &c->scratch, &c->V,
                     &c->state, &c->ipos, &c->partial, &c->det, &c->diff, &c->exitw, &c->total, &c->pkx2, &c->pky2,
                     &c->probe_partial, &c->rx.tw, &c->rx.chirp_f, &c->rx.vhat_f, &c->rx.chirp_b, &c->rx.vhat_b,
                     &c->ry.tw, &c->ry.chirp_f, &c->ry.vhat_f, &c->ry.chirp_b, &c->ry.vhat_b,
@@ -387,23 +393,47 @@ static int bandlimit_buf(qs_ctx* c, float2* tr, int nslice) {
   return QS_OK;
 }
 
-// initSTEMSlices on the device-resident potential c->V.
+// The transmission functions a context runs on: its own, or those of the
+// context it shares them with (qs_share_trans).
+static qs_ctx* trans_owner(qs_ctx* c) { return c->trans_src ? c->trans_src : c; }
+static float2* trans_base(qs_ctx* c) { return trans_owner(c)->trans.as<float2>(); }
+static int trans_slices_of(qs_ctx* c) { return trans_owner(c)->trans_slices; }
+static int nconfig_of(qs_ctx* c) { return trans_owner(c)->nconfig; }
+
+// Room for nconfig configurations of nslice slices; returns the selected
+// configuration's first slice.  Configuration 0 fixes the slice count (and may
+// reallocate, dropping the others); the others must match it.
+static int trans_slot(qs_ctx* c, int nslice, const char* who, float2** out) {
+  const size_t per = (size_t)nslice * c->p.potNx * c->p.potNy;
+  if (c->trans_src) return fail(c, QS_ERR_STATE, std::string(who) + ": this context uses another's transmission functions");
+  if (c->cur_config > 0 && nslice != c->trans_slices)
+    return fail(c, QS_ERR_ARG, std::string(who) + ": every configuration needs the slice count of configuration 0");
+  if (c->cur_config == 0) {
+    HIPCHK(c, c->trans.ensure(sizeof(float2) * per * c->nconfig));
+    c->trans_slices = nslice;
+  }
+  *out = c->trans.as<float2>() + per * c->cur_config;
+  return QS_OK;
+}
+
+// initSTEMSlices on the device-resident potential c->V, into the selected configuration.
 static int build_trans(qs_ctx* c, int nslice) {
   const qs_params& p = c->p;
   const size_t n = (size_t)nslice * p.potNx * p.potNy;
-  HIPCHK(c, c->trans.ensure(sizeof(float2) * n));
+  float2* t = nullptr;
+  int rc = trans_slot(c, nslice, "potential", &t);
+  if (rc != QS_OK) return rc;
   // sigma = gamma*lambda with gamma = 1 + v0/511 in float (stemlib.cpp:2271-2274)
   const float mm0 = 1.0F + p.v0 / 511.0F;
   const double scale = mm0 * wavelength(p.v0);
-  qs::launch_phase_grating(c->V.as<float>(), c->trans.as<float2>(), n, scale, c->stream);
+  qs::launch_phase_grating(c->V.as<float>(), t, n, scale, c->stream);
   HIPCHK(c, hipGetLastError());
   if (p.bandlimittrans) {
     HIPCHK(c, c->scratch.ensure(sizeof(float2) * n));
-    int rc = bandlimit_buf(c, c->trans.as<float2>(), nslice);
+    rc = bandlimit_buf(c, t, nslice);
     if (rc != QS_OK) return rc;
   }
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->trans_slices = nslice;
   return QS_OK;
 }
 
@@ -562,19 +592,21 @@ int qs_set_trans(qs_ctx* c, const float* trans, int nslice) {
   if (!c || !trans || nslice < 1) return fail(c, QS_ERR_ARG, "qs_set_trans: bad arguments");
   drop_graph(c);  // kernel arguments baked into a captured graph may change
   const size_t n = (size_t)nslice * c->p.potNx * c->p.potNy;
-  HIPCHK(c, c->trans.ensure(sizeof(float2) * n));
-  HIPCHK(c, hipMemcpyAsync(c->trans.p, trans, sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
+  float2* t = nullptr;
+  const int rc = trans_slot(c, nslice, "qs_set_trans", &t);
+  if (rc != QS_OK) return rc;
+  HIPCHK(c, hipMemcpyAsync(t, trans, sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->trans_slices = nslice;
   c->v_valid = false;
   return QS_OK;
 }
 
 int qs_get_trans(qs_ctx* c, float* out) {
   if (!c || !out) return fail(c, QS_ERR_ARG, "qs_get_trans: bad arguments");
-  if (c->trans_slices < 1) return fail(c, QS_ERR_STATE, "no transmission function set");
-  const size_t n = (size_t)c->trans_slices * c->p.potNx * c->p.potNy;
-  HIPCHK(c, hipMemcpyAsync(out, c->trans.p, sizeof(float2) * n, hipMemcpyDeviceToHost, c->stream));
+  if (trans_slices_of(c) < 1) return fail(c, QS_ERR_STATE, "no transmission function set");
+  const size_t n = (size_t)trans_slices_of(c) * c->p.potNx * c->p.potNy;
+  const int k = c->trans_src ? 0 : c->cur_config;
+  HIPCHK(c, hipMemcpyAsync(out, trans_base(c) + n * k, sizeof(float2) * n, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   return QS_OK;
 }
@@ -700,6 +732,42 @@ int qs_set_graph(qs_ctx* c, int on) {
   return QS_OK;
 }
 
+int qs_set_configs(qs_ctx* c, int nconfig) {
+  if (!c || nconfig < 1) return fail(c, QS_ERR_ARG, "qs_set_configs: bad arguments");
+  if (c->trans_src) return fail(c, QS_ERR_STATE, "qs_set_configs: this context uses another's transmission functions");
+  drop_graph(c);
+  c->nconfig = nconfig;
+  c->cur_config = 0;
+  c->trans_slices = 0;  // every configuration is (re)loaded, configuration 0 first
+  c->v_valid = false;
+  return QS_OK;
+}
+
+int qs_select_config(qs_ctx* c, int config) {
+  if (!c || config < 0 || config >= c->nconfig) return fail(c, QS_ERR_ARG, "qs_select_config: index out of range");
+  if (config > 0 && c->trans_slices < 1) return fail(c, QS_ERR_STATE, "qs_select_config: load configuration 0 first");
+  c->cur_config = config;
+  return QS_OK;
+}
+
+int qs_share_trans(qs_ctx* dst, qs_ctx* src) {
+  if (!dst || dst == src) return fail(dst, QS_ERR_ARG, "qs_share_trans: bad arguments");
+  drop_graph(dst);
+  if (!src) {
+    dst->trans_src = nullptr;
+    return QS_OK;
+  }
+  if (src->trans_src) return fail(dst, QS_ERR_ARG, "qs_share_trans: the source shares another context's functions");
+  const qs_params &a = dst->p, &b = src->p;
+  if (a.potNx != b.potNx || a.potNy != b.potNy || a.resolutionX != b.resolutionX || a.resolutionY != b.resolutionY ||
+      a.v0 != b.v0 || a.bandlimittrans != b.bandlimittrans || dst->k2max != src->k2max)
+    return fail(dst, QS_ERR_ARG,
+                "qs_share_trans: potential grid, sampling, voltage and band limit must match (window sizes may differ)");
+  dst->trans_src = src;
+  dst->v_valid = false;
+  return QS_OK;
+}
+
 int qs_set_tuning(qs_ctx* c, int rows_per_wave, int tiles_per_block) {
   if (!c || rows_per_wave < 0 || tiles_per_block < 0 || (rows_per_wave & (rows_per_wave - 1)) != 0)
     return fail(c, QS_ERR_ARG, "qs_set_tuning: rows_per_wave must be 0 or a power of two, tiles_per_block >= 0");
@@ -737,25 +805,42 @@ int qs_kernel_times(const qs_ctx* c, double* ms_rows, double* ms_cols, long* n_r
   return QS_OK;
 }
 
-int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bin_of_slice, int nslice, int nbins,
-                float* det, float* diff, float* exit_wave, float* int_intensity) {
-  if (!c) return QS_ERR_ARG;
+// ---------------------------------------------------------------------------------
+// Multislice runs.  A RunPlan holds one context's validated call: device buffers,
+// kernel arguments, and the probe-group size; enqueue_slice issues the launches of
+// one probe group at one slice.  qs_run_stem advances each probe group through all
+// slices; qs_run_groups interleaves the groups of several contexts slice by slice.
+struct RunPlan {
+  qs_ctx* c = nullptr;
+  int nprobe = 0, start = 0, nslice = 0, nbins = 0, B = 0;
+  const int* bins = nullptr;
+  bool want_det = false, has_cfg = false;
+  float *det = nullptr, *diff = nullptr, *exit_wave = nullptr, *intensity = nullptr;
+  size_t slice_elems = 0;
+  SweepArgs a{};
+};
+
+static int plan_run(qs_ctx* c, const char* who, const int* ipos, const int* config, int nprobe, int start,
+                    const int* bin_of_slice, int nslice, int nbins, float* det, float* diff, float* exit_wave,
+                    float* int_intensity, RunPlan& r) {
   const qs_params& p = c->p;
-  if (!ipos || nprobe < 1 || nslice < 1 || (start != 0 && start != 1))
-    return fail(c, QS_ERR_ARG, "qs_run_stem: bad arguments");
-  if (nslice > c->trans_slices) return fail(c, QS_ERR_STATE, "qs_run_stem: fewer transmission slices than nslice");
-  if (start == 0 && !c->have_probe) return fail(c, QS_ERR_STATE, "qs_run_stem: qs_make_probe not called");
+  const std::string w(who);
+  if (!ipos || nprobe < 1 || nslice < 1 || (start != 0 && start != 1)) return fail(c, QS_ERR_ARG, w + ": bad arguments");
+  if (nslice > trans_slices_of(c)) return fail(c, QS_ERR_STATE, w + ": fewer transmission slices than nslice");
+  if (start == 0 && !c->have_probe) return fail(c, QS_ERR_STATE, w + ": qs_make_probe not called");
   if (start == 1 && (!c->state_valid || c->state_probes != nprobe))
-    return fail(c, QS_ERR_STATE, "qs_run_stem: no wave state to continue for this probe count");
+    return fail(c, QS_ERR_STATE, w + ": no wave state to continue for this probe count");
   for (int i = 0; i < nprobe; i++) {
     const int x = ipos[2 * i], y = ipos[2 * i + 1];
     if (x < 0 || y < 0 || x + p.nx > p.potNx || y + p.ny > p.potNy)
-      return fail(c, QS_ERR_ARG, "qs_run_stem: window offset outside the transmission array");
+      return fail(c, QS_ERR_ARG, w + ": window offset outside the transmission array");
+    if (config && (config[i] < 0 || config[i] >= nconfig_of(c)))
+      return fail(c, QS_ERR_ARG, w + ": configuration index out of range");
   }
   if (det && p.ndet > 0) {
-    if (!bin_of_slice || nbins < 1) return fail(c, QS_ERR_ARG, "qs_run_stem: det needs bin_of_slice and nbins");
+    if (!bin_of_slice || nbins < 1) return fail(c, QS_ERR_ARG, w + ": det needs bin_of_slice and nbins");
     for (int s = 0; s < nslice; s++)
-      if (bin_of_slice[s] >= nbins) return fail(c, QS_ERR_ARG, "qs_run_stem: bin index >= nbins");
+      if (bin_of_slice[s] >= nbins) return fail(c, QS_ERR_ARG, w + ": bin index >= nbins");
   }
   const int nx = p.nx, ny = p.ny;
   // probe group advanced through all slices together: the library default keeps the
@@ -764,15 +849,32 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
   const size_t per_probe = sizeof(float2) * (size_t)nx * c->kcp;
   int Bdef = (int)std::max<size_t>(32, std::min<size_t>(1024, (size_t)(180u << 20) / per_probe));
   Bdef = std::max(32, (Bdef / 16) * 16);
-  const int B = c->batch > 0 ? std::min(c->batch, nprobe) : std::min(Bdef, nprobe);
+  r = RunPlan{};
+  r.c = c;
+  r.nprobe = nprobe;
+  r.start = start;
+  r.nslice = nslice;
+  r.nbins = nbins;
+  r.bins = bin_of_slice;
+  r.B = c->batch > 0 ? std::min(c->batch, nprobe) : std::min(Bdef, nprobe);
+  r.want_det = det && p.ndet > 0;
+  r.det = det;
+  r.diff = diff;
+  r.exit_wave = exit_wave;
+  r.intensity = int_intensity;
+  r.has_cfg = config != nullptr;
+  r.slice_elems = (size_t)p.potNx * p.potNy;
   const size_t state_elems = (size_t)nprobe * nx * c->kcp;
   HIPCHK(c, c->state.ensure(sizeof(float2) * state_elems));
   if (start == 0) qs::launch_fill_zero(c->state.as<float2>(), state_elems, c->stream);  // padding columns stay zero
   HIPCHK(c, c->ipos.ensure(sizeof(int) * 2 * nprobe));
   HIPCHK(c, hipMemcpyAsync(c->ipos.p, ipos, sizeof(int) * 2 * nprobe, hipMemcpyHostToDevice, c->stream));
-  HIPCHK(c, c->partial.ensure(sizeof(float) * (size_t)B * c->ntiles * (qs::kMaxDet + 1)));
-  const bool want_det = det && p.ndet > 0;
-  if (want_det) {
+  if (config) {
+    HIPCHK(c, c->cfgidx.ensure(sizeof(int) * nprobe));
+    HIPCHK(c, hipMemcpyAsync(c->cfgidx.p, config, sizeof(int) * nprobe, hipMemcpyHostToDevice, c->stream));
+  }
+  HIPCHK(c, c->partial.ensure(sizeof(float) * (size_t)r.B * c->ntiles * (qs::kMaxDet + 1)));
+  if (r.want_det) {
     const size_t nd = (size_t)nbins * p.ndet * nprobe;
     HIPCHK(c, c->det.ensure(sizeof(float) * nd));
     HIPCHK(c, hipMemcpyAsync(c->det.p, det, sizeof(float) * nd, hipMemcpyHostToDevice, c->stream));
@@ -784,7 +886,7 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
   if (exit_wave) HIPCHK(c, c->exitw.ensure(sizeof(float2) * (size_t)nprobe * nx * ny));
   if (int_intensity) HIPCHK(c, c->total.ensure(sizeof(float) * nprobe));
 
-  SweepArgs a{};
+  SweepArgs& a = r.a;
   a.nx = nx;
   a.ny = ny;
   a.kc = c->kc;
@@ -797,6 +899,8 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
   a.tpb = c->tiles_per_block;
   a.probe = c->probe.as<float2>();
   a.ipos = c->ipos.as<int2>();
+  a.cfg = config ? c->cfgidx.as<int>() : nullptr;
+  a.cfg_stride = (long long)trans_slices_of(c) * (long long)r.slice_elems;
   a.state = c->state.as<float2>();
   a.out_wave = c->exitw.as<float2>();
   a.twx = c->twx.as<float2>();
@@ -815,11 +919,15 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
   }
   a.partial = c->partial.as<float>();
   a.diff_scale = (float)pow((double)nx * ny, 1.5);
-  const size_t slice_elems = (size_t)p.potNx * p.potNy;
-  // profiling: event pairs around every sweep launch, accumulated after the run
+  return QS_OK;
+}
+
+// Event pairs around every sweep launch (qs_set_profiling), accumulated after the run.
+struct LaunchMarks {
+  qs_ctx* c;
   std::vector<std::pair<int, int>> marks;  // (kind 0=rows 1=cols, first event index)
   size_t nev = 0;
-  auto mark = [&](int kind, bool begin) -> hipError_t {
+  hipError_t mark(int kind, bool begin, hipStream_t st) {
     if (!c->profiling) return hipSuccess;
     if (nev >= c->evpool.size()) {
       hipEvent_t e;
@@ -828,44 +936,82 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
       c->evpool.push_back(e);
     }
     if (begin) marks.push_back({kind, (int)nev});
-    return hipEventRecord(c->evpool[nev++], c->stream);
-  };
+    return hipEventRecord(c->evpool[nev++], st);
+  }
+};
+
+// Launches of probe group [p0, p0 + B) at slice s; the exit-wave sweep follows the last slice.
+static hipError_t enqueue_slice(RunPlan& r, int p0, int s, hipStream_t st, LaunchMarks* lm) {
+  qs_ctx* c = r.c;
+  const qs_params& p = c->p;
+  SweepArgs& a = r.a;
+  const int nb = std::min(r.B, r.nprobe - p0);
+  a.nprobe = nb;
+  a.probe0 = p0;
+  a.trans = trans_base(c) + (size_t)s * r.slice_elems;
+  const bool last = (s == r.nslice - 1);
+  hipError_t e = lm ? lm->mark(0, true, st) : hipSuccess;
+  if (e != hipSuccess) return e;
+  qs::launch_sweep_rows(a, (s == 0 && r.start == 0) ? 0 : 1, st);
+  if (lm && (e = lm->mark(0, false, st)) != hipSuccess) return e;
+  const int bin = (r.want_det && r.bins) ? r.bins[s] : -1;
+  const bool tot = last && r.intensity;
+  a.rec_bin = (bin >= 0 || tot) ? (bin >= 0 ? bin : 0) : -1;
+  a.diff = (last && r.diff) ? c->diff.as<float>() : nullptr;
+  if (lm && (e = lm->mark(1, true, st)) != hipSuccess) return e;
+  qs::launch_sweep_cols(a, a.rec_bin >= 0, a.diff != nullptr, st);
+  if (lm && (e = lm->mark(1, false, st)) != hipSuccess) return e;
+  if (bin >= 0 || tot) {
+    float* dbin = bin >= 0 ? c->det.as<float>() + (size_t)bin * p.ndet * r.nprobe : nullptr;
+    qs::launch_reduce_partials(c->partial.as<float>(), nb, c->ntiles, bin >= 0 ? p.ndet : 0, p.electronScale, dbin,
+                               r.nprobe, tot ? c->total.as<float>() : nullptr, p0, st);
+  }
+  if (last && r.exit_wave) qs::launch_sweep_rows(a, 2, st);
+  return hipSuccess;
+}
+
+// Device-to-host copies of the requested outputs on stream st (then synchronised).
+static int finish_run(RunPlan& r, hipStream_t st) {
+  qs_ctx* c = r.c;
+  const qs_params& p = c->p;
+  const size_t n2 = (size_t)r.nprobe * p.nx * p.ny;
+  if (r.want_det)
+    HIPCHK(c, hipMemcpyAsync(r.det, c->det.p, sizeof(float) * (size_t)r.nbins * p.ndet * r.nprobe,
+                             hipMemcpyDeviceToHost, st));
+  if (r.diff) HIPCHK(c, hipMemcpyAsync(r.diff, c->diff.p, sizeof(float) * n2, hipMemcpyDeviceToHost, st));
+  if (r.exit_wave) HIPCHK(c, hipMemcpyAsync(r.exit_wave, c->exitw.p, sizeof(float2) * n2, hipMemcpyDeviceToHost, st));
+  if (r.intensity)
+    HIPCHK(c, hipMemcpyAsync(r.intensity, c->total.p, sizeof(float) * r.nprobe, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  c->state_valid = true;
+  c->state_probes = r.nprobe;
+  return QS_OK;
+}
+
+int qs_run_stem_cfg(qs_ctx* c, const int* ipos, const int* config, int nprobe, int start, const int* bin_of_slice,
+                    int nslice, int nbins, float* det, float* diff, float* exit_wave, float* int_intensity) {
+  if (!c) return QS_ERR_ARG;
+  RunPlan r;
+  int rc = plan_run(c, "qs_run_stem", ipos, config, nprobe, start, bin_of_slice, nslice, nbins, det, diff, exit_wave,
+                    int_intensity, r);
+  if (rc != QS_OK) return rc;
+  LaunchMarks lm{c};
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   auto enqueue = [&]() -> hipError_t {
-    for (int p0 = 0; p0 < nprobe; p0 += B) {
-      const int nb = (nprobe - p0) < B ? (nprobe - p0) : B;
-      a.nprobe = nb;
-      a.probe0 = p0;
-      for (int s = 0; s < nslice; s++) {
-        a.trans = c->trans.as<float2>() + (size_t)s * slice_elems;
-        const bool last = (s == nslice - 1);
-        hipError_t e = mark(0, true);
+    for (int p0 = 0; p0 < r.nprobe; p0 += r.B)
+      for (int s = 0; s < r.nslice; s++) {
+        const hipError_t e = enqueue_slice(r, p0, s, c->stream, &lm);
         if (e != hipSuccess) return e;
-        qs::launch_sweep_rows(a, (s == 0 && start == 0) ? 0 : 1, c->stream);
-        if ((e = mark(0, false)) != hipSuccess) return e;
-        const int bin = (want_det && bin_of_slice) ? bin_of_slice[s] : -1;
-        const bool tot = last && int_intensity;
-        a.rec_bin = (bin >= 0 || tot) ? (bin >= 0 ? bin : 0) : -1;
-        a.diff = (last && diff) ? c->diff.as<float>() : nullptr;
-        if ((e = mark(1, true)) != hipSuccess) return e;
-        qs::launch_sweep_cols(a, a.rec_bin >= 0, a.diff != nullptr, c->stream);
-        if ((e = mark(1, false)) != hipSuccess) return e;
-        if (bin >= 0 || tot) {
-          float* dbin = bin >= 0 ? c->det.as<float>() + (size_t)bin * p.ndet * nprobe : nullptr;
-          qs::launch_reduce_partials(c->partial.as<float>(), nb, c->ntiles, bin >= 0 ? p.ndet : 0, p.electronScale,
-                                     dbin, nprobe, tot ? c->total.as<float>() : nullptr, p0, c->stream);
-        }
       }
-      if (exit_wave) qs::launch_sweep_rows(a, 2, c->stream);
-    }
     return hipGetLastError();
   };
   if (c->use_graph && !c->profiling) {
     // the graph bakes in every kernel argument: key on the call shape and the buffers
-    std::vector<long long> key = {nprobe, B, nslice, start, nbins, want_det, diff != nullptr, exit_wave != nullptr,
-                                  int_intensity != nullptr, (long long)c->trans.p, (long long)c->state.p,
-                                  (long long)c->det.p, (long long)c->diff.p, (long long)c->exitw.p,
-                                  (long long)c->total.p, (long long)c->partial.p, (long long)c->ipos.p,
+    std::vector<long long> key = {nprobe, r.B, nslice, start, nbins, r.want_det, diff != nullptr,
+                                  exit_wave != nullptr, int_intensity != nullptr, r.has_cfg,
+                                  (long long)trans_base(c), (long long)c->state.p, (long long)c->det.p,
+                                  (long long)c->diff.p, (long long)c->exitw.p, (long long)c->total.p,
+                                  (long long)c->partial.p, (long long)c->ipos.p, (long long)c->cfgidx.p,
                                   (long long)c->probe.p, c->rows_per_wave, c->tiles_per_block};
     for (int s = 0; s < nslice; s++) key.push_back(bin_of_slice ? bin_of_slice[s] : -1);
     if (!c->gexec || key != c->gkey) {
@@ -892,26 +1038,65 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
     HIPCHK(c, enqueue());
   }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-  if (want_det)
-    HIPCHK(c, hipMemcpyAsync(det, c->det.p, sizeof(float) * (size_t)nbins * p.ndet * nprobe, hipMemcpyDeviceToHost,
-                             c->stream));
-  if (diff)
-    HIPCHK(c, hipMemcpyAsync(diff, c->diff.p, sizeof(float) * (size_t)nprobe * nx * ny, hipMemcpyDeviceToHost, c->stream));
-  if (exit_wave)
-    HIPCHK(c, hipMemcpyAsync(exit_wave, c->exitw.p, sizeof(float2) * (size_t)nprobe * nx * ny, hipMemcpyDeviceToHost,
-                             c->stream));
-  if (int_intensity)
-    HIPCHK(c, hipMemcpyAsync(int_intensity, c->total.p, sizeof(float) * nprobe, hipMemcpyDeviceToHost, c->stream));
-  HIPCHK(c, hipStreamSynchronize(c->stream));
+  rc = finish_run(r, c->stream);
+  if (rc != QS_OK) return rc;
   HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
-  for (auto& m : marks) {
+  for (auto& m : lm.marks) {
     float ms = 0.f;
     HIPCHK(c, hipEventElapsedTime(&ms, c->evpool[m.second], c->evpool[m.second + 1]));
     if (m.first == 0) { c->ms_rows += ms; c->n_rows++; }
     else { c->ms_cols += ms; c->n_cols++; }
   }
-  c->state_valid = true;
-  c->state_probes = nprobe;
+  return QS_OK;
+}
+
+int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bin_of_slice, int nslice, int nbins,
+                float* det, float* diff, float* exit_wave, float* int_intensity) {
+  return qs_run_stem_cfg(c, ipos, nullptr, nprobe, start, bin_of_slice, nslice, nbins, det, diff, exit_wave,
+                         int_intensity);
+}
+
+int qs_run_groups(const qs_group* groups, int ngroup, const int* bin_of_slice, int nslice, int nbins) {
+  if (!groups || ngroup < 1 || !groups[0].ctx) return QS_ERR_ARG;
+  qs_ctx* c0 = groups[0].ctx;
+  const float2* t0 = trans_base(c0);
+  for (int g = 0; g < ngroup; g++) {
+    qs_ctx* c = groups[g].ctx;
+    if (!c) return fail(c0, QS_ERR_ARG, "qs_run_groups: null context");
+    if (trans_base(c) != t0 || trans_slices_of(c) != trans_slices_of(c0))
+      return fail(c0, QS_ERR_ARG, "qs_run_groups: contexts must share one transmission function (qs_share_trans)");
+    for (int h = 0; h < g; h++)
+      if (groups[h].ctx == c) return fail(c0, QS_ERR_ARG, "qs_run_groups: a context appears twice");
+  }
+  std::vector<RunPlan> plans(ngroup);
+  for (int g = 0; g < ngroup; g++) {
+    const qs_group& G = groups[g];
+    const int rc = plan_run(G.ctx, "qs_run_groups", G.ipos, G.config, G.nitem, 0, bin_of_slice, nslice, nbins, G.det,
+                            nullptr, nullptr, G.int_intensity, plans[g]);
+    if (rc != QS_OK) {
+      if (G.ctx != c0) c0->err = G.ctx->err;
+      return rc;
+    }
+    HIPCHK(c0, hipStreamSynchronize(G.ctx->stream));  // uploads done before the shared stream reads them
+  }
+  // round j advances probe group j of every context through all slices; within a
+  // slice the contexts follow each other, so slice s of the shared transmission
+  // function is read by every window size while it is cache-resident
+  int rounds = 0;
+  for (auto& r : plans) rounds = std::max(rounds, (r.nprobe + r.B - 1) / r.B);
+  HIPCHK(c0, hipEventRecord(c0->ev0, c0->stream));
+  for (int j = 0; j < rounds; j++)
+    for (int s = 0; s < nslice; s++)
+      for (auto& r : plans)
+        if ((long)j * r.B < r.nprobe) HIPCHK(c0, enqueue_slice(r, j * r.B, s, c0->stream, nullptr));
+  HIPCHK(c0, hipGetLastError());
+  HIPCHK(c0, hipEventRecord(c0->ev1, c0->stream));
+  HIPCHK(c0, hipStreamSynchronize(c0->stream));
+  for (auto& r : plans) {
+    const int rc = finish_run(r, r.c->stream);
+    if (rc != QS_OK) return rc;
+  }
+  HIPCHK(c0, hipEventElapsedTime(&c0->last_ms, c0->ev0, c0->ev1));
   return QS_OK;
 }
 
@@ -941,8 +1126,10 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
                float* grad_V, float* grad_probe) {
   if (!c || !ipos || nprobe < 1 || !target) return fail(c, QS_ERR_ARG, "qs_adjoint: bad arguments");
   const qs_params& p = c->p;
-  const int S = c->trans_slices;
+  const int S = trans_slices_of(c);
   if (S < 1) return fail(c, QS_ERR_STATE, "qs_adjoint: no transmission functions");
+  if (nconfig_of(c) > 1 || c->trans_src)
+    return fail(c, QS_ERR_UNSUPPORTED, "qs_adjoint: needs one configuration of the context's own transmission function");
   if (!c->have_probe) return fail(c, QS_ERR_STATE, "qs_adjoint: qs_make_probe not called");
   if (grad_V && !c->v_valid) return fail(c, QS_ERR_STATE, "qs_adjoint: grad_V needs qs_set_potential/qs_build_potential");
   if (grad_probe && (p.gaussFlag || p.aAIS > 0))
@@ -982,7 +1169,7 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
   // forward pass, keeping X_{jK-1} for every segment j >= 1
   qs::launch_fill_zero(a.state, st_elems, c->stream);
   for (int s = 0; s < S; s++) {
-    a.trans = c->trans.as<float2>() + (size_t)s * slice_elems;
+    a.trans = trans_base(c) + (size_t)s * slice_elems;
     qs::launch_sweep_rows(a, s == 0 ? 0 : 1, c->stream);
     qs::launch_sweep_cols(a, false, false, c->stream);
     if ((s + 1) % K == 0 && s + 1 < S)
@@ -1004,13 +1191,13 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
     if (s0 > 0) HIPCHK(c, hipMemcpyAsync(w.state, c->ckpt[j].p, st_bytes, hipMemcpyDeviceToDevice, c->stream));
     else qs::launch_fill_zero(w.state, st_elems, c->stream);
     for (int s = s0; s < s1; s++) {
-      w.trans = c->trans.as<float2>() + (size_t)s * slice_elems;
+      w.trans = trans_base(c) + (size_t)s * slice_elems;
       qs::launch_sweep_rows(w, s == 0 ? 0 : 1, c->stream);
       qs::launch_sweep_cols(w, false, false, c->stream);
       HIPCHK(c, hipMemcpyAsync(c->ckpt[nseg + (s - s0)].p, w.state, st_bytes, hipMemcpyDeviceToDevice, c->stream));
     }
     for (int s = s1; s >= s0; s--) {
-      A.f.trans = c->trans.as<float2>() + (size_t)s * slice_elems;
+      A.f.trans = trans_base(c) + (size_t)s * slice_elems;
       A.gT = c->gT.as<float>() + 2 * (size_t)s * slice_elems;
       const bool last = s == S - 1;
       A.xprev = last ? c->state.as<float2>() : nullptr;
@@ -1105,9 +1292,10 @@ int qs_atom_gradients(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_p
 // ---------------------------------------------------------------------------------
 // Autotuner: picks the probe group size (batch) and the launch shape of the two
 // sweeps by timing the slice loop on the context's own transmission function and
-// probe, over the first few slices and up to 512 of the caller's window offsets.
+// probe, over the first 8 slices and up to 512 of the caller's window offsets.
 // Coordinate descent: group size first (library launch defaults), then row-sweep
-// steps per wavefront, then column tiles per block.  Results are cached per
+// steps per wavefront, then column tiles per block; a launch shape replaces the
+// current best only when 2% faster (timing noise).  Results are cached per
 // device and wave geometry for the life of the process.
 namespace {
 struct TuneKey {
@@ -1127,7 +1315,7 @@ std::map<TuneKey, TuneVal> g_tune_cache;
 int qs_autotune(qs_ctx* c, const int* ipos, int nprobe, int* batch, int* rows_per_wave, int* tiles_per_block,
                 float* rate) {
   if (!c || !ipos || nprobe < 1) return fail(c, QS_ERR_ARG, "qs_autotune: bad arguments");
-  if (c->trans_slices < 1 || !c->have_probe)
+  if (trans_slices_of(c) < 1 || !c->have_probe)
     return fail(c, QS_ERR_STATE, "qs_autotune: set the potential and the probe first");
   int dev = 0;
   HIPCHK(c, hipGetDevice(&dev));
@@ -1144,7 +1332,7 @@ int qs_autotune(qs_ctx* c, const int* ipos, int nprobe, int* batch, int* rows_pe
   }
   if (!cached) {
     const int n = std::min(nprobe, 512);
-    const int S = std::min(4, c->trans_slices);
+    const int S = std::min(8, trans_slices_of(c));
     const bool graph = c->use_graph, prof = c->profiling;
     c->use_graph = false;
     c->profiling = false;
@@ -1165,7 +1353,7 @@ int qs_autotune(qs_ctx* c, const int* ipos, int nprobe, int* batch, int* rows_pe
       float r = 0.f;
       const int rc = measure(B, rw, tpb, &r);
       if (rc != QS_OK) return rc;
-      if (r > best.rate) best = TuneVal{B, rw, tpb, r};
+      if (r > best.rate * (best.batch == B ? 1.02f : 1.0f)) best = TuneVal{B, rw, tpb, r};  // launch shapes: 2% hysteresis
       return QS_OK;
     };
     int rc = QS_OK;

@@ -21,6 +21,8 @@ struct SweepArgs {
   const float2* probe;     // shared real-space probe [nx][ny] (sweep A mode 0)
   const float2* trans;     // transmission slice [potNx][potNy]
   const int2* ipos;        // window offset per global probe
+  const int* cfg;          // nullable: frozen-phonon configuration per global probe
+  long long cfg_stride;    // elements between consecutive configurations' transmission functions
   float2* state;           // [nprobe_total][nx][kcp]
   float2* out_wave;        // exit waves [nprobe_total][nx][ny] (sweep A mode 2)
   const float2* twx;       // forward twiddles W_nx^m

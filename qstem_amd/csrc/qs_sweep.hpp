@@ -101,8 +101,12 @@ __global__ __launch_bounds__(kBlock, Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 
     for (int m = 0; m < E; m++) boff[m] = bo(m);
   }
   int2 ip = make_int2(0, 0);
-  if constexpr (MODE != 2) ip = a.ipos[pg];
-  const float2* tr0 = a.trans + (size_t)(x0 + ip.x) * a.potNy + ip.y;
+  const float2* trc = a.trans;  // this probe's configuration, slice s
+  if constexpr (MODE != 2) {
+    ip = a.ipos[pg];
+    if (a.cfg) trc += (size_t)a.cfg[pg] * (size_t)a.cfg_stride;
+  }
+  const float2* tr0 = trc + (size_t)(x0 + ip.x) * a.potNy + ip.y;
 
   auto load = [&](int i, float2* dv, float2* dt) {
     const int xr = i * TPW + sub;  // row within the chunk

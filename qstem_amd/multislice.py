@@ -300,14 +300,33 @@ class Multislice:
     def last_run_ms(self) -> float:
         return float(self._lib.qs_last_run_ms(self._h))
 
+    def set_configs(self, nconfig: int):
+        """Hold nconfig frozen-phonon transmission functions (reload all, configuration 0 first)."""
+        self._check(self._lib.qs_set_configs(self._h, int(nconfig)), "qs_set_configs")
+
+    def select_config(self, config: int):
+        """Configuration that set_potential / set_trans / build_potential / get_trans address."""
+        self._check(self._lib.qs_select_config(self._h, int(config)), "qs_select_config")
+
+    def share_trans(self, src: Optional["Multislice"]):
+        """Run on src's transmission functions (same potential grid, any window size)."""
+        self._check(self._lib.qs_share_trans(self._h, src._h if src is not None else None), "qs_share_trans")
+        self._trans_src = src  # keeps src alive while this context uses its buffers
+        if src is not None:
+            self._nslice = src._nslice
+
     def run_stem(self, ipos: np.ndarray, nslice: Optional[int] = None, start: int = 0,
                  bins: Optional[Sequence[int]] = None, nbins: int = 0, det: Optional[np.ndarray] = None,
                  want_det: bool = True, want_diff: bool = False, want_exit: bool = False,
-                 want_intensity: bool = False) -> dict:
-        """Advance len(ipos) probes through nslice slices.  Returns a dict with
+                 want_intensity: bool = False, config: Optional[np.ndarray] = None) -> dict:
+        """Advance len(ipos) probes through nslice slices (item i on phonon
+        configuration config[i] when given).  Returns a dict with
         'det' [nbins][ndet][nprobe], 'diff', 'exit', 'intensity' as requested."""
         ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
         n = ipos.shape[0]
+        if config is not None:
+            config = np.ascontiguousarray(config, dtype=np.int32).reshape(-1)
+            assert config.shape[0] == n
         nslice = self._nslice if nslice is None else nslice
         if bins is None:
             b, nb = record_bins(nslice)
@@ -325,9 +344,9 @@ class Multislice:
         diff = np.empty((n, self.cfg.nx, self.cfg.ny), dtype=np.float32) if want_diff else None
         exitw = np.empty((n, self.cfg.nx, self.cfg.ny), dtype=np.complex64) if want_exit else None
         inten = np.empty(n, dtype=np.float32) if want_intensity else None
-        rc = self._lib.qs_run_stem(self._h, _iptr(ipos), n, int(start), _iptr(bins), int(nslice), int(nbins),
-                                   _fptr(det), _fptr(diff), _fptr(exitw.view(np.float32) if exitw is not None else None),
-                                   _fptr(inten))
+        rc = self._lib.qs_run_stem_cfg(self._h, _iptr(ipos), _iptr(config), n, int(start), _iptr(bins), int(nslice),
+                                       int(nbins), _fptr(det), _fptr(diff),
+                                       _fptr(exitw.view(np.float32) if exitw is not None else None), _fptr(inten))
         self._check(rc, "qs_run_stem")
         if diff is not None:
             out["diff"] = diff
@@ -336,3 +355,41 @@ class Multislice:
         if inten is not None:
             out["intensity"] = inten
         return out
+
+
+def run_groups(groups, bins=None, nbins: int = 0, nslice: Optional[int] = None, want_intensity: bool = False):
+    """Uneven batch (qs_run_groups): groups = [(Multislice, ipos, config or None), ...]
+    whose contexts share one set of transmission functions (share_trans).  Every
+    group starts from its probe; slices are interleaved across groups.  Returns a
+    list of dicts with 'det' [nbins][ndet][nitem] (and 'intensity')."""
+    from ._lib import QsGroup
+    ms0 = groups[0][0]
+    nslice = ms0._nslice if nslice is None else nslice
+    if bins is None:
+        bins, nbins = record_bins(nslice)
+    bins = np.ascontiguousarray(bins, dtype=np.int32)
+    arr = (QsGroup * len(groups))()
+    keep, outs = [], []
+    for g, (ms, ipos, config) in enumerate(groups):
+        ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
+        n = ipos.shape[0]
+        cfg = None if config is None else np.ascontiguousarray(config, dtype=np.int32).reshape(-1)
+        ndet = len(ms.cfg.detectors)
+        det = np.zeros((nbins, ndet, n), dtype=np.float32) if ndet > 0 else None
+        inten = np.empty(n, dtype=np.float32) if want_intensity else None
+        keep += [ipos, cfg, det, inten]
+        arr[g].ctx = ms._h
+        arr[g].ipos = _iptr(ipos)
+        arr[g].config = _iptr(cfg)
+        arr[g].nitem = n
+        arr[g].det = _fptr(det)
+        arr[g].int_intensity = _fptr(inten)
+        out = {}
+        if det is not None:
+            out["det"] = det
+        if inten is not None:
+            out["intensity"] = inten
+        outs.append(out)
+    rc = ms0._lib.qs_run_groups(arr, len(groups), _iptr(bins), int(nslice), int(nbins))
+    ms0._check(rc, "qs_run_groups")
+    return outs

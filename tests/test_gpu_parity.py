@@ -170,7 +170,8 @@ def test_autotune_keeps_results():
         ref = ms.run_stem(ipos, want_exit=True)
         best = ms.autotune(ipos)
         assert best["batch"] in (16, 32, 40) and best["rate"] > 0
-        assert best["rows_per_wave"] in (1, 2, 4, 8) and best["tiles_per_block"] in (1, 2, 4)
+        # 0 = library default (kept unless another shape is measurably faster)
+        assert best["rows_per_wave"] in (0, 1, 2, 4, 8) and best["tiles_per_block"] in (0, 1, 2, 4)
         assert ms.get_tuning() == (best["rows_per_wave"], best["tiles_per_block"], best["batch"])
         out = ms.run_stem(ipos, want_exit=True)
     assert np.array_equal(out["exit"], ref["exit"])
