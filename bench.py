@@ -72,6 +72,74 @@ def synthetic_potential(cfg, potoff):
     return np.concatenate([V] * (cfg.slices // 2), axis=0)
 
 
+def uneven_workload(nconfig=16, steps=3):
+    """Uneven batch with frozen phonons: seeded amorphous carbon slab 80 x 80 x 100 A
+    at 2.0 g/cm^3 (1.4 A minimum separation) with a 2 nm gold FCC sphere (a = 4.08 A)
+    at its centre; nconfig Gaussian phonon configurations (sigma = 0.08 A); 100 slices
+    of 1 A; 200 kV, 20 mrad; dx = 0.08 A on a 1088^2 potential grid; windows 256^2,
+    512^2 and 1024^2 in the ratio 1/2 : 1/3 : 1/6 (6, 4 and 2 positions per
+    configuration), all advanced by one qs_run_groups call.  Returns the record."""
+    import torch
+    from qstem_amd import Detector, Multislice, StemConfig, record_bins, run_groups
+    from qstem_amd.specimen import carbon_gold_specimen, gaussian_phonons
+    t0 = time.perf_counter()
+    atoms = carbon_gold_specimen(seed=1)
+    t_spec = time.perf_counter() - t0
+    dx, potN, nslice = 0.08, 1088, 100
+    dets = [Detector(0.0, 10.0, "BF"), Detector(40.0, 100.0, "ADF")]
+
+    def make(n):
+        return Multislice(StemConfig(nx=n, ny=n, resolutionX=dx, resolutionY=dx, v0=200.0, slices=nslice,
+                                     sliceThickness=1.0, potNx=potN, potNy=potN, bandlimittrans=1, Cs=0.0, df0=0.0,
+                                     alpha=20.0, detectors=dets))
+    big = make(1024)
+    big.set_configs(nconfig)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for k in range(nconfig):
+        big.select_config(k)
+        big.build_potential(gaussian_phonons(atoms, 0.08, seed=k + 1), tds=1)
+    torch.cuda.synchronize()
+    t_pot = time.perf_counter() - t0
+    ctxs = {1024: big, 512: make(512), 256: make(256)}
+    for n in (512, 256):
+        ctxs[n].share_trans(big)
+    per_cfg = {256: 6, 512: 4, 1024: 2}
+    groups = []
+    for n, ms in ctxs.items():
+        ms.make_probe()
+        span = potN - n
+        g = np.linspace(0, span, per_cfg[n]).astype(np.int32)
+        pos = np.stack([g, g[::-1]], 1)
+        ipos = np.tile(pos, (nconfig, 1))
+        config = np.repeat(np.arange(nconfig, dtype=np.int32), per_cfg[n])
+        groups.append((ms, ipos, config))
+    bins, nbins = record_bins(nslice)
+    run_groups(groups, bins=bins, nbins=nbins)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        outs = run_groups(groups, bins=bins, nbins=nbins)
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    items = sum(len(g[1]) for g in groups)
+    # the same items one window size at a time (each context's own qs_run_stem)
+    t0 = time.perf_counter()
+    for ms, ipos, config in groups:
+        ms.run_stem(ipos, bins=bins, nbins=nbins, config=config)
+    torch.cuda.synchronize()
+    el_seq = time.perf_counter() - t0
+    trans_bytes = nconfig * nslice * potN * potN * 8
+    return {"metric": "item-slices/s, uneven batch (256^2/512^2/1024^2) x %d phonon configurations" % nconfig,
+            "value": round(items * nslice / el, 1), "unit": "item-slices/s", "items": items,
+            "items_per_size": {str(n): len(g[1]) for n, g in zip(ctxs, groups)}, "slices": nslice,
+            "ms_per_batch": round(el * 1e3, 2), "sequential_sizes_ms": round(el_seq * 1e3, 2),
+            "atoms": int(len(atoms)), "specimen_s": round(t_spec, 2),
+            "potential_build_s_per_config": round(t_pot / nconfig, 3),
+            "transmission_GB": round(trans_bytes / 1e9, 2),
+            "adf_mean_per_size": {str(n): float(np.mean(o["det"][-1, 1])) for n, o in zip(ctxs, outs)}}
+
+
 def adjoint_workload(steps=3):
     """4D-STEM adjoint workload: 32x32 scan, 256x256 wave (dx = 0.05 A), 200 kV, 20 slices of
     seeded Gaussian-filtered random potentials (sigma 0.5 A, max phase 0.3 rad); loss = L2
@@ -168,6 +236,8 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=8, help="probes for the oracle CPU baseline (0 = skip)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches only")
+    ap.add_argument("--workload", default="batched", choices=["batched", "uneven"],
+                    help="batched: the headline line; uneven: frozen-phonon uneven batch (own JSON line)")
     ap.add_argument("--rows-per-wave", type=int, default=0, help="with --group: row-sweep steps per wavefront")
     ap.add_argument("--tiles-per-block", type=int, default=0, help="with --group: column tiles per block")
     a = ap.parse_args()
@@ -175,6 +245,10 @@ def main():
         print("bench.py: this build runs on one GPU only (no multi-GPU path); got --gpus %d" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
+
+    if a.workload == "uneven":
+        print(json.dumps(uneven_workload()), flush=True)
+        return
 
     from qstem_amd import Multislice, record_bins
 

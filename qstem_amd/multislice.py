@@ -110,6 +110,21 @@ def _iptr(a: Optional[np.ndarray]):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
+def _atom_array(atoms):
+    """(n, 7) x, y, z, dw, occ, q, Z -> ctypes array of struct qs_atom (filled through numpy)."""
+    atoms = np.asarray(atoms, dtype=np.float64).reshape(-1, 7)
+    arr = (_lib.QsAtom * max(len(atoms), 1))()
+    dt = np.dtype({"names": ["x", "y", "z", "dw", "occ", "q", "Znum"],
+                   "formats": ["<f4"] * 6 + ["<i4"],
+                   "offsets": [getattr(_lib.QsAtom, f).offset for f in ("x", "y", "z", "dw", "occ", "q", "Znum")],
+                   "itemsize": ctypes.sizeof(_lib.QsAtom)})
+    view = np.frombuffer(arr, dtype=dt, count=len(atoms))
+    for j, f in enumerate(("x", "y", "z", "dw", "occ", "q")):
+        view[f] = atoms[:, j]
+    view["Znum"] = atoms[:, 6].astype(np.int32)
+    return arr
+
+
 def record_bins(slices: int, cellDiv: int = 1, outputInterval: int = 0, totalSliceCount: int = 0):
     """Bin written by each slice as its final value (collectIntensity, stemlib.cpp:2588-2607)."""
     lib = _lib.load()
@@ -203,11 +218,7 @@ class Multislice:
         x, y, z, dw, occ, q, Z (Cartesian Angstrom)."""
         if not getattr(self, "_have_table", False):
             self.set_scattering_table()
-        atoms = np.asarray(atoms, dtype=np.float64).reshape(-1, 7)
-        arr = (_lib.QsAtom * max(len(atoms), 1))()
-        for i, a in enumerate(atoms):
-            arr[i] = _lib.QsAtom(float(a[0]), float(a[1]), float(a[2]), float(a[3]), float(a[4]), float(a[5]),
-                                 int(a[6]))
+        arr = _atom_array(atoms)
         pp = _lib.QsPotParams(atomRadius, potOffsetX, potOffsetY, czOffset, int(centerSlices), int(periodicXY),
                               int(tds), int(compat_q6), int(lut_dr_fix))
         V = np.empty((self.cfg.slices, self.cfg.potNx, self.cfg.potNy), np.float32) if return_V else None
@@ -246,9 +257,7 @@ class Multislice:
                        periodicXY=0, tds=0, compat_q6=0, lut_dr_fix=0) -> np.ndarray:
         """dL/d(x, y, z) per atom from the last adjoint (same atoms as build_potential)."""
         atoms = np.asarray(atoms, dtype=np.float64).reshape(-1, 7)
-        arr = (_lib.QsAtom * max(len(atoms), 1))()
-        for i, a in enumerate(atoms):
-            arr[i] = _lib.QsAtom(*[float(v) for v in a[:6]], int(a[6]))
+        arr = _atom_array(atoms)
         pp = _lib.QsPotParams(atomRadius, potOffsetX, potOffsetY, czOffset, int(centerSlices), int(periodicXY),
                               int(tds), int(compat_q6), int(lut_dr_fix))
         g = np.zeros((len(atoms), 3), np.float32)

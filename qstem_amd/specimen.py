@@ -128,3 +128,103 @@ def perovskite_cell(a=3.905, dw=(0.6214, 0.4390, 0.7323), Zs=(38, 22, 8), charge
     return UnitCell(np.eye(3) * a, frac, np.array([dw[0], dw[1], dw[2], dw[2], dw[2]]), np.ones(5),
                     np.array([charges[0], charges[1], charges[2], charges[2], charges[2]]),
                     np.array([Zs[0], Zs[1], Zs[2], Zs[2], Zs[2]]))
+
+
+# ---------------------------------------------------------------------------------
+# Amorphous-carbon slab with an embedded gold nanoparticle, and Gaussian frozen
+# phonons.  The reference builds amorphous models with its separate gbmaker tool
+# (random insertion with a minimum distance, gbmaker.cpp:1090-1134); this is an
+# independent seeded generator of the same kind of specimen.
+
+CARBON_MASS = 12.011        # g/mol
+AVOGADRO = 6.02214076e23
+
+
+def amorphous_positions(box, density_gcm3=2.0, rmin=1.4, mass=CARBON_MASS, seed=1, max_tries=200):
+    """Random sequential insertion of carbon atoms into an orthogonal box
+    (Angstrom) at the given mass density, no two closer than rmin (periodic in x
+    and y, open in z).  A cell list of side >= rmin keeps each test local.
+    Returns an (n, 3) array; n = round(density * volume / mass)."""
+    box = np.asarray(box, dtype=np.float64)
+    n = int(round(density_gcm3 / mass * AVOGADRO * 1e-24 * float(np.prod(box))))
+    rng = np.random.default_rng(seed)
+    ncell = np.maximum((box // rmin).astype(int), 1)
+    side = box / ncell
+    cells = {}
+    pos = np.empty((n, 3))
+    r2 = rmin * rmin
+    k = 0
+    while k < n:
+        for _ in range(max_tries):
+            p = rng.random(3) * box
+            c = (p // side).astype(int)
+            ok = True
+            for dx in (-1, 0, 1):
+                for dy in (-1, 0, 1):
+                    for dz in (-1, 0, 1):
+                        cz = c[2] + dz
+                        if cz < 0 or cz >= ncell[2]:
+                            continue
+                        key = ((c[0] + dx) % ncell[0], (c[1] + dy) % ncell[1], cz)
+                        for j in cells.get(key, ()):
+                            d = pos[j] - p
+                            d[:2] -= box[:2] * np.round(d[:2] / box[:2])
+                            if d @ d < r2:
+                                ok = False
+                                break
+                        if not ok:
+                            break
+                    if not ok:
+                        break
+                if not ok:
+                    break
+            if ok:
+                pos[k] = p
+                cells.setdefault(tuple(c), []).append(k)
+                k += 1
+                break
+        else:
+            raise RuntimeError("amorphous_positions: no free site after %d tries (density too high)" % max_tries)
+    return pos
+
+
+def fcc_sphere(center, radius, a=4.08):
+    """Sites of an FCC lattice (cube edge a, one site at the centre) within radius."""
+    m = int(np.ceil(radius / a)) + 1
+    g = np.arange(-m, m + 1)
+    corners = np.stack(np.meshgrid(g, g, g, indexing="ij"), -1).reshape(-1, 3).astype(np.float64)
+    basis = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    sites = (corners[:, None, :] + basis[None, :, :]).reshape(-1, 3) * a
+    keep = np.einsum("ij,ij->i", sites, sites) <= radius * radius
+    return sites[keep] + np.asarray(center, dtype=np.float64)
+
+
+def carbon_gold_specimen(box=(80.0, 80.0, 100.0), density_gcm3=2.0, rmin=1.4, sphere_radius=20.0, a_au=4.08,
+                         seed=1, dw_c=0.5, dw_au=0.6):
+    """Amorphous carbon slab with a gold FCC sphere at its centre (carbon removed
+    inside the sphere).  Returns (n, 7) atoms x, y, z, dw, occ, q, Z for
+    qs_build_potential, sorted by z."""
+    box = np.asarray(box, dtype=np.float64)
+    c = amorphous_positions(box, density_gcm3, rmin, seed=seed)
+    centre = box / 2
+    d = c - centre
+    c = c[np.einsum("ij,ij->i", d, d) > sphere_radius * sphere_radius]
+    au = fcc_sphere(centre, sphere_radius, a_au)
+    atoms = np.zeros((len(c) + len(au), 7))
+    atoms[:len(c), :3] = c
+    atoms[:len(c), 3] = dw_c
+    atoms[:len(c), 6] = 6
+    atoms[len(c):, :3] = au
+    atoms[len(c):, 3] = dw_au
+    atoms[len(c):, 6] = 79
+    atoms[:, 4] = 1.0
+    return atoms[np.argsort(atoms[:, 2], kind="stable")]
+
+
+def gaussian_phonons(atoms, sigma=0.08, seed=1, sort=True):
+    """One frozen-phonon configuration: every atom displaced by N(0, sigma^2) per
+    axis (Angstrom), from a seeded generator; rows re-sorted by z unless sort is False."""
+    rng = np.random.default_rng(seed)
+    out = np.array(atoms, dtype=np.float64, copy=True)
+    out[:, :3] += rng.normal(0.0, sigma, size=(len(out), 3))
+    return out[np.argsort(out[:, 2], kind="stable")] if sort else out
