@@ -140,6 +140,66 @@ def uneven_workload(nconfig=16, steps=3):
             "adf_mean_per_size": {str(n): float(np.mean(o["det"][-1, 1])) for n, o in zip(ctxs, outs)}}
 
 
+def tem_workload(nslice=200, ckpt=14):
+    """Large TEM with the checkpointed adjoint: plane wave on a 4096^2 grid at
+    dx = 390.5/4096 A (100 x 100 perovskite cells), 300 kV, 200 slices of a/2
+    (synthetic projected potential of the two perovskite slice types, as in the
+    batched workload); forward FP32 with the exit wave, then the adjoint of
+    L = sum (I - target)^2 with checkpoints every 14 slices.  Reports device times,
+    their ratio and the library's peak device memory per phase."""
+    import torch
+    from qstem_amd import Multislice, StemConfig, device_memory
+    n, a = 4096, 3.905
+    dx = 100 * a / n
+    cfg = StemConfig(nx=n, ny=n, resolutionX=dx, resolutionY=dx, v0=300.0, slices=nslice, sliceThickness=a / 2,
+                     potNx=n, potNy=n, bandlimittrans=1, Cs=0.0, df0=0.0, alpha=20.0, electronScale=1.0,
+                     detectors=[])
+    x = np.arange(n) * dx
+    planes = {0: [(0.0, 0.0, 38), (0.5, 0.5, 8)], 1: [(0.5, 0.5, 22), (0.5, 0.0, 8), (0.0, 0.5, 8)]}
+    V = np.empty((nslice, n, n), np.float32)
+    for sl, atoms in planes.items():
+        P = np.zeros((n, n), np.float32)
+        for fx, fy, Z in atoms:
+            ddx = (((x - fx * a + a / 2) % a) - a / 2).astype(np.float32)
+            ddy = (((x - fy * a + a / 2) % a) - a / 2).astype(np.float32)
+            P += np.float32(3.0 * Z ** 0.7) * np.exp(-(ddx[:, None] ** 2 + ddy[None, :] ** 2) / np.float32(2 * 0.25 ** 2))
+        V[sl::2] = P
+    ipos = np.zeros((1, 2), np.int32)
+    with Multislice(cfg) as ms:
+        device_memory(reset_peak=True)
+        t0 = time.perf_counter()
+        ms.set_potential(V)
+        t_pot = time.perf_counter() - t0
+        del V
+        ms.set_incident_wave(np.ones((n, n), np.complex64))
+        _, peak_setup = device_memory(reset_peak=True)
+        ms.run_stem(ipos, want_det=False, want_exit=True)  # warm-up
+        fw = []
+        for _ in range(2):
+            ex = ms.run_stem(ipos, want_det=False, want_exit=True)["exit"]
+            fw.append(ms.last_run_ms)
+        _, peak_fwd = device_memory(reset_peak=True)
+        I = np.abs(np.fft.fft2(ex[0].astype(np.complex128)) / (n * n)) ** 2
+        target = (0.9 * I).astype(np.float32)[None]
+        torch.cuda.synchronize()
+        ad = []
+        for _ in range(2):
+            t0 = time.perf_counter()
+            out = ms.adjoint(ipos, target, ckpt_every=ckpt, want_grad_V=True, want_grad_probe=False)
+            torch.cuda.synchronize()
+            ad.append(time.perf_counter() - t0)
+            adm = ms.last_run_ms
+        _, peak_adj = device_memory(reset_peak=True)
+    f_ms = min(fw)
+    return {"workload": "TEM plane wave 4096^2, perovskite 100x100 cells, %d slices, 300 kV" % nslice,
+            "forward_ms": round(f_ms, 2), "adjoint_ms": round(adm, 2), "adjoint_over_forward": round(adm / f_ms, 3),
+            "adjoint_wall_s": round(min(ad), 3), "checkpoint_every": ckpt, "loss": out["loss"],
+            "exit_norm": float(np.sum(np.abs(ex[0]) ** 2) / (n * n)),
+            "peak_device_GB": {"potential_setup": round(peak_setup / 1e9, 2), "forward": round(peak_fwd / 1e9, 2),
+                               "adjoint": round(peak_adj / 1e9, 2)},
+            "potential_set_s": round(t_pot, 2)}
+
+
 def adjoint_workload(steps=3):
     """4D-STEM adjoint workload: 32x32 scan, 256x256 wave (dx = 0.05 A), 200 kV, 20 slices of
     seeded Gaussian-filtered random potentials (sigma 0.5 A, max phase 0.3 rad); loss = L2
@@ -236,8 +296,10 @@ def main():
     ap.add_argument("--cpu-sample", type=int, default=8, help="probes for the oracle CPU baseline (0 = skip)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches only")
-    ap.add_argument("--workload", default="batched", choices=["batched", "uneven"],
-                    help="batched: the headline line; uneven: frozen-phonon uneven batch (own JSON line)")
+    ap.add_argument("--tem-slices", type=int, default=200, help="slices of the tem workload")
+    ap.add_argument("--workload", default="batched", choices=["batched", "uneven", "tem"],
+                    help="batched: the headline line; uneven: frozen-phonon uneven batch; tem: 4096^2 plane wave "
+                         "with the checkpointed adjoint (own JSON lines)")
     ap.add_argument("--rows-per-wave", type=int, default=0, help="with --group: row-sweep steps per wavefront")
     ap.add_argument("--tiles-per-block", type=int, default=0, help="with --group: column tiles per block")
     a = ap.parse_args()
@@ -248,6 +310,9 @@ def main():
 
     if a.workload == "uneven":
         print(json.dumps(uneven_workload()), flush=True)
+        return
+    if a.workload == "tem":
+        print(json.dumps(tem_workload(nslice=a.tem_slices)), flush=True)
         return
 
     from qstem_amd import Multislice, record_bins

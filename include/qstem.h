@@ -133,6 +133,17 @@ int qs_make_probe(qs_ctx *ctx, float *out);
  * (tCount+1), or -1 on bad arguments. */
 int qs_record_bins(int slices, int cellDiv, int outputInterval, int totalSliceCount, int *bin_of_slice);
 
+/* Incident wave [nx][ny] complex64 (host) in place of the STEM probe, e.g. the
+ * plane wave psi = 1 of TEM/CBED-style runs (the reference's doTEM initialises
+ * the wave to 1 before its slice loop, stem3/stem3.cpp:2545-2672).  qs_run_stem
+ * then starts every item from this wave; qs_adjoint gives no probe gradients. */
+int qs_set_incident_wave(qs_ctx *ctx, const float *psi);
+
+/* Device memory held by the library over all contexts, current and peak since
+ * the last reset (reset_peak != 0 resets it to the current value).  No reference
+ * counterpart. */
+int qs_device_memory(long long *current_bytes, long long *peak_bytes, int reset_peak);
+
 /* Runs nslice slices for nprobe scan positions.
  *   ipos          [nprobe][2] integer window offsets into the transmission array
  *   start         0: start from the probe; 1: continue the waves left by the previous call

@@ -8,6 +8,7 @@ from .multislice import (  # noqa: F401
     Multislice,
     QstemError,
     StemConfig,
+    device_memory,
     MILLISEC_PICOAMP,
     record_bins,
     run_groups,

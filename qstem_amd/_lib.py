@@ -84,6 +84,8 @@ SIGNATURES = {
     "qs_set_scattering_table": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "qs_build_potential": (_I, [_P, ctypes.POINTER(QsAtom), _I, ctypes.POINTER(QsPotParams), _FP]),
     "qs_record_bins": (_I, [_I, _I, _I, _I, _IP]),
+    "qs_set_incident_wave": (_I, [_P, _FP]),
+    "qs_device_memory": (_I, [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), _I]),
     "qs_run_stem": (_I, [_P, _IP, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
     "qs_run_stem_cfg": (_I, [_P, _IP, _IP, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
     "qs_set_configs": (_I, [_P, _I]),

@@ -7,6 +7,7 @@
 // per-pixel work runs in the kernels of qs_kernels.hip.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cmath>
 #include <algorithm>
 #include <complex>
@@ -29,20 +30,35 @@ namespace {
 
 thread_local std::string g_create_error;
 
+// Device memory held by the library (all contexts): current and peak bytes.
+static std::atomic<long long> g_dev_bytes{0}, g_dev_peak{0};
+static void dev_account(long long delta) {
+  const long long now = g_dev_bytes.fetch_add(delta) + delta;
+  long long peak = g_dev_peak.load();
+  while (now > peak && !g_dev_peak.compare_exchange_weak(peak, now)) {
+  }
+}
+
 struct DevBuf {
   void* p = nullptr;
   size_t bytes = 0;
   hipError_t ensure(size_t n) {
     if (n <= bytes && p) return hipSuccess;
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    bytes = 0;
+    release();
     hipError_t e = hipMalloc(&p, n > 0 ? n : 16);
-    if (e == hipSuccess) bytes = n;
+    if (e == hipSuccess) {
+      bytes = n;
+      dev_account((long long)n);
+    } else {
+      p = nullptr;
+    }
     return e;
   }
   void release() {
-    if (p) (void)hipFree(p);
+    if (p) {
+      (void)hipFree(p);
+      dev_account(-(long long)bytes);
+    }
     p = nullptr;
     bytes = 0;
   }
@@ -139,6 +155,7 @@ struct qs_ctx {
   int trans_slices = 0;
   bool v_valid = false;     // c->V holds the potential of the current transmission slices
   double probe_norm = 1.0;  // c of psi0 = c * IFFT(Psi0)
+  bool custom_wave = false;  // incident wave set by qs_set_incident_wave
   ProbeArgs probe_args{};
   bool have_probe = false;
   // adjoint
@@ -680,6 +697,7 @@ int qs_make_probe(qs_ctx* c, float* out) {
   c->probe_norm = sqrt((double)n / tot);
   c->probe_args = a;
   c->have_probe = true;
+  c->custom_wave = false;
   return QS_OK;
 }
 
@@ -696,6 +714,25 @@ int qs_set_optics(qs_ctx* c, const qs_params* p) {
   q.ismoth = p->ismoth; q.gaussFlag = p->gaussFlag; q.gaussScale = p->gaussScale;
   q.compat = (q.compat & ~QS_COMPAT_Q2) | (p->compat & QS_COMPAT_Q2);
   c->have_probe = false;
+  return QS_OK;
+}
+
+int qs_set_incident_wave(qs_ctx* c, const float* psi) {
+  if (!c || !psi) return fail(c, QS_ERR_ARG, "qs_set_incident_wave: bad arguments");
+  drop_graph(c);
+  const size_t n = (size_t)c->p.nx * c->p.ny;
+  HIPCHK(c, c->probe.ensure(sizeof(float2) * n));
+  HIPCHK(c, hipMemcpyAsync(c->probe.p, psi, sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->have_probe = true;
+  c->custom_wave = true;  // no aberration function behind it: no probe gradients
+  return QS_OK;
+}
+
+int qs_device_memory(long long* current_bytes, long long* peak_bytes, int reset_peak) {
+  if (current_bytes) *current_bytes = g_dev_bytes.load();
+  if (peak_bytes) *peak_bytes = g_dev_peak.load();
+  if (reset_peak) g_dev_peak.store(g_dev_bytes.load());
   return QS_OK;
 }
 
@@ -1132,6 +1169,8 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
     return fail(c, QS_ERR_UNSUPPORTED, "qs_adjoint: needs one configuration of the context's own transmission function");
   if (!c->have_probe) return fail(c, QS_ERR_STATE, "qs_adjoint: qs_make_probe not called");
   if (grad_V && !c->v_valid) return fail(c, QS_ERR_STATE, "qs_adjoint: grad_V needs qs_set_potential/qs_build_potential");
+  if (grad_probe && c->custom_wave)
+    return fail(c, QS_ERR_UNSUPPORTED, "qs_adjoint: probe gradients need the probe of qs_make_probe");
   if (grad_probe && (p.gaussFlag || p.aAIS > 0))
     return fail(c, QS_ERR_UNSUPPORTED, "qs_adjoint: probe gradients need gaussian: no and no AIS aperture");
   for (int i = 0; i < nprobe; i++) {

@@ -233,6 +233,12 @@ class Multislice:
         self._check(self._lib.qs_make_probe(self._h, _fptr(out.view(np.float32))), "qs_make_probe")
         return out
 
+    def set_incident_wave(self, psi: np.ndarray):
+        """Incident wave [nx][ny] in place of the probe (plane wave: np.ones)."""
+        psi = np.ascontiguousarray(psi, dtype=np.complex64)
+        assert psi.shape == (self.cfg.nx, self.cfg.ny), psi.shape
+        self._check(self._lib.qs_set_incident_wave(self._h, _fptr(psi.view(np.float32))), "qs_set_incident_wave")
+
     # -- adjoint ----------------------------------------------------------------------
     def adjoint(self, ipos: np.ndarray, target: np.ndarray, ckpt_every: int = 0, want_grad_V: bool = True,
                 want_grad_probe: bool = True) -> dict:
@@ -402,3 +408,11 @@ def run_groups(groups, bins=None, nbins: int = 0, nslice: Optional[int] = None, 
     rc = ms0._lib.qs_run_groups(arr, len(groups), _iptr(bins), int(nslice), int(nbins))
     ms0._check(rc, "qs_run_groups")
     return outs
+
+
+def device_memory(reset_peak: bool = False):
+    """(current, peak) bytes of device memory held by the library (all contexts)."""
+    lib = _lib.load()
+    cur, peak = ctypes.c_longlong(), ctypes.c_longlong()
+    lib.qs_device_memory(ctypes.byref(cur), ctypes.byref(peak), 1 if reset_peak else 0)
+    return cur.value, peak.value

@@ -154,3 +154,39 @@ def test_torch_autograd_function(oracle):
             V2 = (Vt - step * Vt.grad).detach()
         loss2 = multislice_loss(V2, df.detach(), cs.detach(), ms, ipos, target)
         assert float(loss2) < float(loss)
+
+
+@pytest.mark.parametrize("nx", [1024, 2048, 4096])
+def test_adjoint_plane_wave_large(oracle, nx):
+    """Plane-wave (TEM) adjoint at the large transform sizes: loss and a directional
+    dL/dV against FP64 central differences of the oracle (2 slices, checkpoint every slice)."""
+    from qstem_amd import Multislice, StemConfig
+    print("plane-wave adjoint", nx, flush=True)
+    cfg = StemConfig(nx=nx, ny=nx, resolutionX=0.05, resolutionY=0.05, v0=300.0, slices=2, sliceThickness=1.9525,
+                     potNx=nx, potNy=nx, bandlimittrans=1, Cs=0.0, df0=0.0, alpha=20.0, electronScale=1.0)
+    # a strong phase object (up to ~2.5 rad per slice): dL/dV is Im(g conj(u)) of nearly
+    # parallel factors for a weak one, which FP32 resolves poorly at these sizes
+    V = smooth_potential(2, nx, nx, seed=61, vmax=80.0, sigma_px=4.0)
+    ipos = np.zeros((1, 2), np.int32)
+    psi = np.ones((nx, nx), np.complex64)
+    psi_d = psi.astype(np.complex128)
+    # target 0 (L = sum I^2): with a plane wave the unscattered beam holds almost all
+    # intensity, and a target close to it would leave the loss a difference of nearly
+    # equal FP32 numbers; without a target the check is well conditioned
+    target = np.zeros((1, nx, nx), np.float32)
+    with Multislice(cfg) as ms:
+        ms.set_potential(V)
+        ms.set_incident_wave(psi)
+        out = ms.adjoint(ipos, target, ckpt_every=1, want_grad_probe=False)
+
+    def L(Vx):
+        return oracle.diffraction_loss(cfg, oracle.make_trans_d(cfg, Vx), ipos, psi_d, target)[0]
+
+    Vd = V.astype(np.float64)
+    L0 = L(Vd)
+    assert abs(out["loss"] / L0 - 1) < 1e-4, (out["loss"], L0)
+    h = 1e-6
+    D = smooth_potential(2, nx, nx, seed=62, vmax=1.0, sigma_px=4.0).astype(np.float64)
+    fd = (L(Vd + h * D) - L(Vd - h * D)) / (2 * h)
+    ad = float(np.sum(out["grad_V"].astype(np.float64) * D))
+    assert abs(ad - fd) <= 1e-3 * abs(fd), (ad, fd)
