@@ -297,9 +297,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches only")
     ap.add_argument("--tem-slices", type=int, default=200, help="slices of the tem workload")
-    ap.add_argument("--workload", default="batched", choices=["batched", "uneven", "tem"],
+    ap.add_argument("--workload", default="batched", choices=["batched", "uneven", "tem", "copy"],
                     help="batched: the headline line; uneven: frozen-phonon uneven batch; tem: 4096^2 plane wave "
-                         "with the checkpointed adjoint (own JSON lines)")
+                         "with the checkpointed adjoint (own JSON lines); copy: a plain 512 MiB device copy "
+                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters)")
     ap.add_argument("--rows-per-wave", type=int, default=0, help="with --group: row-sweep steps per wavefront")
     ap.add_argument("--tiles-per-block", type=int, default=0, help="with --group: column tiles per block")
     a = ap.parse_args()
@@ -310,6 +311,15 @@ def main():
 
     if a.workload == "uneven":
         print(json.dumps(uneven_workload()), flush=True)
+        return
+    if a.workload == "copy":  # counter calibration: a plain 512 MiB device-to-device copy, 3 times
+        import torch
+        x = torch.ones(2 ** 27, device="cuda")
+        y = torch.empty_like(x)
+        for _ in range(3):
+            y.copy_(x)
+        torch.cuda.synchronize()
+        print(json.dumps({"workload": "copy", "bytes_per_copy": 2 ** 29}), flush=True)
         return
     if a.workload == "tem":
         print(json.dumps(tem_workload(nslice=a.tem_slices)), flush=True)
@@ -394,11 +404,14 @@ def main():
     # algorithmic bytes: wave in + out, the union transmission window read once per launch, detector sums
     bytes_per_probe_slice = 16.0 * N2 + 8.0 * W / G + 4.0 * ndet
     achieved = bytes_per_probe_slice * G / (t_step_ms * 1e-3) / 1e9
+    # measured HBM-side traffic of the launch pair (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated
+    # against a plain copy), per probe-slice at 128 probes per launch, scaled to this launch size
     traffic = None
     tp = os.path.join(ROOT, "profiles", "traffic_sweeps_512.json")
     if os.path.exists(tp):
         with open(tp) as f:
-            traffic = json.load(f).get("bytes_per_launch_pair")
+            per = json.load(f).get("bytes_per_probe_slice")
+        traffic = round(per * G) if per and G else None
 
     result = {
         "metric": METRIC,
