@@ -173,7 +173,11 @@ __global__ __launch_bounds__(kBlock, Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 
 // ============================================================================
 // Sweep B: ky columns of the band, TC columns per block staged through LDS.
 template <int NX> struct ColCfg {
-  static constexpr int TC = NX <= 512 ? 8 : (NX == 1024 ? 8 : (NX == 2048 ? 4 : 2));
+  // TC columns per tile; BLK threads per block.  At 512 a tile is 16 columns =
+  // one whole 128-byte line of each state row, staged by a 512-thread block, so
+  // no line is fetched by two blocks.
+  static constexpr int TC = NX == 512 ? 16 : (NX <= 1024 ? 8 : (NX == 2048 ? 4 : 2));
+  static constexpr int BLK = NX == 512 ? 512 : kBlock;
   // column slots use the PAD layout (registers are the scarcer resource here);
   // float2 per LDS column, rounded so the tile staging is at most 2-way
   static constexpr int STRIDE = ((lds_slot<NX, false>() + 31) / 32) * 32 + 2;
@@ -189,8 +193,9 @@ template <int NX> struct ColCfg {
 template <int NX>
 struct TileStage {
   static constexpr int TC = ColCfg<NX>::TC, ST = ColCfg<NX>::STRIDE, H = TC / 2;
-  static constexpr int IT = H * NX / kBlock, IB = IT < 8 ? IT : 8, XS = kBlock / H;
-  static_assert(H * NX % kBlock == 0 && kBlock % H == 0 && IT % IB == 0 && XS % 16 == 0,
+  static constexpr int BLK = ColCfg<NX>::BLK;
+  static constexpr int IT = H * NX / BLK, IB = IT < 8 ? IT : 8, XS = BLK / H;
+  static_assert(H * NX % BLK == 0 && BLK % H == 0 && IT % IB == 0 && XS % 16 == 0,
                 "tile staging assumes whole rounds");
   static constexpr int lstep(int i) { return XS * i + XS * i / 16; }
   rsrc_t rs;          // the probe's band-compact state
@@ -257,12 +262,12 @@ __device__ __forceinline__ float wave_sum(float s) {
 // A block advances a.tpb consecutive tiles of one probe (the two halves of a
 // 128-byte state line are adjacent tiles, so the second read hits L2).
 template <int NX, bool REC, bool DIFF>
-__global__ __launch_bounds__(kBlock) void k_sweep_cols(SweepArgs a) {
+__global__ __launch_bounds__(ColCfg<NX>::BLK) void k_sweep_cols(SweepArgs a) {
   constexpr int E = Plan<NX>::E;
   constexpr int T = NX / E;
   constexpr int TC = ColCfg<NX>::TC;
   constexpr int ST = ColCfg<NX>::STRIDE;
-  constexpr int G = kBlock / T;  // transforms in flight per block
+  constexpr int G = ColCfg<NX>::BLK / T;  // transforms in flight per block
   constexpr int ROUNDS = (TC + G - 1) / G;
   constexpr bool PXR = E <= 8;   // per-thread propagator factors and k^2 kept in registers
   extern __shared__ float2 tile[];  // TC * ST
@@ -486,10 +491,10 @@ void launch_cols_t(const SweepArgs& a0, bool rec, bool diff, hipStream_t st) {
   SweepArgs a = a0;
   if (a.tpb <= 0) a.tpb = 1;
   dim3 grid((unsigned)(a.nprobe * ((a.ntiles + a.tpb - 1) / a.tpb)));
-  if (rec && diff) hipLaunchKernelGGL((k_sweep_cols<N, true, true>), grid, dim3(kBlock), lds, st, a);
-  else if (rec) hipLaunchKernelGGL((k_sweep_cols<N, true, false>), grid, dim3(kBlock), lds, st, a);
-  else if (diff) hipLaunchKernelGGL((k_sweep_cols<N, false, true>), grid, dim3(kBlock), lds, st, a);
-  else hipLaunchKernelGGL((k_sweep_cols<N, false, false>), grid, dim3(kBlock), lds, st, a);
+  if (rec && diff) hipLaunchKernelGGL((k_sweep_cols<N, true, true>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
+  else if (rec) hipLaunchKernelGGL((k_sweep_cols<N, true, false>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
+  else if (diff) hipLaunchKernelGGL((k_sweep_cols<N, false, true>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
+  else hipLaunchKernelGGL((k_sweep_cols<N, false, false>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
 }
 
 template <int N>
@@ -531,13 +536,13 @@ void blue_t(float2* d, long nrows, int n, const float2* chirp, const float2* vha
 // Column sweep (slice s, reverse):  g_Phi = colFFT(g_X)  [last slice: 4 (I - T) Phi,
 // Phi = colFFT(X_{S-1})/nx, I = |Phi|^2]  ->  g_Psi = conj(P) g_Phi  ->  g_R = colIFFT(g_Psi).
 template <int NX, bool LAST>
-__global__ __launch_bounds__(kBlock) void k_adj_cols(AdjArgs A) {
+__global__ __launch_bounds__(ColCfg<NX>::BLK) void k_adj_cols(AdjArgs A) {
   const SweepArgs& a = A.f;
   constexpr int E = Plan<NX>::E;
   constexpr int T = NX / E;
   constexpr int TC = ColCfg<NX>::TC;
   constexpr int ST = ColCfg<NX>::STRIDE;
-  constexpr int G = kBlock / T;
+  constexpr int G = ColCfg<NX>::BLK / T;
   constexpr int ROUNDS = (TC + G - 1) / G;
   extern __shared__ float2 tile[];
   __shared__ float red;
@@ -688,8 +693,8 @@ template <int N>
 void launch_adj_cols_t(const AdjArgs& a, bool last, hipStream_t st) {
   const size_t lds = col_lds_bytes_t<N>();
   dim3 grid((unsigned)(a.f.nprobe * a.f.ntiles));
-  if (last) hipLaunchKernelGGL((k_adj_cols<N, true>), grid, dim3(kBlock), lds, st, a);
-  else hipLaunchKernelGGL((k_adj_cols<N, false>), grid, dim3(kBlock), lds, st, a);
+  if (last) hipLaunchKernelGGL((k_adj_cols<N, true>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
+  else hipLaunchKernelGGL((k_adj_cols<N, false>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
 }
 
 template <int N>
