@@ -16,6 +16,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 namespace qs {
 
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
@@ -52,6 +54,28 @@ __device__ __forceinline__ void dft4(float2& a0, float2& a1, float2& a2, float2&
   a3 = csub(t1, t3);
 }
 
+// dft4 with inputs known to be zero (bit j of Z: input j is zero); the
+// additions with a zero operand are left out.
+__device__ __forceinline__ float2 cneg(float2 a) { return make_float2(-a.x, -a.y); }
+template <int DIR, int Z>
+__device__ __forceinline__ void dft4z(float2& a0, float2& a1, float2& a2, float2& a3) {
+  if constexpr (Z == 0) {
+    dft4<DIR>(a0, a1, a2, a3);
+  } else {
+    float2 t0, t1, t2, t3;
+    if constexpr ((Z & 4) != 0) { t0 = a0; t1 = a0; }
+    else if constexpr ((Z & 1) != 0) { t0 = a2; t1 = cneg(a2); }
+    else { t0 = cadd(a0, a2); t1 = csub(a0, a2); }
+    if constexpr ((Z & 8) != 0) { t2 = a1; t3 = mul_i<DIR>(a1); }
+    else if constexpr ((Z & 2) != 0) { t2 = a3; t3 = mul_i<DIR>(cneg(a3)); }
+    else { t2 = cadd(a1, a3); t3 = mul_i<DIR>(csub(a1, a3)); }
+    a0 = cadd(t0, t2);
+    a2 = csub(t0, t2);
+    a1 = cadd(t1, t3);
+    a3 = csub(t1, t3);
+  }
+}
+
 // W8^k for k = 1,3 with sign DIR: e^{DIR*i*pi*k/4}
 template <int DIR>
 __device__ __forceinline__ float2 w8_1(float2 a) {  // a * e^{DIR i pi/4}
@@ -64,34 +88,49 @@ __device__ __forceinline__ float2 w8_3(float2 a) {  // a * e^{DIR i 3pi/4}
   return DIR < 0 ? make_float2((a.y - a.x) * h, -(a.x + a.y) * h) : make_float2(-(a.x + a.y) * h, (a.x - a.y) * h);
 }
 
-template <int DIR>
+// Z: bit j set when input j is zero; O: bit k set when output k is needed
+// (the others are left unspecified).
+template <int DIR, int Z = 0, int O = 0xff>
 __device__ __forceinline__ void dft8(float2* a) {
   // decimation in time: even/odd halves, then combine with W8^k
+  constexpr int ZE = ((Z >> 0) & 1) | (((Z >> 2) & 1) << 1) | (((Z >> 4) & 1) << 2) | (((Z >> 6) & 1) << 3);
+  constexpr int ZO = ((Z >> 1) & 1) | (((Z >> 3) & 1) << 1) | (((Z >> 5) & 1) << 2) | (((Z >> 7) & 1) << 3);
   float2 e0 = a[0], e1 = a[2], e2 = a[4], e3 = a[6];
   float2 o0 = a[1], o1 = a[3], o2 = a[5], o3 = a[7];
-  dft4<DIR>(e0, e1, e2, e3);
-  dft4<DIR>(o0, o1, o2, o3);
+  dft4z<DIR, ZE>(e0, e1, e2, e3);
+  dft4z<DIR, ZO>(o0, o1, o2, o3);
   o1 = w8_1<DIR>(o1);
   o2 = mul_i<DIR>(o2);
   o3 = w8_3<DIR>(o3);
-  a[0] = cadd(e0, o0); a[4] = csub(e0, o0);
-  a[1] = cadd(e1, o1); a[5] = csub(e1, o1);
-  a[2] = cadd(e2, o2); a[6] = csub(e2, o2);
-  a[3] = cadd(e3, o3); a[7] = csub(e3, o3);
+  if constexpr ((O & 0x01) != 0) a[0] = cadd(e0, o0);
+  if constexpr ((O & 0x10) != 0) a[4] = csub(e0, o0);
+  if constexpr ((O & 0x02) != 0) a[1] = cadd(e1, o1);
+  if constexpr ((O & 0x20) != 0) a[5] = csub(e1, o1);
+  if constexpr ((O & 0x04) != 0) a[2] = cadd(e2, o2);
+  if constexpr ((O & 0x40) != 0) a[6] = csub(e2, o2);
+  if constexpr ((O & 0x08) != 0) a[3] = cadd(e3, o3);
+  if constexpr ((O & 0x80) != 0) a[7] = csub(e3, o3);
 }
 
-template <int DIR>
+template <int DIR, int Z = 0>
 __device__ __forceinline__ void dft16(float2* a) {
   // 16 = 4 x 4: columns of stride 4, twiddles W16^{n1 k2}, rows of length 4
   const float c1 = 0.92387953251128675613f, s1 = 0.38268343236508977173f;
   const float h = 0.70710678118654752440f;
   float2 b[16];
-#pragma unroll
-  for (int n1 = 0; n1 < 4; n1++) {
+  auto colz = [](int n1) constexpr {
+    return ((Z >> n1) & 1) | (((Z >> (n1 + 4)) & 1) << 1) | (((Z >> (n1 + 8)) & 1) << 2) | (((Z >> (n1 + 12)) & 1) << 3);
+  };
+  auto col = [&](auto n1c) {
+    constexpr int n1 = decltype(n1c)::value;
     float2 x0 = a[n1], x1 = a[n1 + 4], x2 = a[n1 + 8], x3 = a[n1 + 12];
-    dft4<DIR>(x0, x1, x2, x3);
+    dft4z<DIR, colz(n1)>(x0, x1, x2, x3);
     b[n1 * 4 + 0] = x0; b[n1 * 4 + 1] = x1; b[n1 * 4 + 2] = x2; b[n1 * 4 + 3] = x3;
-  }
+  };
+  col(std::integral_constant<int, 0>{});
+  col(std::integral_constant<int, 1>{});
+  col(std::integral_constant<int, 2>{});
+  col(std::integral_constant<int, 3>{});
   // twiddle b[n1*4+k2] *= W16^{n1*k2}
   const float sg = (float)DIR;
   const float2 w1 = make_float2(c1, sg * s1), w2 = make_float2(h, sg * h), w3 = make_float2(s1, sg * c1);
@@ -113,12 +152,13 @@ __device__ __forceinline__ void dft16(float2* a) {
   }
 }
 
-template <int R, int DIR>
+// Z, O: zero inputs and needed outputs (dft8 prunes both, dft16 the inputs).
+template <int R, int DIR, int Z = 0, int O = -1>
 __device__ __forceinline__ void dft(float2* a) {
   if constexpr (R == 2) dft2<DIR>(a[0], a[1]);
-  else if constexpr (R == 4) dft4<DIR>(a[0], a[1], a[2], a[3]);
-  else if constexpr (R == 8) dft8<DIR>(a);
-  else if constexpr (R == 16) dft16<DIR>(a);
+  else if constexpr (R == 4) dft4z<DIR, Z & 15>(a[0], a[1], a[2], a[3]);
+  else if constexpr (R == 8) dft8<DIR, Z & 0xff, O & 0xff>(a);
+  else if constexpr (R == 16) dft16<DIR, Z & 0xffff>(a);
 }
 
 // ---- radix plans -------------------------------------------------------------
@@ -215,7 +255,10 @@ struct Twiddles {
 // output to registers (natural layout).  Otherwise through the LDS slot.
 // wr: this pass's register twiddles (or nullptr: read tab, the forward table
 // W_N^m = exp(-2 pi i m/N), m < N).
-template <int N, int R, int NS, int DIR, bool FIRST, bool LAST, bool SWZ>
+// ZIN (first pass) / NEED (last pass): masks over the natural-layout register
+// slots m (element t + T*m) that are zero on input / needed on output; used when
+// one butterfly covers all E slots (R == E), ignored otherwise.
+template <int N, int R, int NS, int DIR, bool FIRST, bool LAST, bool SWZ, int ZIN = 0, int NEED = -1>
 __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const float2* wr,
                                               const float2* __restrict__ tab, int t) {
   constexpr int E = Plan<N>::E;
@@ -250,7 +293,10 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
     }
   }
 #pragma unroll
-  for (int b = 0; b < NB; b++) dft<R, DIR>(a[b]);
+  for (int b = 0; b < NB; b++) {
+    if constexpr (NB == 1) dft<R, DIR, FIRST ? ZIN : 0, LAST ? NEED : -1>(a[b]);
+    else dft<R, DIR>(a[b]);
+  }
   if constexpr (LAST) {
 #pragma unroll
     for (int b = 0; b < NB; b++)
@@ -269,21 +315,35 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
   }
 }
 
+// Register slots m (element t + T*m) that lie, for every thread, strictly
+// inside the gap (N/3, N - N/3) of the 2/3 band limit of propagate_slow
+// (stem3/stemlib.cpp:2815-2849): the band keeps |index| <= floor(N/3) on each
+// axis, so these slots are zero in k-space after the propagator.
+template <int N>
+constexpr int gap_mask() {
+  constexpr int E = Plan<N>::E, T = N / E, A = N / 3;
+  int m = 0;
+  for (int j = 0; j < E && j < 31; j++)
+    if (T * j > A && T * j + T - 1 < N - A) m |= 1 << j;
+  return m;
+}
+
 // Full length-N transform of the thread-distributed vector v (natural layout
 // in and out).  slot: this transform's LDS region of lds_slot<N, SWZ>() float2.
-template <int N, int DIR, bool SWZ = kSwz<N>>
+// ZIN: slots known to be zero on input; NEED: slots needed on output.
+template <int N, int DIR, bool SWZ = kSwz<N>, int ZIN = 0, int NEED = -1>
 __device__ __forceinline__ void fft_wave(float2* v, float2* slot, const Twiddles<N>& tw, int t) {
   using P = Plan<N>;
   constexpr int S1 = P::R0;
   constexpr int S2 = P::R0 * P::R1;
   constexpr bool REG = Twiddles<N>::REG;
-  stockham_pass<N, P::R0, 1, DIR, true, false, SWZ>(v, slot, nullptr, tw.tab, t);
+  stockham_pass<N, P::R0, 1, DIR, true, false, SWZ, ZIN>(v, slot, nullptr, tw.tab, t);
   stockham_pass<N, P::R1, S1, DIR, false, false, SWZ>(v, slot, REG ? tw.w1 : nullptr, tw.tab, t);
   if constexpr (P::NP == 3) {
-    stockham_pass<N, P::R2, S2, DIR, false, true, SWZ>(v, slot, REG ? tw.w2 : nullptr, tw.tab, t);
+    stockham_pass<N, P::R2, S2, DIR, false, true, SWZ, 0, NEED>(v, slot, REG ? tw.w2 : nullptr, tw.tab, t);
   } else {
     stockham_pass<N, P::R2, S2, DIR, false, false, SWZ>(v, slot, nullptr, tw.tab, t);
-    stockham_pass<N, P::R3, S2 * P::R2, DIR, false, true, SWZ>(v, slot, nullptr, tw.tab, t);
+    stockham_pass<N, P::R3, S2 * P::R2, DIR, false, true, SWZ, 0, NEED>(v, slot, nullptr, tw.tab, t);
   }
 }
 

@@ -72,6 +72,7 @@ __global__ __launch_bounds__(kBlock, Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 
   constexpr int UPB = T <= 64 ? kBlock / 64 : 1;  // units (wavefronts, or the block) per block
   constexpr int SPB = kBlock / T;    // transform slots per block
   constexpr bool BOFF = E <= 16;     // keep the band offsets in registers
+  constexpr int GAP = gap_mask<NY>();  // slots in the band gap: never loaded, stored or computed
   __shared__ float2 lds[SPB * lds_slot<NY>()];
   const int lane = T <= 64 ? (threadIdx.x & 63) : threadIdx.x;
   const int sub = lane / T;
@@ -122,15 +123,17 @@ __global__ __launch_bounds__(kBlock, Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 
     } else {
       const unsigned so = (unsigned)(i * TPW) * rowb;
 #pragma unroll
-      for (int m = 0; m < E; m++)
-        dv[m] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, BOFF ? boff[m] : bo(m), so, 0));
+      for (int m = 0; m < E; m++) {
+        if ((GAP >> m) & 1) dv[m] = make_float2(0.f, 0.f);  // inside the band gap for every thread
+        else dv[m] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, BOFF ? boff[m] : bo(m), so, 0));
+      }
     }
   };
   Twiddles<NY> tw;
   tw.init(a.twy, t);
   // one step: finish row i (v: state or probe row, tv: its transmission row)
   auto step = [&](int i, float2* v, const float2* tv) {
-    if constexpr (MODE != 0) fft_wave<NY, +1>(v, slot, tw, t);
+    if constexpr (MODE != 0) fft_wave<NY, +1, kSwz<NY>, GAP>(v, slot, tw, t);
     const int xr = i * TPW + sub;
     if constexpr (MODE == 2) {
       float2* dst = a.out_wave + ((size_t)pg * a.nx + x0 + xr) * NY;
@@ -139,12 +142,13 @@ __global__ __launch_bounds__(kBlock, Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 
     } else {
 #pragma unroll
       for (int m = 0; m < E; m++) v[m] = cmul(v[m], tv[m]);
-      fft_wave<NY, -1>(v, slot, tw, t);
+      fft_wave<NY, -1, kSwz<NY>, 0, ~GAP>(v, slot, tw, t);
       const unsigned so = (unsigned)(i * TPW) * rowb;
       using U2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0u, 0, 0));
 #pragma unroll
       for (int m = 0; m < E; m++)
-        __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v[m]), rs, BOFF ? boff[m] : bo(m), so, 0);
+        if (!((GAP >> m) & 1))
+          __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v[m]), rs, BOFF ? boff[m] : bo(m), so, 0);
     }
   };
   if constexpr (E <= 8) {
@@ -270,6 +274,7 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_sweep_cols(SweepArgs a) {
   constexpr int G = ColCfg<NX>::BLK / T;  // transforms in flight per block
   constexpr int ROUNDS = (TC + G - 1) / G;
   constexpr bool PXR = E <= 8;   // per-thread propagator factors and k^2 kept in registers
+  constexpr int GAP = gap_mask<NX>();  // kx slots in the band gap: zero after the propagator
   extern __shared__ float2 tile[];  // TC * ST
   __shared__ float red[kMaxDet + 1];
 
@@ -321,13 +326,17 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_sweep_cols(SweepArgs a) {
       float2 v[E];
 #pragma unroll
       for (int m = 0; m < E; m++) v[m] = col[lidx<NX, false>(t + T * m)];
-      fft_wave<NX, -1, false>(v, col, tw, t);
+      fft_wave<NX, -1, false, 0, ~GAP>(v, col, tw, t);
       // propagate: P = px[ix]*py[iy] (1/(nx*ny) folded into px), zero outside k2max
       if (iy >= 0) {
         const float ky2 = a.ky2[iy];
         const float2 py = a.py[iy];
 #pragma unroll
         for (int m = 0; m < E; m++) {
+          if ((GAP >> m) & 1) {  // kx in the band gap for every thread
+            v[m] = make_float2(0.f, 0.f);
+            continue;
+          }
           const float kx2 = kx2_of(m);
           if (kx2 < a.k2max && (kx2 + ky2) < a.k2max) v[m] = cmul(cmul(v[m], py), px_of(m));
           else v[m] = make_float2(0.f, 0.f);
@@ -373,7 +382,7 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_sweep_cols(SweepArgs a) {
 #pragma unroll
         for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
       }
-      fft_wave<NX, +1, false>(v, col, tw, t);
+      fft_wave<NX, +1, false, GAP>(v, col, tw, t);
       xform_sync<NX>();
 #pragma unroll
       for (int m = 0; m < E; m++) col[lidx<NX, false>(t + T * m)] = v[m];
