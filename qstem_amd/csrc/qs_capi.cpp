@@ -1221,6 +1221,7 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
   A.target = c->target.as<float>();
   A.loss_partial = c->lossp.as<float>();
   A.gprobe = c->gprobe.as<float>();
+  A.single_writer = nprobe == 1;  // a single window: every transmission pixel has one writer per launch
   qs::launch_fill_zero(A.gx, st_elems, c->stream);
   for (int j = nseg - 1; j >= 0; j--) {
     const int s0 = j * K, s1 = std::min(S, s0 + K) - 1;

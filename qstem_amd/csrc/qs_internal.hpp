@@ -50,6 +50,7 @@ struct AdjArgs {
   float* loss_partial;      // [nprobe][ntiles] (last column sweep)
   float* gT;                // [potNx][potNy][2] gradient w.r.t. the transmission slice (atomics)
   float* gprobe;            // [nx][ny][2] sum over probes of the gradient w.r.t. psi_0 (atomics)
+  int single_writer;        // one probe per launch: gT updated without atomics
 };
 
 struct ProbeArgs {

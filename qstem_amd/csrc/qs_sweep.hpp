@@ -61,10 +61,10 @@ __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_rea
 // Each wavefront owns a chunk of TPW*a.rw consecutive rows of one probe
 // (TPW = 64/T transforms side by side, a.rw steps; the chunk divides nx), so
 // probe, window offset and row bases are wave-uniform scalars.  Twiddles and
-// band offsets are set up once per wavefront, and the loads of step i+1 are
-// in flight while step i is transformed.
+// band offsets are set up once per wavefront; all loads of a step are issued
+// before its first transform.
 template <int NY, int MODE>
-__global__ __launch_bounds__(kBlock, Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 ? 3 : 1) void k_sweep_rows(SweepArgs a) {
+__global__ __launch_bounds__(kBlock, Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 ? 4 : 1) void k_sweep_rows(SweepArgs a) {
   constexpr int E = Plan<NY>::E;
   constexpr int T = NY / E;
   static_assert(T <= 64 || T == kBlock, "a transform is a wavefront part or the whole block");
@@ -151,26 +151,12 @@ __global__ __launch_bounds__(kBlock, Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 
           __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v[m]), rs, BOFF ? boff[m] : bo(m), so, 0);
     }
   };
-  if constexpr (E <= 8) {
-    // two register sets in ping-pong: the loads of step i+1 are in flight during step i
-    float2 va[E], ta[E], vb[E], tb[E];
-    load(0, va, ta);
-    if (a.rw == 1) {
-      step(0, va, ta);
-      return;
-    }
-    for (int i = 0; i < a.rw; i += 2) {  // a.rw is even here
-      load(i + 1, vb, tb);
-      step(i, va, ta);
-      if (i + 2 < a.rw) load(i + 2, va, ta);
-      step(i + 1, vb, tb);
-    }
-  } else {  // long rows: one register set
-    float2 va[E], ta[E];
-    for (int i = 0; i < a.rw; i++) {
-      load(i, va, ta);
-      step(i, va, ta);
-    }
+  // one register set: at 4 wavefronts per SIMD the other wavefronts cover the
+  // load latency (measured equal to a two-set software pipeline at 3 per SIMD)
+  float2 va[E], ta[E];
+  for (int i = 0; i < a.rw; i++) {
+    load(i, va, ta);
+    step(i, va, ta);
   }
 }
 
@@ -666,8 +652,14 @@ __global__ __launch_bounds__(kBlock) void k_adj_rows(AdjArgs A) {
   for (int m = 0; m < E; m++) {
     const int iy = t + T * m;
     const float2 gtv = cmulc(g[m], w[m]);  // conj(psi) * g_a
-    atomicAdd(gt + 2 * iy, gtv.x);
-    atomicAdd(gt + 2 * iy + 1, gtv.y);
+    if (A.single_writer) {  // one probe per launch: each transmission pixel has one writer
+      float2* p2 = reinterpret_cast<float2*>(gt + 2 * iy);
+      const float2 o = *p2;
+      *p2 = make_float2(o.x + gtv.x, o.y + gtv.y);
+    } else {
+      atomicAdd(gt + 2 * iy, gtv.x);
+      atomicAdd(gt + 2 * iy + 1, gtv.y);
+    }
     g[m] = cmulc(g[m], tr[iy]);  // conj(t) * g_a
   }
   if constexpr (FIRST) {
