@@ -22,6 +22,9 @@
 namespace qs {
 
 constexpr int kBlock = 256;
+// Threads per block of the row-type kernels for transform length N: 256, or
+// the transform's own thread count when larger (one transform per block).
+template <int N> constexpr int row_block() { return (N / Plan<N>::E) > kBlock ? N / Plan<N>::E : kBlock; }
 
 
 // ============================================================================
@@ -64,13 +67,14 @@ __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_rea
 // band offsets are set up once per wavefront; all loads of a step are issued
 // before its first transform.
 template <int NY, int MODE>
-__global__ __launch_bounds__(kBlock, Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 ? 4 : 1) void k_sweep_rows(SweepArgs a) {
+__global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 ? 4 : 1) void k_sweep_rows(SweepArgs a) {
   constexpr int E = Plan<NY>::E;
   constexpr int T = NY / E;
-  static_assert(T <= 64 || T == kBlock, "a transform is a wavefront part or the whole block");
+  constexpr int BLK = row_block<NY>();
+  static_assert(T <= 64 || T == BLK, "a transform is a wavefront part or the whole block");
   constexpr int TPW = T <= 64 ? 64 / T : 1;   // transforms side by side per unit
-  constexpr int UPB = T <= 64 ? kBlock / 64 : 1;  // units (wavefronts, or the block) per block
-  constexpr int SPB = kBlock / T;    // transform slots per block
+  constexpr int UPB = T <= 64 ? BLK / 64 : 1;  // units (wavefronts, or the block) per block
+  constexpr int SPB = BLK / T;       // transform slots per block
   constexpr bool BOFF = E <= 16;     // keep the band offsets in registers
   constexpr int GAP = gap_mask<NY>();  // slots in the band gap: never loaded, stored or computed
   __shared__ float2 lds[SPB * lds_slot<NY>()];
@@ -167,7 +171,7 @@ template <int NX> struct ColCfg {
   // one whole 128-byte line of each state row, staged by a 512-thread block, so
   // no line is fetched by two blocks.
   static constexpr int TC = NX == 512 ? 16 : (NX <= 1024 ? 8 : (NX == 2048 ? 4 : 2));
-  static constexpr int BLK = NX == 512 ? 512 : kBlock;
+  static constexpr int BLK = NX == 512 ? 512 : row_block<NX>();
   // column slots use the PAD layout (registers are the scarcer resource here);
   // float2 per LDS column, rounded so the tile staging is at most 2-way
   static constexpr int STRIDE = ((lds_slot<NX, false>() + 31) / 32) * 32 + 2;
@@ -394,11 +398,11 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_sweep_cols(SweepArgs a) {
 // ============================================================================
 // Generic batched row transforms of length L (power of two), full rows.
 template <int L, int DIR>
-__global__ __launch_bounds__(kBlock) void k_rows_fft(float2* __restrict__ data, long nrows, const float2* __restrict__ tw,
-                                                     float scale) {
+__global__ __launch_bounds__(row_block<L>()) void k_rows_fft(float2* __restrict__ data, long nrows,
+                                                             const float2* __restrict__ tw, float scale) {
   constexpr int E = Plan<L>::E;
   constexpr int T = L / E;
-  constexpr int RPB = kBlock / T;
+  constexpr int RPB = row_block<L>() / T;
   __shared__ float2 lds[RPB * lds_slot<L>()];
   const int slot_id = threadIdx.x / T, t = threadIdx.x % T;
   const long row = (long)blockIdx.x * RPB + slot_id;
@@ -417,13 +421,13 @@ __global__ __launch_bounds__(kBlock) void k_rows_fft(float2* __restrict__ data, 
 // Bluestein transform of rows of length n through a length-M power-of-two FFT.
 // chirp[k] = exp(DIR*i*pi*k^2/n); vhat = FFT_M of the wrapped conj(chirp).
 template <int M>
-__global__ __launch_bounds__(kBlock) void k_rows_bluestein(float2* __restrict__ data, long nrows, int n,
+__global__ __launch_bounds__(row_block<M>()) void k_rows_bluestein(float2* __restrict__ data, long nrows, int n,
                                                            const float2* __restrict__ chirp,
                                                            const float2* __restrict__ vhat,
                                                            const float2* __restrict__ tw, float scale) {
   constexpr int E = Plan<M>::E;
   constexpr int T = M / E;
-  constexpr int RPB = kBlock / T;
+  constexpr int RPB = row_block<M>() / T;
   __shared__ float2 lds[RPB * lds_slot<M>()];
   const int slot_id = threadIdx.x / T, t = threadIdx.x % T;
   const long row = (long)blockIdx.x * RPB + slot_id;
@@ -460,7 +464,7 @@ template <int N>
 void launch_rows_mode(const SweepArgs& a0, int mode, hipStream_t st) {
   constexpr int T = N / Plan<N>::E;
   constexpr int TPW = T <= 64 ? 64 / T : 1;
-  constexpr int UPB = T <= 64 ? kBlock / 64 : 1;
+  constexpr int UPB = T <= 64 ? row_block<N>() / 64 : 1;
   SweepArgs a = a0;
   const long rows = (long)a.nprobe * a.nx;
   const int max_rw = a.rw > 0 ? a.rw : 4;  // caller's cap (0: default, measured best at 512^2)
@@ -470,9 +474,9 @@ void launch_rows_mode(const SweepArgs& a0, int mode, hipStream_t st) {
   a.rw = rw;
   const long units = rows / ((long)TPW * rw);
   dim3 grid((unsigned)((units + UPB - 1) / UPB));
-  if (mode == 0) hipLaunchKernelGGL((k_sweep_rows<N, 0>), grid, dim3(kBlock), 0, st, a);
-  else if (mode == 1) hipLaunchKernelGGL((k_sweep_rows<N, 1>), grid, dim3(kBlock), 0, st, a);
-  else hipLaunchKernelGGL((k_sweep_rows<N, 2>), grid, dim3(kBlock), 0, st, a);
+  if (mode == 0) hipLaunchKernelGGL((k_sweep_rows<N, 0>), grid, dim3(row_block<N>()), 0, st, a);
+  else if (mode == 1) hipLaunchKernelGGL((k_sweep_rows<N, 1>), grid, dim3(row_block<N>()), 0, st, a);
+  else hipLaunchKernelGGL((k_sweep_rows<N, 2>), grid, dim3(row_block<N>()), 0, st, a);
 }
 
 template <int N>
@@ -511,16 +515,16 @@ hipError_t set_col_lds_t() {
 
 template <int L, int DIR>
 void rows_fft_t(float2* d, long nrows, const float2* tw, float scale, hipStream_t st) {
-  constexpr int RPB = kBlock / (L / Plan<L>::E);
-  hipLaunchKernelGGL((k_rows_fft<L, DIR>), dim3((unsigned)((nrows + RPB - 1) / RPB)), dim3(kBlock), 0, st, d, nrows, tw,
+  constexpr int RPB = row_block<L>() / (L / Plan<L>::E);
+  hipLaunchKernelGGL((k_rows_fft<L, DIR>), dim3((unsigned)((nrows + RPB - 1) / RPB)), dim3(row_block<L>()), 0, st, d, nrows, tw,
                      scale);
 }
 
 template <int M>
 void blue_t(float2* d, long nrows, int n, const float2* chirp, const float2* vhat, const float2* tw, float scale,
             hipStream_t st) {
-  constexpr int RPB = kBlock / (M / Plan<M>::E);
-  hipLaunchKernelGGL((k_rows_bluestein<M>), dim3((unsigned)((nrows + RPB - 1) / RPB)), dim3(kBlock), 0, st, d, nrows, n,
+  constexpr int RPB = row_block<M>() / (M / Plan<M>::E);
+  hipLaunchKernelGGL((k_rows_bluestein<M>), dim3((unsigned)((nrows + RPB - 1) / RPB)), dim3(row_block<M>()), 0, st, d, nrows, n,
                      chirp, vhat, tw, scale);
 }
 
@@ -606,11 +610,11 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_adj_cols(AdjArgs A) {
 // probe);  g_t += conj(psi_s) g_a;  g_psi = conj(t_s) g_a;  g_X_{s-1} = rowFFT(g_psi)
 // (s > 0) or sum_p g_psi -> gprobe (s = 0).
 template <int NY, bool FIRST>
-__global__ __launch_bounds__(kBlock) void k_adj_rows(AdjArgs A) {
+__global__ __launch_bounds__(row_block<NY>()) void k_adj_rows(AdjArgs A) {
   const SweepArgs& a = A.f;
   constexpr int E = Plan<NY>::E;
   constexpr int T = NY / E;
-  constexpr int RPB = kBlock / T;
+  constexpr int RPB = row_block<NY>() / T;
   __shared__ float2 lds[RPB * lds_slot<NY>()];
   const int slot_id = threadIdx.x / T;
   const int t = threadIdx.x % T;
@@ -683,11 +687,11 @@ __global__ __launch_bounds__(kBlock) void k_adj_rows(AdjArgs A) {
 template <int N>
 void launch_adj_rows_t(const AdjArgs& a, bool first, hipStream_t st) {
   constexpr int T = N / Plan<N>::E;
-  constexpr int RPB = kBlock / T;
+  constexpr int RPB = row_block<N>() / T;
   const long rows = (long)a.f.nprobe * a.f.nx;
   dim3 grid((unsigned)((rows + RPB - 1) / RPB));
-  if (first) hipLaunchKernelGGL((k_adj_rows<N, true>), grid, dim3(kBlock), 0, st, a);
-  else hipLaunchKernelGGL((k_adj_rows<N, false>), grid, dim3(kBlock), 0, st, a);
+  if (first) hipLaunchKernelGGL((k_adj_rows<N, true>), grid, dim3(row_block<N>()), 0, st, a);
+  else hipLaunchKernelGGL((k_adj_rows<N, false>), grid, dim3(row_block<N>()), 0, st, a);
 }
 
 template <int N>

