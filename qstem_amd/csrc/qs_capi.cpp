@@ -147,6 +147,14 @@ struct qs_ctx {
   DevBuf state, ipos, partial, det, diff, exitw, total;
   DevBuf pkx2, pky2, probe_partial;
   RowXform rx, ry;  // potNx and potNy row transforms
+  // band limit of the transmission functions as band-compact sweeps (potNx and
+  // potNy powers of two): row FFT into the ky band, column FFT / disk mask with
+  // 1/(potNx potNy) / column IFFT, row IFFT; else row transforms and transposes
+  struct BandSweeps {
+    bool on = false;
+    int band_a = 0, band_b = 0, kc = 0, kcp = 0, ntiles = 0;
+  } bl;
+  DevBuf blpx, blpy, blstate;
   // potential builder
   std::vector<double> scat;  // 93 x 123 Rez table
   DevBuf atoms, kinds, lutptrs, spl, luttmp, lutscr;
@@ -228,6 +236,15 @@ static void run_rowxform(qs_ctx* c, RowXform& r, float2* data, long nrows, int d
   else
     qs::launch_rows_bluestein(data, nrows, r.n, r.M, (dir < 0 ? r.chirp_f : r.chirp_b).as<float2>(),
                               (dir < 0 ? r.vhat_f : r.vhat_b).as<float2>(), r.tw.as<float2>(), scale, c->stream);
+}
+
+// k^2 limit of the transmission band limit (stemlib.cpp:2219-2262), float arithmetic
+static float bl_k2max(const qs_params& p) {
+  float k2max = p.potNx / (2.0F * p.potSizeX);
+  const float t = p.potNy / (2.0F * p.potSizeY);
+  if (t < k2max) k2max = t;
+  k2max = (float)((2.0F / 3.0F) * k2max);
+  return k2max * k2max;
 }
 
 extern "C" {
@@ -352,6 +369,33 @@ int qs_create(const qs_params* p, qs_ctx** out) {
     int rc = setup_rowxform(c, c->rx, p->potNx);
     if (rc == QS_OK) rc = setup_rowxform(c, c->ry, p->potNy);
     if (rc != QS_OK) return bad(rc);
+    // band-compact sweeps when both lengths have Stockham plans and the ky band
+    // is two contiguous column ranges, as for the wave
+    if (qs::supported_size(p->potNx) && qs::supported_size(p->potNy)) {
+      const float k2m = bl_k2max(*p);
+      const int pny = p->potNy;
+      int ba = -1;
+      while (ba + 1 <= pny / 2 && ky2t[ba + 1] < k2m) ba++;
+      int bb = 0;
+      while (bb < pny / 2 - 1 && ky2t[pny - 1 - bb] < k2m) bb++;
+      bool contiguous = ba >= 0;
+      for (int i = 0; i < pny && contiguous; i++) contiguous = ((i <= ba) || (i >= pny - bb)) == (ky2t[i] < k2m);
+      if (contiguous) {
+        c->bl.band_a = ba;
+        c->bl.band_b = bb;
+        c->bl.kc = ba + 1 + bb;
+        const int tc = qs::col_tile(p->potNx);
+        c->bl.ntiles = (c->bl.kc + tc - 1) / tc;
+        c->bl.kcp = (c->bl.ntiles * tc + 15) / 16 * 16;
+        const float s = (float)(1.0 / ((double)p->potNx * p->potNy));
+        if (upload(c->blpx, std::vector<float2>(p->potNx, make_float2(s, 0.f)), c->stream) != hipSuccess ||
+            upload(c->blpy, std::vector<float2>(pny, make_float2(1.f, 0.f)), c->stream) != hipSuccess) {
+          c->err = "device allocation failed";
+          return bad(QS_ERR_HIP);
+        }
+        c->bl.on = true;
+      }
+    }
   }
   if (hipStreamSynchronize(c->stream) != hipSuccess) { c->err = "stream sync failed"; return bad(QS_ERR_HIP); }
   *out = c;
@@ -375,7 +419,8 @@ void qs_destroy(qs_ctx* c) {
                     &c->atoms, &c->kinds, &c->lutptrs, &c->spl, &c->luttmp, &c->lutscr,
                     &c->lx.tw, &c->lx.chirp_f, &c->lx.vhat_f, &c->lx.chirp_b, &c->lx.vhat_b,
                     &c->lz.tw, &c->lz.chirp_f, &c->lz.vhat_f, &c->lz.chirp_b, &c->lz.vhat_b,
-                    &c->gx, &c->work, &c->target, &c->lossp, &c->gT, &c->gprobe, &c->gV, &c->dsum};
+                    &c->gx, &c->work, &c->target, &c->lossp, &c->gT, &c->gprobe, &c->gV, &c->dsum,
+                    &c->blpx, &c->blpy, &c->blstate};
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : c->luts) b.release();
   for (DevBuf& b : c->ckpt) b.release();
@@ -386,21 +431,57 @@ void qs_destroy(qs_ctx* c) {
   delete c;
 }
 
+// Band-limit scratch: band-compact rows of this many bytes at most per call chunk.
+static constexpr size_t kBandScratchBytes = size_t(1) << 30;
+
 // Band limit of initSTEMSlices (FFT, 2/3 mask with 1/(potNx potNy), IFFT) applied in
 // place to `tr` [nslice][potNx][potNy]; the operator is self-adjoint, so the adjoint
 // uses it unchanged on the transmission gradient.
 static int bandlimit_buf(qs_ctx* c, float2* tr, int nslice) {
   const qs_params& p = c->p;
   const int nx = p.potNx, ny = p.potNy;
+  const float k2max = bl_k2max(p);
+  if (c->bl.on) {
+    // three sweeps per chunk of slices, each slice a "probe" of the sweep kernels:
+    // the column sweep's propagator is 1/(nx ny) inside the disk k^2 < k2max
+    const size_t slice = (size_t)nx * ny;
+    const size_t per = sizeof(float2) * (size_t)nx * c->bl.kcp;
+    const int q = (int)std::max<size_t>(1, std::min<size_t>((size_t)nslice, kBandScratchBytes / per));
+    HIPCHK(c, c->blstate.ensure(per * q));
+    qs::SweepArgs a{};
+    a.nx = nx;
+    a.ny = ny;
+    a.kc = c->bl.kc;
+    a.kcp = c->bl.kcp;
+    a.band_a = c->bl.band_a;
+    a.band_b = c->bl.band_b;
+    a.ntiles = c->bl.ntiles;
+    a.potNy = ny;
+    a.twx = c->rx.tw.as<float2>();
+    a.twy = c->ry.tw.as<float2>();
+    a.px = c->blpx.as<float2>();
+    a.py = c->blpy.as<float2>();
+    a.kx2 = c->pkx2.as<float>();
+    a.ky2 = c->pky2.as<float>();
+    a.k2max = k2max;
+    a.rec_bin = -1;
+    a.state = c->blstate.as<float2>();
+    for (int s0 = 0; s0 < nslice; s0 += q) {
+      a.nprobe = std::min(q, nslice - s0);
+      a.probe = tr + (size_t)s0 * slice;
+      a.out_wave = tr + (size_t)s0 * slice;
+      qs::launch_sweep_rows(a, 3, c->stream);
+      qs::launch_sweep_cols(a, false, false, c->stream);
+      qs::launch_sweep_rows(a, 2, c->stream);
+    }
+    HIPCHK(c, hipGetLastError());
+    return QS_OK;
+  }
+  HIPCHK(c, c->scratch.ensure(sizeof(float2) * (size_t)nslice * nx * ny));
   float2* sc = c->scratch.as<float2>();
   run_rowxform(c, c->ry, tr, (long)nslice * nx, -1, 1.0f);
   qs::launch_transpose(tr, sc, nslice, nx, ny, c->stream);
   run_rowxform(c, c->rx, sc, (long)nslice * ny, -1, 1.0f);
-  float k2max = nx / (2.0F * p.potSizeX);
-  const float t = ny / (2.0F * p.potSizeY);
-  if (t < k2max) k2max = t;
-  k2max = (float)((2.0F / 3.0F) * k2max);
-  k2max = k2max * k2max;
   qs::launch_bandlimit_T(sc, nslice, ny, nx, c->pkx2.as<float>(), c->pky2.as<float>(), k2max,
                          (float)(1.0 / ((double)nx * ny)), c->stream);
   run_rowxform(c, c->rx, sc, (long)nslice * ny, +1, 1.0f);
@@ -446,7 +527,6 @@ static int build_trans(qs_ctx* c, int nslice) {
   qs::launch_phase_grating(c->V.as<float>(), t, n, scale, c->stream);
   HIPCHK(c, hipGetLastError());
   if (p.bandlimittrans) {
-    HIPCHK(c, c->scratch.ensure(sizeof(float2) * n));
     rc = bandlimit_buf(c, t, nslice);
     if (rc != QS_OK) return rc;
   }
@@ -1198,7 +1278,9 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
   HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * S));
   HIPCHK(c, c->gprobe.ensure(sizeof(float2) * (size_t)p.nx * p.ny));
   HIPCHK(c, c->dsum.ensure(sizeof(double) * 4));
-  qs::launch_cplx_fill_zero(c->gT.as<float>(), 2 * slice_elems * S, c->stream);
+  // one window covering the whole slice: every gT element is stored once per slice
+  const bool whole = nprobe == 1 && p.nx == p.potNx && p.ny == p.potNy;
+  if (!whole) qs::launch_cplx_fill_zero(c->gT.as<float>(), 2 * slice_elems * S, c->stream);
   qs::launch_cplx_fill_zero(c->gprobe.as<float>(), 2 * (size_t)p.nx * p.ny, c->stream);
   qs::launch_cplx_fill_zero(reinterpret_cast<float*>(c->dsum.p), 8, c->stream);
   SweepArgs a;
@@ -1221,7 +1303,7 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
   A.target = c->target.as<float>();
   A.loss_partial = c->lossp.as<float>();
   A.gprobe = c->gprobe.as<float>();
-  A.single_writer = nprobe == 1;  // a single window: every transmission pixel has one writer per launch
+  A.single_writer = nprobe == 1 ? (whole ? 2 : 1) : 0;  // a single window: one writer per pixel and launch
   qs::launch_fill_zero(A.gx, st_elems, c->stream);
   for (int j = nseg - 1; j >= 0; j--) {
     const int s0 = j * K, s1 = std::min(S, s0 + K) - 1;
@@ -1253,7 +1335,6 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
   // dL/dV: band limit (self-adjoint) of dL/dt, then sigma Im(g conj(u))
   if (grad_V) {
     if (p.bandlimittrans) {
-      HIPCHK(c, c->scratch.ensure(sizeof(float2) * slice_elems * S));
       int rc = bandlimit_buf(c, c->gT.as<float2>(), S);
       if (rc != QS_OK) return rc;
     }

@@ -50,7 +50,8 @@ struct AdjArgs {
   float* loss_partial;      // [nprobe][ntiles] (last column sweep)
   float* gT;                // [potNx][potNy][2] gradient w.r.t. the transmission slice (atomics)
   float* gprobe;            // [nx][ny][2] sum over probes of the gradient w.r.t. psi_0 (atomics)
-  int single_writer;        // one probe per launch: gT updated without atomics
+  int single_writer;        // 1: one probe per launch, gT updated without atomics;
+                            // 2: and its window is the whole slice, gT stored (no prior clear)
 };
 
 struct ProbeArgs {

@@ -61,6 +61,8 @@ __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_rea
 // mode 0: input is the shared real-space probe (first slice of a fresh run)
 // mode 1: input is the band-compact state; row IFFT first
 // mode 2: exit: row IFFT of the state, write the real-space wave, no transmit
+// mode 3: input is a full real-space row of each probe (a.probe + probe*nx*NY);
+//         row FFT only, no transmit (first sweep of a band limit)
 // Each wavefront owns a chunk of TPW*a.rw consecutive rows of one probe
 // (TPW = 64/T transforms side by side, a.rw steps; the chunk divides nx), so
 // probe, window offset and row bases are wave-uniform scalars.  Twiddles and
@@ -107,7 +109,7 @@ __global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>:
   }
   int2 ip = make_int2(0, 0);
   const float2* trc = a.trans;  // this probe's configuration, slice s
-  if constexpr (MODE != 2) {
+  if constexpr (MODE <= 1) {
     ip = a.ipos[pg];
     if (a.cfg) trc += (size_t)a.cfg[pg] * (size_t)a.cfg_stride;
   }
@@ -115,13 +117,13 @@ __global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>:
 
   auto load = [&](int i, float2* dv, float2* dt) {
     const int xr = i * TPW + sub;  // row within the chunk
-    if constexpr (MODE != 2) {
+    if constexpr (MODE <= 1) {
       const float2* tr = tr0 + (size_t)xr * a.potNy;
 #pragma unroll
       for (int m = 0; m < E; m++) dt[m] = tr[t + T * m];
     }
-    if constexpr (MODE == 0) {
-      const float2* src = a.probe + (size_t)(x0 + xr) * NY;
+    if constexpr (MODE == 0 || MODE == 3) {
+      const float2* src = a.probe + ((MODE == 3 ? (size_t)pg * a.nx : (size_t)0) + x0 + xr) * NY;
 #pragma unroll
       for (int m = 0; m < E; m++) dv[m] = src[t + T * m];
     } else {
@@ -137,15 +139,17 @@ __global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>:
   tw.init(a.twy, t);
   // one step: finish row i (v: state or probe row, tv: its transmission row)
   auto step = [&](int i, float2* v, const float2* tv) {
-    if constexpr (MODE != 0) fft_wave<NY, +1, kSwz<NY>, GAP>(v, slot, tw, t);
+    if constexpr (MODE == 1 || MODE == 2) fft_wave<NY, +1, kSwz<NY>, GAP>(v, slot, tw, t);
     const int xr = i * TPW + sub;
     if constexpr (MODE == 2) {
       float2* dst = a.out_wave + ((size_t)pg * a.nx + x0 + xr) * NY;
 #pragma unroll
       for (int m = 0; m < E; m++) dst[t + T * m] = v[m];
     } else {
+      if constexpr (MODE <= 1) {
 #pragma unroll
-      for (int m = 0; m < E; m++) v[m] = cmul(v[m], tv[m]);
+        for (int m = 0; m < E; m++) v[m] = cmul(v[m], tv[m]);
+      }
       fft_wave<NY, -1, kSwz<NY>, 0, ~GAP>(v, slot, tw, t);
       const unsigned so = (unsigned)(i * TPW) * rowb;
       using U2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(rs, 0u, 0, 0));
@@ -172,6 +176,9 @@ template <int NX> struct ColCfg {
   // no line is fetched by two blocks.
   static constexpr int TC = NX == 512 ? 16 : (NX <= 1024 ? 8 : (NX == 2048 ? 4 : 2));
   static constexpr int BLK = NX == 512 ? 512 : row_block<NX>();
+  // tiles grouped on one XCD: those sharing a 128-byte line (16 columns); pairs
+  // of whole-line tiles at 512 (measured faster than no grouping)
+  static constexpr int GS = TC >= 16 ? 2 : 16 / TC;
   // column slots use the PAD layout (registers are the scarcer resource here);
   // float2 per LDS column, rounded so the tile staging is at most 2-way
   static constexpr int STRIDE = ((lds_slot<NX, false>() + 31) / 32) * 32 + 2;
@@ -235,15 +242,16 @@ struct TileStage {
   }
 };
 
-// Work item of block blockIdx.x such that work items 2j and 2j+1 (the two
-// 64-byte halves of the state's 128-byte lines when a block stages one tile)
+// Work item of block blockIdx.x such that work items GS*j ... GS*j+GS-1 (the
+// tiles that share the state's 128-byte lines when a block stages one tile)
 // run back to back on the same XCD: blocks are dispatched round-robin over the
-// 8 XCDs, each with its own L2, so blocks b and b+8 share an L2.  The last
-// partial group of 16 blocks keeps the identity mapping.
-__device__ __forceinline__ int xcd_pair_block(int blk, int nblk) {
-  if (blk >= (nblk & ~15)) return blk;
+// 8 XCDs, each with its own L2, so blocks b, b+8, ..., b+8*(GS-1) share an L2.
+// The last partial group of 8*GS blocks keeps the identity mapping.
+template <int GS>
+__device__ __forceinline__ int xcd_group_block(int blk, int nblk) {
+  if (blk >= (nblk & ~(8 * GS - 1))) return blk;
   const int xcd = blk & 7, slot = blk >> 3;
-  return (slot >> 1) * 16 + xcd * 2 + (slot & 1);
+  return (slot / GS) * (8 * GS) + xcd * GS + (slot % GS);
 }
 
 // Sum over the 64 lanes of a wavefront (result valid in every lane).
@@ -253,8 +261,9 @@ __device__ __forceinline__ float wave_sum(float s) {
   return s;
 }
 
-// A block advances a.tpb consecutive tiles of one probe (the two halves of a
-// 128-byte state line are adjacent tiles, so the second read hits L2).
+// A block advances a.tpb consecutive tiles of one probe; tiles that share
+// 128-byte state lines run on one XCD at about the same time, so the line is
+// fetched from HBM once and the others hit L2.
 template <int NX, bool REC, bool DIFF>
 __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_sweep_cols(SweepArgs a) {
   constexpr int E = Plan<NX>::E;
@@ -269,7 +278,7 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_sweep_cols(SweepArgs a) {
   __shared__ float red[kMaxDet + 1];
 
   const int bpp = (a.ntiles + a.tpb - 1) / a.tpb;  // blocks per probe
-  const int b = xcd_pair_block(blockIdx.x, gridDim.x);
+  const int b = xcd_group_block<ColCfg<NX>::GS>(blockIdx.x, gridDim.x);
   const int p = b / bpp;
   const int tile_first = (b % bpp) * a.tpb;
   const int tile_end = min(tile_first + a.tpb, a.ntiles);
@@ -476,7 +485,8 @@ void launch_rows_mode(const SweepArgs& a0, int mode, hipStream_t st) {
   dim3 grid((unsigned)((units + UPB - 1) / UPB));
   if (mode == 0) hipLaunchKernelGGL((k_sweep_rows<N, 0>), grid, dim3(row_block<N>()), 0, st, a);
   else if (mode == 1) hipLaunchKernelGGL((k_sweep_rows<N, 1>), grid, dim3(row_block<N>()), 0, st, a);
-  else hipLaunchKernelGGL((k_sweep_rows<N, 2>), grid, dim3(row_block<N>()), 0, st, a);
+  else if (mode == 2) hipLaunchKernelGGL((k_sweep_rows<N, 2>), grid, dim3(row_block<N>()), 0, st, a);
+  else hipLaunchKernelGGL((k_sweep_rows<N, 3>), grid, dim3(row_block<N>()), 0, st, a);
 }
 
 template <int N>
@@ -545,8 +555,9 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_adj_cols(AdjArgs A) {
   constexpr int ROUNDS = (TC + G - 1) / G;
   extern __shared__ float2 tile[];
   __shared__ float red;
-  const int p = blockIdx.x / a.ntiles;
-  const int tile_id = blockIdx.x % a.ntiles;
+  const int b = xcd_group_block<ColCfg<NX>::GS>(blockIdx.x, gridDim.x);
+  const int p = b / a.ntiles;
+  const int tile_id = b % a.ntiles;
   const int pg = a.probe0 + p;
   const int c0 = tile_id * TC;
   const size_t off = (size_t)pg * a.nx * a.kcp;
@@ -656,7 +667,9 @@ __global__ __launch_bounds__(row_block<NY>()) void k_adj_rows(AdjArgs A) {
   for (int m = 0; m < E; m++) {
     const int iy = t + T * m;
     const float2 gtv = cmulc(g[m], w[m]);  // conj(psi) * g_a
-    if (A.single_writer) {  // one probe per launch: each transmission pixel has one writer
+    if (A.single_writer == 2) {  // the window is the whole slice: this is the pixel's only term
+      *reinterpret_cast<float2*>(gt + 2 * iy) = gtv;
+    } else if (A.single_writer) {  // one probe per launch: each transmission pixel has one writer
       float2* p2 = reinterpret_cast<float2*>(gt + 2 * iy);
       const float2 o = *p2;
       *p2 = make_float2(o.x + gtv.x, o.y + gtv.y);

@@ -28,10 +28,12 @@ def _target(oracle, cfg, V, ipos):
     return I.astype(np.float32)
 
 
-@pytest.mark.parametrize("bandlimit", [1, 0])
-def test_adjoint_loss_and_gradients_fd(oracle, bandlimit):
+@pytest.mark.parametrize("bandlimit,pad", [(1, 24), (0, 24), (1, 128)])
+def test_adjoint_loss_and_gradients_fd(oracle, bandlimit, pad):
+    """pad 24: 152^2 transmission grid (band limit by row transforms and transposes);
+    pad 128: 256^2 (band limit by band-compact sweeps, gT from three windows)."""
     from qstem_amd import Multislice
-    cfg = _cfg(bandlimit)
+    cfg = _cfg(bandlimit, pad=pad)
     V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=31, vmax=20.0)
     ipos = probe_positions(cfg, 3, seed=4)
     target = _target(oracle, cfg, V, ipos)

@@ -29,8 +29,10 @@ def test_probe_matches_oracle(oracle, n):
     assert rel_l2(g, o) < 2e-5
 
 
-@pytest.mark.parametrize("pot", [(160, 160), (128, 128), (200, 136)])
+@pytest.mark.parametrize("pot", [(160, 160), (128, 128), (200, 136), (256, 128), (128, 512), (1024, 2048)])
 def test_transmission_matches_oracle(oracle, pot):
+    """Power-of-two grids take the band-compact sweeps, the others row transforms
+    (Bluestein where needed) and transposes."""
     cfg = small_config(nx=128, ny=128, pad=0)
     cfg.potNx, cfg.potNy = pot
     V = smooth_potential(3, pot[0], pot[1], seed=3)
@@ -41,8 +43,9 @@ def test_transmission_matches_oracle(oracle, pot):
     assert rel_l2(t, o) < 2e-5
 
 
-def test_transmission_quirk_q1(oracle):
-    cfg = small_config(nx=128, ny=128, pad=32, compat=1)
+@pytest.mark.parametrize("pad", [32, 128])
+def test_transmission_quirk_q1(oracle, pad):
+    cfg = small_config(nx=128, ny=128, pad=pad, compat=1)
     V = smooth_potential(2, cfg.potNx, cfg.potNy, seed=4)
     with _ctx(cfg) as ms:
         ms.set_potential(V)
