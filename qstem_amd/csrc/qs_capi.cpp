@@ -168,7 +168,7 @@ struct qs_ctx {
   bool have_probe = false;
   // adjoint
   std::vector<DevBuf> ckpt;
-  DevBuf gx, work, target, lossp, gT, gprobe, gV, dsum;
+  DevBuf gx, target, lossp, gT, gprobe, gV, dsum;
   int state_probes = 0;
   bool state_valid = false;
   // optional per-launch timing of the two sweeps (HIP events on the context stream)
@@ -419,7 +419,7 @@ void qs_destroy(qs_ctx* c) {
                     &c->atoms, &c->kinds, &c->lutptrs, &c->spl, &c->luttmp, &c->lutscr,
                     &c->lx.tw, &c->lx.chirp_f, &c->lx.vhat_f, &c->lx.chirp_b, &c->lx.vhat_b,
                     &c->lz.tw, &c->lz.chirp_f, &c->lz.vhat_f, &c->lz.chirp_b, &c->lz.vhat_b,
-                    &c->gx, &c->work, &c->target, &c->lossp, &c->gT, &c->gprobe, &c->gV, &c->dsum,
+                    &c->gx, &c->target, &c->lossp, &c->gT, &c->gprobe, &c->gV, &c->dsum,
                     &c->blpx, &c->blpy, &c->blstate};
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : c->luts) b.release();
@@ -437,46 +437,58 @@ static constexpr size_t kBandScratchBytes = size_t(1) << 30;
 // Band limit of initSTEMSlices (FFT, 2/3 mask with 1/(potNx potNy), IFFT) applied in
 // place to `tr` [nslice][potNx][potNy]; the operator is self-adjoint, so the adjoint
 // uses it unchanged on the transmission gradient.
+// Band limit as band-compact sweeps, chunks of slices at a time, each slice a
+// "probe" of the sweep kernels: row FFT into the ky band (rows mode 3), column
+// FFT / propagator 1/(nx ny) inside the disk k^2 < k2max / column IFFT, row IFFT
+// written back to `tr` (rows mode 2) or, with gv given, turned into
+// dL/dV = sigma Im(g conj(exp(i sigma V))) of the slices' potential vpot (mode 4).
+static int bl_sweeps(qs_ctx* c, float2* tr, int nslice, const float* vpot, float* gv, float sigma) {
+  const qs_params& p = c->p;
+  const int nx = p.potNx, ny = p.potNy;
+  const size_t slice = (size_t)nx * ny;
+  const size_t per = sizeof(float2) * (size_t)nx * c->bl.kcp;
+  const int q = (int)std::max<size_t>(1, std::min<size_t>((size_t)nslice, kBandScratchBytes / per));
+  HIPCHK(c, c->blstate.ensure(per * q));
+  qs::SweepArgs a{};
+  a.nx = nx;
+  a.ny = ny;
+  a.kc = c->bl.kc;
+  a.kcp = c->bl.kcp;
+  a.band_a = c->bl.band_a;
+  a.band_b = c->bl.band_b;
+  a.ntiles = c->bl.ntiles;
+  a.potNy = ny;
+  a.twx = c->rx.tw.as<float2>();
+  a.twy = c->ry.tw.as<float2>();
+  a.px = c->blpx.as<float2>();
+  a.py = c->blpy.as<float2>();
+  a.kx2 = c->pkx2.as<float>();
+  a.ky2 = c->pky2.as<float>();
+  a.k2max = bl_k2max(p);
+  a.rec_bin = -1;
+  a.state = c->blstate.as<float2>();
+  a.gv_sigma = sigma;
+  for (int s0 = 0; s0 < nslice; s0 += q) {
+    a.nprobe = std::min(q, nslice - s0);
+    a.probe = tr + (size_t)s0 * slice;
+    a.out_wave = tr + (size_t)s0 * slice;
+    if (gv) {
+      a.vpot = vpot + (size_t)s0 * slice;
+      a.gv = gv + (size_t)s0 * slice;
+    }
+    qs::launch_sweep_rows(a, 3, c->stream);
+    qs::launch_sweep_cols(a, false, false, c->stream);
+    qs::launch_sweep_rows(a, gv ? 4 : 2, c->stream);
+  }
+  HIPCHK(c, hipGetLastError());
+  return QS_OK;
+}
+
 static int bandlimit_buf(qs_ctx* c, float2* tr, int nslice) {
   const qs_params& p = c->p;
   const int nx = p.potNx, ny = p.potNy;
   const float k2max = bl_k2max(p);
-  if (c->bl.on) {
-    // three sweeps per chunk of slices, each slice a "probe" of the sweep kernels:
-    // the column sweep's propagator is 1/(nx ny) inside the disk k^2 < k2max
-    const size_t slice = (size_t)nx * ny;
-    const size_t per = sizeof(float2) * (size_t)nx * c->bl.kcp;
-    const int q = (int)std::max<size_t>(1, std::min<size_t>((size_t)nslice, kBandScratchBytes / per));
-    HIPCHK(c, c->blstate.ensure(per * q));
-    qs::SweepArgs a{};
-    a.nx = nx;
-    a.ny = ny;
-    a.kc = c->bl.kc;
-    a.kcp = c->bl.kcp;
-    a.band_a = c->bl.band_a;
-    a.band_b = c->bl.band_b;
-    a.ntiles = c->bl.ntiles;
-    a.potNy = ny;
-    a.twx = c->rx.tw.as<float2>();
-    a.twy = c->ry.tw.as<float2>();
-    a.px = c->blpx.as<float2>();
-    a.py = c->blpy.as<float2>();
-    a.kx2 = c->pkx2.as<float>();
-    a.ky2 = c->pky2.as<float>();
-    a.k2max = k2max;
-    a.rec_bin = -1;
-    a.state = c->blstate.as<float2>();
-    for (int s0 = 0; s0 < nslice; s0 += q) {
-      a.nprobe = std::min(q, nslice - s0);
-      a.probe = tr + (size_t)s0 * slice;
-      a.out_wave = tr + (size_t)s0 * slice;
-      qs::launch_sweep_rows(a, 3, c->stream);
-      qs::launch_sweep_cols(a, false, false, c->stream);
-      qs::launch_sweep_rows(a, 2, c->stream);
-    }
-    HIPCHK(c, hipGetLastError());
-    return QS_OK;
-  }
+  if (c->bl.on) return bl_sweeps(c, tr, nslice, nullptr, nullptr, 0.f);
   HIPCHK(c, c->scratch.ensure(sizeof(float2) * (size_t)nslice * nx * ny));
   float2* sc = c->scratch.as<float2>();
   run_rowxform(c, c->ry, tr, (long)nslice * nx, -1, 1.0f);
@@ -1265,7 +1277,6 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
   const size_t slice_elems = (size_t)p.potNx * p.potNy;
   HIPCHK(c, c->state.ensure(st_bytes));
   HIPCHK(c, c->gx.ensure(st_bytes));
-  HIPCHK(c, c->work.ensure(st_bytes));
   if ((int)c->ckpt.size() < nseg + K) c->ckpt.resize(nseg + K);
   for (int j = 0; j < nseg + K; j++) HIPCHK(c, c->ckpt[j].ensure(st_bytes));  // [0,nseg): X_{jK-1}; [nseg,nseg+K): segment
   HIPCHK(c, c->ipos.ensure(sizeof(int) * 2 * nprobe));
@@ -1275,26 +1286,44 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
   HIPCHK(c, hipMemcpyAsync(c->target.p, target, sizeof(float) * nt, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, c->partial.ensure(sizeof(float) * (size_t)nprobe * c->ntiles * (qs::kMaxDet + 1)));
   HIPCHK(c, c->lossp.ensure(sizeof(float) * (size_t)nprobe * c->ntiles));
-  HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * S));
+  // dL/dt of a slice is complete once that slice's reverse row sweep has run (all
+  // probes are in the launch), so dL/dV is formed slice by slice from a one-slice
+  // dL/dt buffer; only the band limit by row transforms and transposes (grids that
+  // are not powers of two) keeps dL/dt of every slice and works on all at once.
+  const bool grad_each = grad_V && (!p.bandlimittrans || c->bl.on);
+  const int gslices = grad_V && !grad_each ? S : 1;
+  const float mm0 = 1.0F + p.v0 / 511.0F;
+  const double sigma = mm0 * wavelength(p.v0);
+  HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * gslices));
+  if (grad_V) HIPCHK(c, c->gV.ensure(sizeof(float) * slice_elems * S));
   HIPCHK(c, c->gprobe.ensure(sizeof(float2) * (size_t)p.nx * p.ny));
   HIPCHK(c, c->dsum.ensure(sizeof(double) * 4));
   // one window covering the whole slice: every gT element is stored once per slice
   const bool whole = nprobe == 1 && p.nx == p.potNx && p.ny == p.potNy;
-  if (!whole) qs::launch_cplx_fill_zero(c->gT.as<float>(), 2 * slice_elems * S, c->stream);
+  if (!whole && gslices > 1) qs::launch_cplx_fill_zero(c->gT.as<float>(), 2 * slice_elems * S, c->stream);
   qs::launch_cplx_fill_zero(c->gprobe.as<float>(), 2 * (size_t)p.nx * p.ny, c->stream);
   qs::launch_cplx_fill_zero(reinterpret_cast<float*>(c->dsum.p), 8, c->stream);
   SweepArgs a;
   sweep_args(c, a, nprobe);
   a.state = c->state.as<float2>();
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  // forward pass, keeping X_{jK-1} for every segment j >= 1
+  // forward pass, keeping X_{jK-1} for every segment j >= 1: the row sweep of slice
+  // jK-1 writes into checkpoint j and the next one reads it back (no copies);
+  // X_{S-1} ends in c->state
   qs::launch_fill_zero(a.state, st_elems, c->stream);
-  for (int s = 0; s < S; s++) {
-    a.trans = trans_base(c) + (size_t)s * slice_elems;
-    qs::launch_sweep_rows(a, s == 0 ? 0 : 1, c->stream);
-    qs::launch_sweep_cols(a, false, false, c->stream);
-    if ((s + 1) % K == 0 && s + 1 < S)
-      HIPCHK(c, hipMemcpyAsync(c->ckpt[(s + 1) / K].p, a.state, st_bytes, hipMemcpyDeviceToDevice, c->stream));
+  {
+    float2* cur = c->state.as<float2>();  // X_{s-1}
+    for (int s = 0; s < S; s++) {
+      float2* out = ((s + 1) % K == 0 && s + 1 < S) ? c->ckpt[(s + 1) / K].as<float2>() : c->state.as<float2>();
+      a.trans = trans_base(c) + (size_t)s * slice_elems;
+      a.state = out;
+      a.state_in = (s > 0 && cur != out) ? cur : nullptr;
+      qs::launch_sweep_rows(a, s == 0 ? 0 : 1, c->stream);
+      qs::launch_sweep_cols(a, false, false, c->stream);
+      cur = out;
+    }
+    a.state = c->state.as<float2>();
+    a.state_in = nullptr;
   }
   // backward, segment by segment, recomputing the forward states inside each segment
   qs::AdjArgs A{};
@@ -1307,41 +1336,48 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
   qs::launch_fill_zero(A.gx, st_elems, c->stream);
   for (int j = nseg - 1; j >= 0; j--) {
     const int s0 = j * K, s1 = std::min(S, s0 + K) - 1;
-    // recompute X_s, s in [s0, s1-1], into segment slots nseg + (s - s0)
+    // recompute X_s, s in [s0, s1-1], straight into segment slots nseg + (s - s0):
+    // each row sweep reads the previous slot (checkpoint j first) and writes the next
     SweepArgs w = a;
-    w.state = c->work.as<float2>();
-    if (s0 > 0) HIPCHK(c, hipMemcpyAsync(w.state, c->ckpt[j].p, st_bytes, hipMemcpyDeviceToDevice, c->stream));
-    else qs::launch_fill_zero(w.state, st_elems, c->stream);
+    const float2* prev = s0 > 0 ? c->ckpt[j].as<float2>() : nullptr;
     for (int s = s0; s < s1; s++) {
       w.trans = trans_base(c) + (size_t)s * slice_elems;
+      w.state = c->ckpt[nseg + (s - s0)].as<float2>();
+      w.state_in = prev;
       qs::launch_sweep_rows(w, s == 0 ? 0 : 1, c->stream);
       qs::launch_sweep_cols(w, false, false, c->stream);
-      HIPCHK(c, hipMemcpyAsync(c->ckpt[nseg + (s - s0)].p, w.state, st_bytes, hipMemcpyDeviceToDevice, c->stream));
+      prev = w.state;
     }
     for (int s = s1; s >= s0; s--) {
       A.f.trans = trans_base(c) + (size_t)s * slice_elems;
-      A.gT = c->gT.as<float>() + 2 * (size_t)s * slice_elems;
+      A.gT = c->gT.as<float>() + (gslices > 1 ? 2 * (size_t)s * slice_elems : 0);
       const bool last = s == S - 1;
       A.xprev = last ? c->state.as<float2>() : nullptr;
       qs::launch_adj_cols(A, last, c->stream);
       if (s > 0) A.xprev = (s - 1 >= s0) ? c->ckpt[nseg + (s - 1 - s0)].as<float2>() : c->ckpt[j].as<float2>();
+      if (grad_each && !whole) qs::launch_cplx_fill_zero(A.gT, 2 * slice_elems, c->stream);
       qs::launch_adj_rows(A, s == 0, c->stream);
+      if (grad_each) {
+        const float* vs = c->V.as<float>() + (size_t)s * slice_elems;
+        float* gs = c->gV.as<float>() + (size_t)s * slice_elems;
+        if (p.bandlimittrans) {
+          const int rc = bl_sweeps(c, reinterpret_cast<float2*>(A.gT), 1, vs, gs, (float)sigma);
+          if (rc != QS_OK) return rc;
+        } else {
+          qs::launch_grad_v(reinterpret_cast<const float2*>(A.gT), vs, gs, slice_elems, sigma, c->stream);
+        }
+      }
     }
   }
   HIPCHK(c, hipGetLastError());
   // loss = sum over band pixels (partials) + sum of target^2 outside the band
   qs::launch_target_outside_band(c->target.as<float>(), nprobe, p.nx, p.ny, c->band_a, c->band_b,
                                  c->dsum.as<double>() + 2, c->stream);
-  // dL/dV: band limit (self-adjoint) of dL/dt, then sigma Im(g conj(u))
-  if (grad_V) {
-    if (p.bandlimittrans) {
-      int rc = bandlimit_buf(c, c->gT.as<float2>(), S);
-      if (rc != QS_OK) return rc;
-    }
-    HIPCHK(c, c->gV.ensure(sizeof(float) * slice_elems * S));
-    const float mm0 = 1.0F + p.v0 / 511.0F;
-    qs::launch_grad_v(c->gT.as<float2>(), c->V.as<float>(), c->gV.as<float>(), slice_elems * S,
-                      mm0 * wavelength(p.v0), c->stream);
+  // dL/dV of every slice at once: band limit (self-adjoint) of dL/dt, then sigma Im(g conj(u))
+  if (grad_V && !grad_each) {
+    int rc = bandlimit_buf(c, c->gT.as<float2>(), S);
+    if (rc != QS_OK) return rc;
+    qs::launch_grad_v(c->gT.as<float2>(), c->V.as<float>(), c->gV.as<float>(), slice_elems * S, sigma, c->stream);
   }
   if (grad_probe) {  // G = c * FFT(sum_p g_psi0)
     float2* g = c->gprobe.as<float2>();

@@ -24,6 +24,7 @@ struct SweepArgs {
   const int* cfg;          // nullable: frozen-phonon configuration per global probe
   long long cfg_stride;    // elements between consecutive configurations' transmission functions
   float2* state;           // [nprobe_total][nx][kcp]
+  const float2* state_in = nullptr;  // sweep A reads the previous state here (if set) and writes a.state
   float2* out_wave;        // exit waves [nprobe_total][nx][ny] (sweep A mode 2)
   const float2* twx;       // forward twiddles W_nx^m
   const float2* twy;       // forward twiddles W_ny^m
@@ -39,6 +40,9 @@ struct SweepArgs {
   float* partial;          // [nprobe][ntiles][kMaxDet+1]
   float* diff;             // nullable: fftshifted diffraction intensities [nprobe_total][nx][ny]
   float diff_scale;
+  const float* vpot = nullptr;  // sweep A mode 4: projected potential [nprobe_total][nx][ny]
+  float* gv = nullptr;          // sweep A mode 4: dL/dV out [nprobe_total][nx][ny]
+  float gv_sigma = 0.f;         // sweep A mode 4: interaction constant sigma
 };
 
 // Backward sweeps of the adjoint (reverse of SweepArgs' forward sweeps).

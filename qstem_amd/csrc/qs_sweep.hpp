@@ -57,19 +57,41 @@ __device__ __forceinline__ unsigned band_off(int iy, int band_a, int gap) {
 __device__ __forceinline__ int wave_uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
 
 // ============================================================================
+// sin and cos of a phase x (|x| well below 2^16 rad): Cody-Waite reduction by
+// pi/2 in two FMA steps (pi/2 = H + L, H = float(pi/2); x - kH is exact under the
+// fused multiply-add), then the Taylor series of sin and cos on |r| <= pi/4 to
+// degrees 9 and 10 (truncation below 2e-9) and the quadrant k mod 4.
+__device__ __forceinline__ void sincos_phase(float x, float& s, float& c) {
+  const float k = rintf(x * 0.63661977236758134f);  // 2/pi
+  float r = fmaf(-k, 1.57079637050628662f, x);       // H
+  r = fmaf(-k, -4.37113900018624283e-8f, r);         // L = pi/2 - H
+  const float r2 = r * r;
+  const float ps = r + r * r2 * fmaf(r2, fmaf(r2, fmaf(r2, 1.0f / 362880.0f, -1.0f / 5040.0f), 1.0f / 120.0f),
+                                     -1.0f / 6.0f);
+  const float pc = fmaf(r2, fmaf(r2, fmaf(r2, fmaf(r2, fmaf(r2, -1.0f / 3628800.0f, 1.0f / 40320.0f),
+                                                   -1.0f / 720.0f), 1.0f / 24.0f), -0.5f), 1.0f);
+  const int q = (int)k & 3;
+  const float s0 = (q & 1) ? pc : ps, c0 = (q & 1) ? ps : pc;
+  s = (q & 2) ? -s0 : s0;
+  c = ((q + 1) & 2) ? -c0 : c0;
+}
+
 // Sweep A: rows.
 // mode 0: input is the shared real-space probe (first slice of a fresh run)
-// mode 1: input is the band-compact state; row IFFT first
+// mode 1: input is the band-compact state (a.state_in if set, else a.state); row IFFT first
 // mode 2: exit: row IFFT of the state, write the real-space wave, no transmit
 // mode 3: input is a full real-space row of each probe (a.probe + probe*nx*NY);
 //         row FFT only, no transmit (first sweep of a band limit)
+// mode 4: as mode 2, but the row g is not stored: dL/dV = sigma Im(g conj(u)),
+//         u = exp(i sigma V), is written to a.gv (last sweep of the band-limited
+//         potential gradient)
 // Each wavefront owns a chunk of TPW*a.rw consecutive rows of one probe
 // (TPW = 64/T transforms side by side, a.rw steps; the chunk divides nx), so
 // probe, window offset and row bases are wave-uniform scalars.  Twiddles and
 // band offsets are set up once per wavefront; all loads of a step are issued
 // before its first transform.
 template <int NY, int MODE>
-__global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 ? 4 : 1) void k_sweep_rows(SweepArgs a) {
+__global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>::E <= 64 ? 4 : (NY == 4096 && MODE >= 3 ? 3 : 1)) void k_sweep_rows(SweepArgs a) {
   constexpr int E = Plan<NY>::E;
   constexpr int T = NY / E;
   constexpr int BLK = row_block<NY>();
@@ -94,6 +116,8 @@ __global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>:
   float2* slot = lds + (threadIdx.x / T) * lds_slot<NY>();
   const size_t srow0 = ((size_t)pg * a.nx + x0) * a.kcp;
   const rsrc_t rs = make_rsrc(a.state + srow0, (long)(nrows - row0) * a.kcp * 8);
+  // loads from a.state_in when given (the state of slice s-1 kept elsewhere)
+  const rsrc_t rsi = a.state_in ? make_rsrc(a.state_in + srow0, (long)(nrows - row0) * a.kcp * 8) : rs;
   const unsigned rowb = (unsigned)a.kcp * 8u;
   const int gap = NY - a.band_b - a.band_a - 1;
   // per-lane byte offset of element m within step 0's rows (kOut outside the band);
@@ -131,7 +155,7 @@ __global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>:
 #pragma unroll
       for (int m = 0; m < E; m++) {
         if ((GAP >> m) & 1) dv[m] = make_float2(0.f, 0.f);  // inside the band gap for every thread
-        else dv[m] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, BOFF ? boff[m] : bo(m), so, 0));
+        else dv[m] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rsi, BOFF ? boff[m] : bo(m), so, 0));
       }
     }
   };
@@ -139,12 +163,22 @@ __global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>:
   tw.init(a.twy, t);
   // one step: finish row i (v: state or probe row, tv: its transmission row)
   auto step = [&](int i, float2* v, const float2* tv) {
-    if constexpr (MODE == 1 || MODE == 2) fft_wave<NY, +1, kSwz<NY>, GAP>(v, slot, tw, t);
+    if constexpr (MODE == 1 || MODE == 2 || MODE == 4) fft_wave<NY, +1, kSwz<NY>, GAP>(v, slot, tw, t);
     const int xr = i * TPW + sub;
     if constexpr (MODE == 2) {
       float2* dst = a.out_wave + ((size_t)pg * a.nx + x0 + xr) * NY;
 #pragma unroll
       for (int m = 0; m < E; m++) dst[t + T * m] = v[m];
+    } else if constexpr (MODE == 4) {
+      const size_t r = ((size_t)pg * a.nx + x0 + xr) * NY;
+      const float* vp = a.vpot + r;
+      float* gp = a.gv + r;
+#pragma unroll
+      for (int m = 0; m < E; m++) {
+        float sn, cs;
+        sincos_phase(vp[t + T * m] * a.gv_sigma, sn, cs);
+        gp[t + T * m] = a.gv_sigma * (v[m].y * cs - v[m].x * sn);
+      }
     } else {
       if constexpr (MODE <= 1) {
 #pragma unroll
@@ -486,7 +520,8 @@ void launch_rows_mode(const SweepArgs& a0, int mode, hipStream_t st) {
   if (mode == 0) hipLaunchKernelGGL((k_sweep_rows<N, 0>), grid, dim3(row_block<N>()), 0, st, a);
   else if (mode == 1) hipLaunchKernelGGL((k_sweep_rows<N, 1>), grid, dim3(row_block<N>()), 0, st, a);
   else if (mode == 2) hipLaunchKernelGGL((k_sweep_rows<N, 2>), grid, dim3(row_block<N>()), 0, st, a);
-  else hipLaunchKernelGGL((k_sweep_rows<N, 3>), grid, dim3(row_block<N>()), 0, st, a);
+  else if (mode == 3) hipLaunchKernelGGL((k_sweep_rows<N, 3>), grid, dim3(row_block<N>()), 0, st, a);
+  else hipLaunchKernelGGL((k_sweep_rows<N, 4>), grid, dim3(row_block<N>()), 0, st, a);
 }
 
 template <int N>
