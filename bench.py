@@ -297,10 +297,10 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches only")
     ap.add_argument("--tem-slices", type=int, default=200, help="slices of the tem workload")
-    ap.add_argument("--workload", default="batched", choices=["batched", "uneven", "tem", "copy"],
+    ap.add_argument("--workload", default="batched", choices=["batched", "uneven", "tem", "copy", "copyrate"],
                     help="batched: the headline line; uneven: frozen-phonon uneven batch; tem: 4096^2 plane wave "
                          "with the checkpointed adjoint (own JSON lines); copy: a plain 512 MiB device copy "
-                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters)")
+                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters); copyrate: plain device copy rates")
     ap.add_argument("--rows-per-wave", type=int, default=0, help="with --group: row-sweep steps per wavefront")
     ap.add_argument("--tiles-per-block", type=int, default=0, help="with --group: column tiles per block")
     a = ap.parse_args()
@@ -320,6 +320,26 @@ def main():
             y.copy_(x)
         torch.cuda.synchronize()
         print(json.dumps({"workload": "copy", "bytes_per_copy": 2 ** 29}), flush=True)
+        return
+    if a.workload == "copyrate":
+        # plain device-to-device copy rate (bytes read + written per second) at 512 MiB
+        # and at the 184.5 MB of one 128-probe 512^2 state, which the Infinity Cache holds
+        import torch
+        x = torch.ones(2 ** 27, device="cuda")
+        y = torch.empty_like(x)
+        rates = {}
+        for nbytes in (2 ** 29, 128 * 512 * 352 * 8):
+            xs, ys = x[: nbytes // 4], y[: nbytes // 4]
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            for _ in range(3):
+                ys.copy_(xs)
+            e0.record()
+            for _ in range(20):
+                ys.copy_(xs)
+            e1.record()
+            torch.cuda.synchronize()
+            rates[str(nbytes)] = round(2 * nbytes * 20 / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+        print(json.dumps({"workload": "copyrate", "copy_GBps_read_plus_write": rates}), flush=True)
         return
     if a.workload == "tem":
         print(json.dumps(tem_workload(nslice=a.tem_slices)), flush=True)
