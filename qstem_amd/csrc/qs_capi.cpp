@@ -4,7 +4,10 @@
 // single-precision lookup tables exactly as the reference derives them
 // (k^2 tables and band limit of propagate_slow, detector k^2 limits of
 // readFile), allocate device buffers, and sequence the sweeps.  All
-// per-pixel work runs in the kernels of qs_kernels.hip.
+// per-pixel work runs in the kernels: the slice sweeps and their reverse
+// (qs_sweep.hpp, instantiated per size by qs_sizes.hip), set-up and probe
+// (qs_setup.hip), potential builder (qs_potential.hip) and adjoint helpers
+// (qs_adjoint.hip).
 #include <hip/hip_runtime.h>
 
 #include <atomic>
