@@ -190,9 +190,16 @@ def tem_workload(nslice=200, ckpt=14):
             ad.append(time.perf_counter() - t0)
             adm = ms.last_run_ms
         _, peak_adj = device_memory(reset_peak=True)
+        # the same adjoint in its two halves (forward pass kept, then the reverse pass)
+        ms.forward_save(ipos, target, ckpt_every=ckpt)
+        fsm = ms.last_run_ms
+        ms.adjoint_saved(want_grad_V=True, want_grad_probe=False)
+        rvm = ms.last_run_ms
     f_ms = min(fw)
     return {"workload": "TEM plane wave 4096^2, perovskite 100x100 cells, %d slices, 300 kV" % nslice,
             "forward_ms": round(f_ms, 2), "adjoint_ms": round(adm, 2), "adjoint_over_forward": round(adm / f_ms, 3),
+            "forward_save_ms": round(fsm, 2), "reverse_ms": round(rvm, 2),
+            "reverse_over_forward": round(rvm / f_ms, 3),
             "adjoint_wall_s": round(min(ad), 3), "checkpoint_every": ckpt, "loss": out["loss"],
             "exit_norm": float(np.sum(np.abs(ex[0]) ** 2) / (n * n)),
             "peak_device_GB": {"potential_setup": round(peak_setup / 1e9, 2), "forward": round(peak_fwd / 1e9, 2),

@@ -212,6 +212,18 @@ int qs_run_groups(const qs_group *groups, int ngroup, const int *bin_of_slice, i
  * reference counterpart (QSTEM has no adjoint). */
 int qs_adjoint(qs_ctx *ctx, const int *ipos, int nprobe, const float *target, int ckpt_every, float *loss,
                float *grad_V, float *grad_probe);
+
+/* qs_adjoint in two halves, for callers that evaluate the loss first and ask for
+ * gradients later (a torch.autograd.Function's forward and backward).
+ * qs_forward_save: the forward pass of qs_adjoint with its checkpoints, and the
+ * loss; the context keeps them.  qs_adjoint_saved: the reverse pass from what the
+ * last qs_forward_save kept (grad_V, grad_probe as in qs_adjoint); it consumes
+ * them, and fails with QS_ERR_STATE when none are kept or when the potential,
+ * probe, optics or configurations changed since (qs_run_stem in between is
+ * allowed).  qs_last_run_ms reports each half's device time.  No reference
+ * counterpart. */
+int qs_forward_save(qs_ctx *ctx, const int *ipos, int nprobe, const float *target, int ckpt_every, float *loss);
+int qs_adjoint_saved(qs_ctx *ctx, float *grad_V, float *grad_probe);
 /* dL/d(x, y, z) of the atoms of the last qs_build_potential (same atoms, same pp), from the
  * grad_V of the last qs_adjoint; z is 0 (the potential samples z by nearest slice). */
 int qs_atom_gradients(qs_ctx *ctx, const qs_atom *atoms, int natom, const qs_pot_params *pp, float *grad_xyz);

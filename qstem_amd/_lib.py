@@ -93,6 +93,8 @@ SIGNATURES = {
     "qs_share_trans": (_I, [_P, _P]),
     "qs_run_groups": (_I, [ctypes.POINTER(QsGroup), _I, _IP, _I, _I]),
     "qs_adjoint": (_I, [_P, _IP, _I, _FP, _I, _FP, _FP, _FP]),
+    "qs_forward_save": (_I, [_P, _IP, _I, _FP, _I, _FP]),
+    "qs_adjoint_saved": (_I, [_P, _FP, _FP]),
     "qs_atom_gradients": (_I, [_P, ctypes.POINTER(QsAtom), _I, ctypes.POINTER(QsPotParams), _FP]),
     "qs_set_batch": (_I, [_P, _I]),
     "qs_get_batch": (_I, [_P]),

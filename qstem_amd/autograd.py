@@ -5,9 +5,11 @@
 
 V is the projected potential [nslice][potNx][potNy] (any device; it is
 moved through host memory to the library, which owns its device buffers),
-df0 and Cs are 0-d tensors in Angstrom.  The forward pass evaluates the loss
-and the gradients in one adjoint call (the backward pass then only scales
-them), which costs one forward plus one reverse sweep of the multislice.
+df0 and Cs are 0-d tensors in Angstrom.  The forward pass runs the forward
+half of the adjoint (qs_forward_save: the loss, checkpoints kept by the
+context); the backward pass runs the reverse half (qs_adjoint_saved) for the
+inputs that need gradients.  Between the two, the context may run other
+qs_run_stem calls but must not change its potential, probe or optics.
 """
 import numpy as np
 import torch
@@ -27,18 +29,24 @@ class MultisliceLoss(torch.autograd.Function):
         ms.set_potential(V.detach().to("cpu", torch.float32).numpy())
         ms.make_probe()
         tgt = target.detach().to("cpu", torch.float32).numpy() if torch.is_tensor(target) else target
-        out = ms.adjoint(np.asarray(ipos), tgt, want_grad_V=True, want_grad_probe=True)
-        ctx.save_for_backward(torch.from_numpy(out["grad_V"]).to(V.device),
-                              torch.tensor(out["grad_df0"], dtype=torch.float64),
-                              torch.tensor(out["grad_Cs"], dtype=torch.float64))
-        return torch.tensor(out["loss"], dtype=torch.float64, device=V.device)
+        loss = ms.forward_save(np.asarray(ipos), tgt)
+        ctx.ms = ms
+        ctx.v_device, ctx.v_dtype = V.device, V.dtype
+        return torch.tensor(loss, dtype=torch.float64, device=V.device)
 
     @staticmethod
     def backward(ctx, grad_out):
-        gV, gdf, gcs = ctx.saved_tensors
-        go = grad_out.to(torch.float64)
-        return (gV * go.to(gV.device, gV.dtype), (gdf * go.cpu()).to(torch.float64),
-                (gcs * go.cpu()).to(torch.float64), None, None, None)
+        need_v, need_df, need_cs = ctx.needs_input_grad[:3]
+        out = ctx.ms.adjoint_saved(want_grad_V=need_v, want_grad_probe=need_df or need_cs)
+        go = grad_out.detach().to("cpu", torch.float64)
+        gV = gdf = gcs = None
+        if need_v:
+            gV = torch.from_numpy(out["grad_V"]).to(ctx.v_device, ctx.v_dtype) * go.to(ctx.v_device, ctx.v_dtype)
+        if need_df:
+            gdf = torch.tensor(out["grad_df0"], dtype=torch.float64) * go
+        if need_cs:
+            gcs = torch.tensor(out["grad_Cs"], dtype=torch.float64) * go
+        return gV, gdf, gcs, None, None, None
 
 
 def multislice_loss(V, df0, Cs, ms, ipos, target):

@@ -171,7 +171,13 @@ struct qs_ctx {
   bool have_probe = false;
   // adjoint
   std::vector<DevBuf> ckpt;
-  DevBuf gx, target, lossp, gT, gprobe, gV, dsum;
+  DevBuf gx, target, lossp, gT, gprobe, gV, dsum, adj_ipos;
+  // forward pass kept by qs_forward_save for qs_adjoint_saved (checkpoints, the
+  // adjoint seed in gx, the loss); cleared by every change of potential, probe or
+  // optics (with the graph) and by the reverse pass itself
+  bool adj_saved = false;
+  int adj_nprobe = 0, adj_K = 0;
+  float adj_loss = 0.f;
   int state_probes = 0;
   bool state_valid = false;
   // optional per-launch timing of the two sweeps (HIP events on the context stream)
@@ -405,10 +411,13 @@ int qs_create(const qs_params* p, qs_ctx** out) {
   return QS_OK;
 }
 
+// Called wherever the potential, probe, optics or launch set-up change: drops the
+// captured graph and the forward pass saved for the adjoint.
 static void drop_graph(qs_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   c->gexec = nullptr;
   c->gkey.clear();
+  c->adj_saved = false;
 }
 
 void qs_destroy(qs_ctx* c) {
@@ -423,7 +432,7 @@ void qs_destroy(qs_ctx* c) {
                     &c->lx.tw, &c->lx.chirp_f, &c->lx.vhat_f, &c->lx.chirp_b, &c->lx.vhat_b,
                     &c->lz.tw, &c->lz.chirp_f, &c->lz.vhat_f, &c->lz.chirp_b, &c->lz.vhat_b,
                     &c->gx, &c->target, &c->lossp, &c->gT, &c->gprobe, &c->gV, &c->dsum,
-                    &c->blpx, &c->blpy, &c->blstate};
+                    &c->blpx, &c->blpy, &c->blstate, &c->adj_ipos};
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : c->luts) b.release();
   for (DevBuf& b : c->ckpt) b.release();
@@ -1254,24 +1263,46 @@ static void sweep_args(qs_ctx* c, SweepArgs& a, int nprobe) {
   a.partial = c->partial.as<float>();
 }
 
-int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int ckpt_every, float* loss_out,
-               float* grad_V, float* grad_probe) {
-  if (!c || !ipos || nprobe < 1 || !target) return fail(c, QS_ERR_ARG, "qs_adjoint: bad arguments");
+// Reverse-pass arguments shared by the two halves of the adjoint.
+static qs::AdjArgs adj_args(qs_ctx* c, int nprobe) {
+  const qs_params& p = c->p;
+  SweepArgs a;
+  sweep_args(c, a, nprobe);
+  a.ipos = c->adj_ipos.as<int2>();
+  a.state = c->state.as<float2>();
+  qs::AdjArgs A{};
+  A.f = a;
+  A.gx = c->gx.as<float2>();
+  A.target = c->target.as<float>();
+  A.loss_partial = c->lossp.as<float>();
+  A.gprobe = c->gprobe.as<float>();
+  // a single window: one writer per pixel and launch; covering the whole slice:
+  // every gT element is stored once per slice
+  const bool whole = nprobe == 1 && p.nx == p.potNx && p.ny == p.potNy;
+  A.single_writer = nprobe == 1 ? (whole ? 2 : 1) : 0;
+  return A;
+}
+
+// First half of the adjoint: the forward pass with checkpoints (X_{jK-1} in
+// ckpt[j], written in place by the row sweep that produces it; X_{S-1} in
+// c->state), the reverse column sweep of the last slice (loss partials and the
+// adjoint seed in gx) and the loss.
+static int adj_forward(qs_ctx* c, const char* who, const int* ipos, int nprobe, const float* target, int ckpt_every,
+                       float* loss_out) {
+  if (!c) return QS_ERR_ARG;
+  c->adj_saved = false;
+  if (!ipos || nprobe < 1 || !target) return fail(c, QS_ERR_ARG, std::string(who) + ": bad arguments");
   const qs_params& p = c->p;
   const int S = trans_slices_of(c);
-  if (S < 1) return fail(c, QS_ERR_STATE, "qs_adjoint: no transmission functions");
+  if (S < 1) return fail(c, QS_ERR_STATE, std::string(who) + ": no transmission functions");
   if (nconfig_of(c) > 1 || c->trans_src)
-    return fail(c, QS_ERR_UNSUPPORTED, "qs_adjoint: needs one configuration of the context's own transmission function");
-  if (!c->have_probe) return fail(c, QS_ERR_STATE, "qs_adjoint: qs_make_probe not called");
-  if (grad_V && !c->v_valid) return fail(c, QS_ERR_STATE, "qs_adjoint: grad_V needs qs_set_potential/qs_build_potential");
-  if (grad_probe && c->custom_wave)
-    return fail(c, QS_ERR_UNSUPPORTED, "qs_adjoint: probe gradients need the probe of qs_make_probe");
-  if (grad_probe && (p.gaussFlag || p.aAIS > 0))
-    return fail(c, QS_ERR_UNSUPPORTED, "qs_adjoint: probe gradients need gaussian: no and no AIS aperture");
+    return fail(c, QS_ERR_UNSUPPORTED,
+                std::string(who) + ": needs one configuration of the context's own transmission function");
+  if (!c->have_probe) return fail(c, QS_ERR_STATE, std::string(who) + ": qs_make_probe not called");
   for (int i = 0; i < nprobe; i++) {
     const int x = ipos[2 * i], y = ipos[2 * i + 1];
     if (x < 0 || y < 0 || x + p.nx > p.potNx || y + p.ny > p.potNy)
-      return fail(c, QS_ERR_ARG, "qs_adjoint: window offset outside the transmission array");
+      return fail(c, QS_ERR_ARG, std::string(who) + ": window offset outside the transmission array");
   }
   const int K = ckpt_every > 0 ? ckpt_every : (int)ceil(sqrt((double)S));
   const int nseg = (S + K - 1) / K;
@@ -1282,13 +1313,75 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
   HIPCHK(c, c->gx.ensure(st_bytes));
   if ((int)c->ckpt.size() < nseg + K) c->ckpt.resize(nseg + K);
   for (int j = 0; j < nseg + K; j++) HIPCHK(c, c->ckpt[j].ensure(st_bytes));  // [0,nseg): X_{jK-1}; [nseg,nseg+K): segment
-  HIPCHK(c, c->ipos.ensure(sizeof(int) * 2 * nprobe));
-  HIPCHK(c, hipMemcpyAsync(c->ipos.p, ipos, sizeof(int) * 2 * nprobe, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, c->adj_ipos.ensure(sizeof(int) * 2 * nprobe));
+  HIPCHK(c, hipMemcpyAsync(c->adj_ipos.p, ipos, sizeof(int) * 2 * nprobe, hipMemcpyHostToDevice, c->stream));
   const size_t nt = (size_t)nprobe * p.nx * p.ny;
   HIPCHK(c, c->target.ensure(sizeof(float) * nt));
   HIPCHK(c, hipMemcpyAsync(c->target.p, target, sizeof(float) * nt, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, c->partial.ensure(sizeof(float) * (size_t)nprobe * c->ntiles * (qs::kMaxDet + 1)));
   HIPCHK(c, c->lossp.ensure(sizeof(float) * (size_t)nprobe * c->ntiles));
+  HIPCHK(c, c->dsum.ensure(sizeof(double) * 4));
+  qs::launch_cplx_fill_zero(reinterpret_cast<float*>(c->dsum.p), 8, c->stream);
+  qs::AdjArgs A = adj_args(c, nprobe);
+  SweepArgs a = A.f;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  qs::launch_fill_zero(a.state, st_elems, c->stream);
+  float2* cur = c->state.as<float2>();  // X_{s-1}
+  for (int s = 0; s < S; s++) {
+    float2* out = ((s + 1) % K == 0 && s + 1 < S) ? c->ckpt[(s + 1) / K].as<float2>() : c->state.as<float2>();
+    a.trans = trans_base(c) + (size_t)s * slice_elems;
+    a.state = out;
+    a.state_in = (s > 0 && cur != out) ? cur : nullptr;
+    qs::launch_sweep_rows(a, s == 0 ? 0 : 1, c->stream);
+    qs::launch_sweep_cols(a, false, false, c->stream);
+    cur = out;
+  }
+  // reverse column sweep of the last slice: loss partials, adjoint seed
+  qs::launch_fill_zero(A.gx, st_elems, c->stream);
+  A.f.trans = trans_base(c) + (size_t)(S - 1) * slice_elems;
+  A.xprev = c->state.as<float2>();
+  qs::launch_adj_cols(A, true, c->stream);
+  // loss = sum over band pixels (partials) + sum of target^2 outside the band
+  qs::launch_target_outside_band(c->target.as<float>(), nprobe, p.nx, p.ny, c->band_a, c->band_b,
+                                 c->dsum.as<double>() + 2, c->stream);
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, c->stream));
+  std::vector<float> lp((size_t)nprobe * c->ntiles);
+  double ds[4];
+  HIPCHK(c, hipMemcpyAsync(lp.data(), c->lossp.p, sizeof(float) * lp.size(), hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipMemcpyAsync(ds, c->dsum.p, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  double loss = ds[2];
+  for (float v : lp) loss += v;
+  if (loss_out) *loss_out = (float)loss;
+  c->adj_loss = (float)loss;
+  c->adj_nprobe = nprobe;
+  c->adj_K = K;
+  c->adj_saved = true;
+  c->state_valid = false;
+  return QS_OK;
+}
+
+// Second half: segment by segment, recompute the forward states inside the
+// segment from its checkpoint, then the reverse sweeps of its slices (the last
+// slice's column sweep ran in the first half); dL/dV and dL/d(df0, Cs).
+static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_probe) {
+  if (!c) return QS_ERR_ARG;
+  if (!c->adj_saved)
+    return fail(c, QS_ERR_STATE, std::string(who) + ": no saved forward pass (qs_forward_save), or the potential, "
+                                                     "probe or optics changed since");
+  const qs_params& p = c->p;
+  if (grad_V && !c->v_valid)
+    return fail(c, QS_ERR_STATE, std::string(who) + ": grad_V needs qs_set_potential/qs_build_potential");
+  if (grad_probe && c->custom_wave)
+    return fail(c, QS_ERR_UNSUPPORTED, std::string(who) + ": probe gradients need the probe of qs_make_probe");
+  if (grad_probe && (p.gaussFlag || p.aAIS > 0))
+    return fail(c, QS_ERR_UNSUPPORTED, std::string(who) + ": probe gradients need gaussian: no and no AIS aperture");
+  c->adj_saved = false;  // the reverse pass consumes the seed and the segment slots
+  const int S = trans_slices_of(c), nprobe = c->adj_nprobe, K = c->adj_K;
+  const int nseg = (S + K - 1) / K;
+  const size_t slice_elems = (size_t)p.potNx * p.potNy;
   // dL/dt of a slice is complete once that slice's reverse row sweep has run (all
   // probes are in the launch), so dL/dV is formed slice by slice from a one-slice
   // dL/dt buffer; only the band limit by row transforms and transposes (grids that
@@ -1300,48 +1393,16 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
   HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * gslices));
   if (grad_V) HIPCHK(c, c->gV.ensure(sizeof(float) * slice_elems * S));
   HIPCHK(c, c->gprobe.ensure(sizeof(float2) * (size_t)p.nx * p.ny));
-  HIPCHK(c, c->dsum.ensure(sizeof(double) * 4));
-  // one window covering the whole slice: every gT element is stored once per slice
-  const bool whole = nprobe == 1 && p.nx == p.potNx && p.ny == p.potNy;
+  qs::AdjArgs A = adj_args(c, nprobe);
+  const bool whole = A.single_writer == 2;
+  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   if (!whole && gslices > 1) qs::launch_cplx_fill_zero(c->gT.as<float>(), 2 * slice_elems * S, c->stream);
   qs::launch_cplx_fill_zero(c->gprobe.as<float>(), 2 * (size_t)p.nx * p.ny, c->stream);
-  qs::launch_cplx_fill_zero(reinterpret_cast<float*>(c->dsum.p), 8, c->stream);
-  SweepArgs a;
-  sweep_args(c, a, nprobe);
-  a.state = c->state.as<float2>();
-  HIPCHK(c, hipEventRecord(c->ev0, c->stream));
-  // forward pass, keeping X_{jK-1} for every segment j >= 1: the row sweep of slice
-  // jK-1 writes into checkpoint j and the next one reads it back (no copies);
-  // X_{S-1} ends in c->state
-  qs::launch_fill_zero(a.state, st_elems, c->stream);
-  {
-    float2* cur = c->state.as<float2>();  // X_{s-1}
-    for (int s = 0; s < S; s++) {
-      float2* out = ((s + 1) % K == 0 && s + 1 < S) ? c->ckpt[(s + 1) / K].as<float2>() : c->state.as<float2>();
-      a.trans = trans_base(c) + (size_t)s * slice_elems;
-      a.state = out;
-      a.state_in = (s > 0 && cur != out) ? cur : nullptr;
-      qs::launch_sweep_rows(a, s == 0 ? 0 : 1, c->stream);
-      qs::launch_sweep_cols(a, false, false, c->stream);
-      cur = out;
-    }
-    a.state = c->state.as<float2>();
-    a.state_in = nullptr;
-  }
-  // backward, segment by segment, recomputing the forward states inside each segment
-  qs::AdjArgs A{};
-  A.f = a;
-  A.gx = c->gx.as<float2>();
-  A.target = c->target.as<float>();
-  A.loss_partial = c->lossp.as<float>();
-  A.gprobe = c->gprobe.as<float>();
-  A.single_writer = nprobe == 1 ? (whole ? 2 : 1) : 0;  // a single window: one writer per pixel and launch
-  qs::launch_fill_zero(A.gx, st_elems, c->stream);
   for (int j = nseg - 1; j >= 0; j--) {
     const int s0 = j * K, s1 = std::min(S, s0 + K) - 1;
     // recompute X_s, s in [s0, s1-1], straight into segment slots nseg + (s - s0):
     // each row sweep reads the previous slot (checkpoint j first) and writes the next
-    SweepArgs w = a;
+    SweepArgs w = A.f;
     const float2* prev = s0 > 0 ? c->ckpt[j].as<float2>() : nullptr;
     for (int s = s0; s < s1; s++) {
       w.trans = trans_base(c) + (size_t)s * slice_elems;
@@ -1354,9 +1415,8 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
     for (int s = s1; s >= s0; s--) {
       A.f.trans = trans_base(c) + (size_t)s * slice_elems;
       A.gT = c->gT.as<float>() + (gslices > 1 ? 2 * (size_t)s * slice_elems : 0);
-      const bool last = s == S - 1;
-      A.xprev = last ? c->state.as<float2>() : nullptr;
-      qs::launch_adj_cols(A, last, c->stream);
+      A.xprev = nullptr;
+      if (s < S - 1) qs::launch_adj_cols(A, false, c->stream);
       if (s > 0) A.xprev = (s - 1 >= s0) ? c->ckpt[nseg + (s - 1 - s0)].as<float2>() : c->ckpt[j].as<float2>();
       if (grad_each && !whole) qs::launch_cplx_fill_zero(A.gT, 2 * slice_elems, c->stream);
       qs::launch_adj_rows(A, s == 0, c->stream);
@@ -1373,9 +1433,6 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
     }
   }
   HIPCHK(c, hipGetLastError());
-  // loss = sum over band pixels (partials) + sum of target^2 outside the band
-  qs::launch_target_outside_band(c->target.as<float>(), nprobe, p.nx, p.ny, c->band_a, c->band_b,
-                                 c->dsum.as<double>() + 2, c->stream);
   // dL/dV of every slice at once: band limit (self-adjoint) of dL/dt, then sigma Im(g conj(u))
   if (grad_V && !grad_each) {
     int rc = bandlimit_buf(c, c->gT.as<float2>(), S);
@@ -1394,22 +1451,41 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
   }
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
-  std::vector<float> lp((size_t)nprobe * c->ntiles);
   double ds[4];
-  HIPCHK(c, hipMemcpyAsync(lp.data(), c->lossp.p, sizeof(float) * lp.size(), hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipMemcpyAsync(ds, c->dsum.p, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
   if (grad_V)
     HIPCHK(c, hipMemcpyAsync(grad_V, c->gV.p, sizeof(float) * slice_elems * S, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
-  double loss = ds[2];
-  for (float v : lp) loss += v;
-  if (loss_out) *loss_out = (float)loss;
   if (grad_probe) {
     grad_probe[0] = (float)ds[0];
     grad_probe[1] = (float)ds[1];
   }
-  c->state_valid = false;
+  return QS_OK;
+}
+
+int qs_forward_save(qs_ctx* c, const int* ipos, int nprobe, const float* target, int ckpt_every, float* loss) {
+  return adj_forward(c, "qs_forward_save", ipos, nprobe, target, ckpt_every, loss);
+}
+
+int qs_adjoint_saved(qs_ctx* c, float* grad_V, float* grad_probe) {
+  return adj_backward(c, "qs_adjoint_saved", grad_V, grad_probe);
+}
+
+int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int ckpt_every, float* loss_out,
+               float* grad_V, float* grad_probe) {
+  if (!c) return QS_ERR_ARG;
+  if (grad_V && !c->v_valid) return fail(c, QS_ERR_STATE, "qs_adjoint: grad_V needs qs_set_potential/qs_build_potential");
+  if (grad_probe && c->custom_wave)
+    return fail(c, QS_ERR_UNSUPPORTED, "qs_adjoint: probe gradients need the probe of qs_make_probe");
+  if (grad_probe && (c->p.gaussFlag || c->p.aAIS > 0))
+    return fail(c, QS_ERR_UNSUPPORTED, "qs_adjoint: probe gradients need gaussian: no and no AIS aperture");
+  int rc = adj_forward(c, "qs_adjoint", ipos, nprobe, target, ckpt_every, loss_out);
+  if (rc != QS_OK) return rc;
+  const float fwd_ms = c->last_ms;
+  rc = adj_backward(c, "qs_adjoint", grad_V, grad_probe);
+  if (rc != QS_OK) return rc;
+  c->last_ms += fwd_ms;
   return QS_OK;
 }
 

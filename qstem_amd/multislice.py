@@ -259,6 +259,29 @@ class Multislice:
             out["grad_df0"], out["grad_Cs"] = float(gp[0]), float(gp[1])
         return out
 
+    def forward_save(self, ipos: np.ndarray, target: np.ndarray, ckpt_every: int = 0) -> float:
+        """Loss of qs_adjoint from its forward pass alone (qs_forward_save); the context
+        keeps the checkpoints for a later adjoint_saved."""
+        ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        assert target.shape == (ipos.shape[0], self.cfg.nx, self.cfg.ny), target.shape
+        loss = np.zeros(1, np.float32)
+        self._check(self._lib.qs_forward_save(self._h, _iptr(ipos), ipos.shape[0], _fptr(target), int(ckpt_every),
+                                              _fptr(loss)), "qs_forward_save")
+        return float(loss[0])
+
+    def adjoint_saved(self, want_grad_V: bool = True, want_grad_probe: bool = True) -> dict:
+        """Gradients from the forward pass kept by the last forward_save (qs_adjoint_saved)."""
+        gV = np.empty((self._nslice, self.cfg.potNx, self.cfg.potNy), np.float32) if want_grad_V else None
+        gp = np.zeros(2, np.float32) if want_grad_probe else None
+        self._check(self._lib.qs_adjoint_saved(self._h, _fptr(gV), _fptr(gp)), "qs_adjoint_saved")
+        out = {}
+        if gV is not None:
+            out["grad_V"] = gV
+        if gp is not None:
+            out["grad_df0"], out["grad_Cs"] = float(gp[0]), float(gp[1])
+        return out
+
     def atom_gradients(self, atoms, atomRadius=5.0, potOffsetX=0.0, potOffsetY=0.0, czOffset=0.0, centerSlices=0,
                        periodicXY=0, tds=0, compat_q6=0, lut_dr_fix=0) -> np.ndarray:
         """dL/d(x, y, z) per atom from the last adjoint (same atoms as build_potential)."""

@@ -156,6 +156,39 @@ def test_torch_autograd_function(oracle):
             V2 = (Vt - step * Vt.grad).detach()
         loss2 = multislice_loss(V2, df.detach(), cs.detach(), ms, ipos, target)
         assert float(loss2) < float(loss)
+        # only V needs a gradient: the reverse pass skips the probe gradients
+        V3 = torch.tensor(V, requires_grad=True)
+        multislice_loss(V3, df.detach(), cs.detach(), ms, ipos, target).backward()
+        assert np.allclose(V3.grad.numpy(), ref["grad_V"], rtol=1e-5, atol=1e-6 * np.abs(ref["grad_V"]).max())
+
+
+def test_forward_save_then_adjoint_saved(oracle):
+    """The two halves give qs_adjoint's loss and gradients, also with a qs_run_stem in
+    between; the kept forward pass is consumed by the reverse pass and dropped when the
+    potential changes."""
+    from qstem_amd import Multislice, QstemError
+    cfg = _cfg(1, slices=5)
+    V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=9, vmax=20.0)
+    ipos = probe_positions(cfg, 3, seed=2)
+    target = _target(oracle, cfg, V, ipos)
+    with Multislice(cfg) as ms:
+        ms.set_potential(V)
+        ms.make_probe()
+        ref = ms.adjoint(ipos, target, ckpt_every=2)
+        loss = ms.forward_save(ipos, target, ckpt_every=2)
+        # same kernels in the same order; float atomics may reorder the last bits
+        assert abs(loss - ref["loss"]) <= 1e-6 * ref["loss"]
+        ms.run_stem(probe_positions(cfg, 2, seed=7), want_det=False, want_exit=True)
+        out = ms.adjoint_saved()
+        assert np.allclose(out["grad_V"], ref["grad_V"], rtol=1e-5, atol=1e-6 * np.abs(ref["grad_V"]).max())
+        assert abs(out["grad_df0"] - ref["grad_df0"]) <= 1e-5 * abs(ref["grad_df0"])
+        assert abs(out["grad_Cs"] - ref["grad_Cs"]) <= 1e-5 * abs(ref["grad_Cs"])
+        with pytest.raises(QstemError, match="no saved forward pass"):
+            ms.adjoint_saved()
+        ms.forward_save(ipos, target)
+        ms.set_potential(V)
+        with pytest.raises(QstemError, match="no saved forward pass"):
+            ms.adjoint_saved(want_grad_probe=False)
 
 
 @pytest.mark.parametrize("nx", [1024, 2048, 4096])
