@@ -278,17 +278,19 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
       for (int k = 0; k < R; k++) a[b][k] = slot[lidx<N, SWZ>(t + T * b + k * (N / R))];
   }
   if constexpr (NS > 1) {
-    // table twiddles are loaded where they are used: the empty asm hides the
-    // pointer's loop invariance, so the loads are not hoisted out of the
-    // callers' loops into registers held across the whole kernel
-    const float2* tp = tab;
-    if (!wr) asm volatile("" : "+s"(tp));
+    // table twiddles are loaded where they are used: the empty asm makes the
+    // index offset opaque, so the loads are not hoisted out of the callers'
+    // loops into registers held across the whole kernel (laundering the offset
+    // rather than the pointer keeps the table in the global address space:
+    // global loads, not flat loads that also hold up the LDS waits)
+    int z = 0;
+    if (!wr) asm volatile("" : "+s"(z));
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int jm = (t + T * b) % NS;
 #pragma unroll
       for (int k = 1; k < R; k++) {
-        const float2 w = wr ? wr[b * (R - 1) + k - 1] : tp[k * jm * (N / (NS * R))];
+        const float2 w = wr ? wr[b * (R - 1) + k - 1] : tab[z + k * jm * (N / (NS * R))];
         a[b][k] = DIR < 0 ? cmul(a[b][k], w) : cmulc(a[b][k], w);
       }
     }
