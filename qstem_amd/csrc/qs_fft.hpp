@@ -288,9 +288,21 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int jm = (t + T * b) % NS;
+      // table path: w^k of w = W_N^{jm N/(NS R)} as w^(k & ~3) * w^(k & 3), both
+      // factors from the table (R/4 - 1 + 3 loads; one rounding more than a
+      // direct table value)
+      float2 pw[R];
+      if (!wr) {
+#pragma unroll
+        for (int k = 1; k < R; k++)
+          if ((k & 3) == 0 || k < 4) pw[k] = tab[z + k * jm * (N / (NS * R))];
+#pragma unroll
+        for (int k = 5; k < R; k++)
+          if (k & 3) pw[k] = cmul(pw[k & ~3], pw[k & 3]);
+      }
 #pragma unroll
       for (int k = 1; k < R; k++) {
-        const float2 w = wr ? wr[b * (R - 1) + k - 1] : tab[z + k * jm * (N / (NS * R))];
+        const float2 w = wr ? wr[b * (R - 1) + k - 1] : pw[k];
         a[b][k] = DIR < 0 ? cmul(a[b][k], w) : cmulc(a[b][k], w);
       }
     }

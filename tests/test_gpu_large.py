@@ -56,3 +56,29 @@ def test_multislice_large(oracle, shape, pad, nprobe):
                 assert rel_l2(out["det"][b, d], det[b, d]) < TOL, (b, d)
     norm = np.sum(np.abs(ex) ** 2, axis=(1, 2)) / (nx * ny)
     assert np.max(np.abs(out["intensity"] - norm) / norm) < TOL
+
+
+@pytest.mark.parametrize("n,slices", [(1024, 64), (2048, 16)])
+def test_multislice_many_slices(oracle, n, slices):
+    """Error growth over many slices at the sizes whose transforms take their twiddles
+    from the table (products of two table values): exit wave and detector sums against
+    the FP64 oracle after 64 slices at 1024^2 and 16 at 2048^2."""
+    print("many-slice multislice", n, slices, flush=True)
+    cfg = small_config(nx=n, ny=n, pad=0, slices=slices, dx=0.02, v0=300.0)
+    V = smooth_potential(slices, cfg.potNx, cfg.potNy, seed=23, sigma_px=3.0)
+    ipos = probe_positions(cfg, 1, seed=3)
+    bins, nbins = oracle.record_bins(slices)
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        t32 = ms.get_trans()
+        psi0 = ms.make_probe()
+        out = ms.run_stem(ipos, bins=bins, nbins=nbins, want_exit=True)
+    ex, det, _ = oracle.multislice(cfg, t32.astype(np.complex128), ipos, psi0.astype(np.complex128),
+                                   bins=bins, nbins=nbins)
+    err = rel_l2(out["exit"], ex)
+    print("exit-wave rel L2", err, flush=True)
+    assert err < TOL
+    for b in range(nbins):
+        for d in range(det.shape[1]):
+            if np.any(det[b, d] != 0):
+                assert rel_l2(out["det"][b, d], det[b, d]) < TOL, (b, d)
