@@ -672,27 +672,26 @@ __global__ __launch_bounds__(row_block<NY>()) void k_adj_rows(AdjArgs A) {
   float2* slot = lds + slot_id * lds_slot<NY>();
   Twiddles<NY> tw;
   tw.init(a.twy, t);
-  float2* grow = A.gx + ((size_t)pg * a.nx + x) * a.kcp;
+  // band columns through buffer resources: offsets in the band gap are kOut, so
+  // their loads return 0 and their stores are dropped (no per-element branches)
+  const int gap = NY - a.band_b - a.band_a - 1;
+  const size_t roff = ((size_t)pg * a.nx + x) * a.kcp;
+  const rsrc_t rg = make_rsrc(A.gx + roff, (long)a.kcp * 8);
+  unsigned bo[E];
+#pragma unroll
+  for (int m = 0; m < E; m++) bo[m] = band_off(t + T * m, a.band_a, gap);
   float2 g[E], w[E];
 #pragma unroll
-  for (int m = 0; m < E; m++) {
-    const int iy = t + T * m;
-    const int c = iy <= a.band_a ? iy : (iy >= NY - a.band_b ? iy - (NY - a.band_b) + a.band_a + 1 : -1);
-    g[m] = c >= 0 ? grow[c] : make_float2(0.f, 0.f);
-  }
+  for (int m = 0; m < E; m++) g[m] = buf_load_c64(rg, bo[m]);
   fft_wave<NY, +1>(g, slot, tw, t);
   if constexpr (FIRST) {
     const float2* pr = a.probe + (size_t)x * NY;
 #pragma unroll
     for (int m = 0; m < E; m++) w[m] = pr[t + T * m];
   } else {
-    const float2* xr = A.xprev + ((size_t)pg * a.nx + x) * a.kcp;
+    const rsrc_t rx = make_rsrc(A.xprev + roff, (long)a.kcp * 8);
 #pragma unroll
-    for (int m = 0; m < E; m++) {
-      const int iy = t + T * m;
-      const int c = iy <= a.band_a ? iy : (iy >= NY - a.band_b ? iy - (NY - a.band_b) + a.band_a + 1 : -1);
-      w[m] = c >= 0 ? xr[c] : make_float2(0.f, 0.f);
-    }
+    for (int m = 0; m < E; m++) w[m] = buf_load_c64(rx, bo[m]);
     fft_wave<NY, +1>(w, slot, tw, t);
   }
   const int2 ip = a.ipos[pg];
@@ -724,11 +723,7 @@ __global__ __launch_bounds__(row_block<NY>()) void k_adj_rows(AdjArgs A) {
   } else {
     fft_wave<NY, -1>(g, slot, tw, t);
 #pragma unroll
-    for (int m = 0; m < E; m++) {
-      const int iy = t + T * m;
-      const int c = iy <= a.band_a ? iy : (iy >= NY - a.band_b ? iy - (NY - a.band_b) + a.band_a + 1 : -1);
-      if (c >= 0) grow[c] = g[m];
-    }
+    for (int m = 0; m < E; m++) buf_store_c64(g[m], rg, bo[m]);
   }
 }
 
