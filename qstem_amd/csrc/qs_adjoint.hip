@@ -132,7 +132,7 @@ __global__ __launch_bounds__(256) void k_atom_grad(const QsAtomDev* __restrict__
   const int iaz1 = iAtomZ + s.iRadZ >= s.slices ? s.slices - iAtomZ - 1 : s.iRadZ;
   if (!(iAtomZ + iaz0 < s.slices && iAtomZ + iaz1 >= 0)) active = false;
   if (active) {
-    const float* lut = luts[kind[ia]];
+    const gfloat* lut = (const gfloat*)luts[kind[ia]];
     const float zrel = atomZ / dzs;
     const double dOffsZ = (double)((float)((float)(iAtomZ + iaz0) - zrel) * (float)s.nzSub);
     const int iOffsZ0 = (int)(dOffsZ + 0.5) * s.Nr;

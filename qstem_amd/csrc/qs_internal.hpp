@@ -7,6 +7,10 @@ namespace qs {
 
 constexpr int kMaxDet = 16;
 
+// A float in device global memory: pointers read from device arrays (the LUT
+// table) are cast to it so their loads are global, not flat, loads.
+using gfloat = __attribute__((address_space(1))) float;
+
 // Arguments of both multislice sweeps (passed by value, lives in the kernarg segment).
 struct SweepArgs {
   int nx, ny;          // wave grid (powers of two)
