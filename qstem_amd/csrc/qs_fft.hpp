@@ -337,7 +337,6 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
 template <int N>
 constexpr int gap_mask() {
   constexpr int E = Plan<N>::E, T = N / E, A = N / 3;
-  if (E > 8) return 0;  // radix-16 plans: the pruned code holds more registers than it saves (measured)
   int m = 0;
   for (int j = 0; j < E && j < 31; j++)
     if (T * j > A && T * j + T - 1 < N - A) m |= 1 << j;
