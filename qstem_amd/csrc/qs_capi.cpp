@@ -411,12 +411,16 @@ int qs_create(const qs_params* p, qs_ctx** out) {
   return QS_OK;
 }
 
-// Called wherever the potential, probe, optics or launch set-up change: drops the
-// captured graph and the forward pass saved for the adjoint.
 static void drop_graph(qs_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
   c->gexec = nullptr;
   c->gkey.clear();
+}
+
+// Called wherever the potential, probe or optics change: the captured graph's
+// kernel arguments and the forward pass kept for the adjoint no longer hold.
+static void invalidate(qs_ctx* c) {
+  drop_graph(c);
   c->adj_saved = false;
 }
 
@@ -560,7 +564,7 @@ static int build_trans(qs_ctx* c, int nslice) {
 
 int qs_set_potential(qs_ctx* c, const float* V, int nslice) {
   if (!c || !V || nslice < 1) return fail(c, QS_ERR_ARG, "qs_set_potential: bad arguments");
-  drop_graph(c);  // kernel arguments baked into a captured graph may change
+  invalidate(c);  // kernel arguments baked into a captured graph may change
   const qs_params& p = c->p;
   const size_t n = (size_t)nslice * p.potNx * p.potNy;
   HIPCHK(c, c->V.ensure(sizeof(float) * n));
@@ -599,7 +603,7 @@ int qs_set_scattering_table(qs_ctx* c, const double* table) {
 
 int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_params* pp, float* V_out) {
   if (!c || !pp || natom < 0 || (natom > 0 && !atoms)) return fail(c, QS_ERR_ARG, "qs_build_potential: bad arguments");
-  drop_graph(c);  // kernel arguments baked into a captured graph may change
+  invalidate(c);  // kernel arguments baked into a captured graph may change
   if (c->scat.size() != 93 * 123) return fail(c, QS_ERR_STATE, "qs_build_potential: scattering table not set");
   const qs_params& p = c->p;
   for (int i = 0; i < natom; i++) {
@@ -711,7 +715,7 @@ int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_
 
 int qs_set_trans(qs_ctx* c, const float* trans, int nslice) {
   if (!c || !trans || nslice < 1) return fail(c, QS_ERR_ARG, "qs_set_trans: bad arguments");
-  drop_graph(c);  // kernel arguments baked into a captured graph may change
+  invalidate(c);  // kernel arguments baked into a captured graph may change
   const size_t n = (size_t)nslice * c->p.potNx * c->p.potNy;
   float2* t = nullptr;
   const int rc = trans_slot(c, nslice, "qs_set_trans", &t);
@@ -734,7 +738,7 @@ int qs_get_trans(qs_ctx* c, float* out) {
 
 int qs_make_probe(qs_ctx* c, float* out) {
   if (!c) return QS_ERR_ARG;
-  drop_graph(c);  // kernel arguments baked into a captured graph may change
+  invalidate(c);  // kernel arguments baked into a captured graph may change
   const qs_params& p = c->p;
   const int nx = p.nx, ny = p.ny;
   ProbeArgs a{};
@@ -807,7 +811,7 @@ int qs_make_probe(qs_ctx* c, float* out) {
 
 int qs_set_optics(qs_ctx* c, const qs_params* p) {
   if (!c || !p) return fail(c, QS_ERR_ARG, "qs_set_optics: bad arguments");
-  drop_graph(c);  // kernel arguments baked into a captured graph may change
+  invalidate(c);  // kernel arguments baked into a captured graph may change
   qs_params& q = c->p;
   q.Cs = p->Cs; q.C5 = p->C5; q.df0 = p->df0; q.astigMag = p->astigMag; q.astigAngle = p->astigAngle;
   q.a33 = p->a33; q.a31 = p->a31; q.a44 = p->a44; q.a42 = p->a42; q.a55 = p->a55;
@@ -823,7 +827,7 @@ int qs_set_optics(qs_ctx* c, const qs_params* p) {
 
 int qs_set_incident_wave(qs_ctx* c, const float* psi) {
   if (!c || !psi) return fail(c, QS_ERR_ARG, "qs_set_incident_wave: bad arguments");
-  drop_graph(c);
+  invalidate(c);
   const size_t n = (size_t)c->p.nx * c->p.ny;
   HIPCHK(c, c->probe.ensure(sizeof(float2) * n));
   HIPCHK(c, hipMemcpyAsync(c->probe.p, psi, sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
@@ -876,7 +880,7 @@ int qs_set_graph(qs_ctx* c, int on) {
 int qs_set_configs(qs_ctx* c, int nconfig) {
   if (!c || nconfig < 1) return fail(c, QS_ERR_ARG, "qs_set_configs: bad arguments");
   if (c->trans_src) return fail(c, QS_ERR_STATE, "qs_set_configs: this context uses another's transmission functions");
-  drop_graph(c);
+  invalidate(c);
   c->nconfig = nconfig;
   c->cur_config = 0;
   c->trans_slices = 0;  // every configuration is (re)loaded, configuration 0 first
@@ -893,7 +897,7 @@ int qs_select_config(qs_ctx* c, int config) {
 
 int qs_share_trans(qs_ctx* dst, qs_ctx* src) {
   if (!dst || dst == src) return fail(dst, QS_ERR_ARG, "qs_share_trans: bad arguments");
-  drop_graph(dst);
+  invalidate(dst);
   if (!src) {
     dst->trans_src = nullptr;
     return QS_OK;

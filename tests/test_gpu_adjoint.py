@@ -178,6 +178,7 @@ def test_forward_save_then_adjoint_saved(oracle):
         loss = ms.forward_save(ipos, target, ckpt_every=2)
         # same kernels in the same order; float atomics may reorder the last bits
         assert abs(loss - ref["loss"]) <= 1e-6 * ref["loss"]
+        ms.set_graph(True)  # a captured run in between leaves the kept pass alone
         ms.run_stem(probe_positions(cfg, 2, seed=7), want_det=False, want_exit=True)
         out = ms.adjoint_saved()
         assert np.allclose(out["grad_V"], ref["grad_V"], rtol=1e-5, atol=1e-6 * np.abs(ref["grad_V"]).max())
