@@ -29,7 +29,8 @@ def test_probe_matches_oracle(oracle, n):
     assert rel_l2(g, o) < 2e-5
 
 
-@pytest.mark.parametrize("pot", [(160, 160), (128, 128), (200, 136), (256, 128), (128, 512), (1024, 2048)])
+@pytest.mark.parametrize("pot", [(160, 160), (128, 128), (200, 136), (256, 128), (128, 512), (1024, 2048),
+                                 (2048, 1024)])
 def test_transmission_matches_oracle(oracle, pot):
     """Power-of-two grids take the band-compact sweeps, the others row transforms
     (Bluestein where needed) and transposes."""
