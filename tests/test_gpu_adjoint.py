@@ -192,7 +192,7 @@ def test_forward_save_then_adjoint_saved(oracle):
             ms.adjoint_saved(want_grad_probe=False)
 
 
-@pytest.mark.parametrize("nx", [1024, 2048, 4096])
+@pytest.mark.parametrize("nx", [512, 1024, 2048, 4096])
 def test_adjoint_plane_wave_large(oracle, nx):
     """Plane-wave (TEM) adjoint at the large transform sizes: loss and a directional
     dL/dV against FP64 central differences of the oracle (2 slices, checkpoint every slice)."""
@@ -235,4 +235,5 @@ def test_adjoint_plane_wave_large(oracle, nx):
     D = smooth_potential(2, nx, nx, seed=62, vmax=1.0, sigma_px=4.0).astype(np.float64)
     fd = (L(Vd + h * D) - L(Vd - h * D)) / (2 * h)
     ad = float(np.sum(g * D))
+    print("random direction: adjoint", ad, "central difference", fd, flush=True)
     assert abs(ad - fd) <= 5e-3 * abs(fd), (ad, fd)
