@@ -222,9 +222,31 @@ __device__ __forceinline__ void xform_sync() {
   else __syncthreads();
 }
 
+// W_N^m = exp(-2 pi i m/N), 0 <= m < N, computed instead of loaded: m/N turns is
+// reduced exactly to the nearest quarter turn q/4 and a remainder a with |a| <= pi/4
+// (a = (4m - qN) * pi/(2N): an exact integer times one constant, one rounding),
+// whose sine and cosine come from their Taylor series (degrees 9 and 10, truncation
+// below 2e-9); the quarter turn permutes and negates them.
+template <int N>
+__device__ __forceinline__ float2 twiddle_at(int m) {
+  const int q = (4 * m + N / 2) / N;
+  const float a = (float)(4 * m - q * N) * (float)(3.14159265358979323846 / (2.0 * N));
+  const float a2 = a * a;
+  const float sa = a + a * a2 * fmaf(a2, fmaf(a2, fmaf(a2, 1.0f / 362880.0f, -1.0f / 5040.0f), 1.0f / 120.0f),
+                                     -1.0f / 6.0f);
+  const float ca = fmaf(a2, fmaf(a2, fmaf(a2, fmaf(a2, fmaf(a2, -1.0f / 3628800.0f, 1.0f / 40320.0f),
+                                                   -1.0f / 720.0f), 1.0f / 24.0f), -0.5f), 1.0f);
+  // (cos, sin) of q pi/2 + a
+  const int qq = q & 3;
+  const float c = qq == 0 ? ca : (qq == 1 ? -sa : (qq == 2 ? -ca : sa));
+  const float sn = qq == 0 ? sa : (qq == 1 ? ca : (qq == 2 ? -sa : -ca));
+  return make_float2(c, -sn);
+}
+
 // Per-thread twiddles of the twiddled passes.  For small three-pass plans they
 // are loaded once into registers (init) and reused by every transform the
-// thread performs; larger plans read the table on every pass.
+// thread performs; larger plans compute them on every pass (twiddle_at: no
+// table loads and their latency inside the passes, and no registers held).
 template <int N>
 struct Twiddles {
   using P = Plan<N>;
@@ -234,9 +256,9 @@ struct Twiddles {
   static constexpr int C1 = NB1 * (P::R1 - 1), C2 = NB2 * (P::R2 - 1);
   static constexpr bool REG = P::NP == 3 && (C1 + C2) <= 16;
   float2 w1[REG ? C1 : 1], w2[REG ? C2 : 1];
-  const float2* tab;
+  // REG: both twiddled passes' factors from the table W_N^m, held in registers;
+  // otherwise each pass computes its own (twiddle_at)
   __device__ __forceinline__ void init(const float2* __restrict__ tw, int t) {
-    tab = tw;
     if constexpr (REG) {
 #pragma unroll
       for (int b = 0; b < NB1; b++) {
@@ -256,14 +278,12 @@ struct Twiddles {
 
 // One Stockham pass.  FIRST: input from registers (natural layout).  LAST:
 // output to registers (natural layout).  Otherwise through the LDS slot.
-// wr: this pass's register twiddles (or nullptr: read tab, the forward table
-// W_N^m = exp(-2 pi i m/N), m < N).
+// wr: this pass's register twiddles, or nullptr: the pass computes them.
 // ZIN (first pass) / NEED (last pass): masks over the natural-layout register
 // slots m (element t + T*m) that are zero on input / needed on output; used when
 // one butterfly covers all E slots (R == E), ignored otherwise.
 template <int N, int R, int NS, int DIR, bool FIRST, bool LAST, bool SWZ, int ZIN = 0, int NEED = -1>
-__device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const float2* wr,
-                                              const float2* __restrict__ tab, int t) {
+__device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const float2* wr, int t) {
   constexpr int E = Plan<N>::E;
   constexpr int T = N / E;
   constexpr int NB = E / R;
@@ -280,24 +300,21 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
       for (int k = 0; k < R; k++) a[b][k] = slot[lidx<N, SWZ>(t + T * b + k * (N / R))];
   }
   if constexpr (NS > 1) {
-    // table twiddles are loaded where they are used: the empty asm makes the
-    // index offset opaque, so the loads are not hoisted out of the callers'
-    // loops into registers held across the whole kernel (laundering the offset
-    // rather than the pointer keeps the table in the global address space:
-    // global loads, not flat loads that also hold up the LDS waits)
+    // computed twiddles are formed where they are used: the empty asm makes the
+    // index offset opaque, so they are not hoisted out of the callers' loops
+    // into registers held across the whole kernel
     int z = 0;
     if (!wr) asm volatile("" : "+s"(z));
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int jm = (t + T * b) % NS;
-      // table path: w^k of w = W_N^{jm N/(NS R)} as w^(k & ~3) * w^(k & 3), both
-      // factors from the table (R/4 - 1 + 3 loads; one rounding more than a
-      // direct table value)
+      // w^k of w = W_N^{jm N/(NS R)}: computed for k < 4 and k a multiple of 4,
+      // the others as w^(k & ~3) * w^(k & 3) (6 evaluations for a radix-16 pass)
       float2 pw[R];
       if (!wr) {
 #pragma unroll
         for (int k = 1; k < R; k++)
-          if ((k & 3) == 0 || k < 4) pw[k] = tab[z + k * jm * (N / (NS * R))];
+          if ((k & 3) == 0 || k < 4) pw[k] = twiddle_at<N>(z + k * jm * (N / (NS * R)));
 #pragma unroll
         for (int k = 5; k < R; k++)
           if (k & 3) pw[k] = cmul(pw[k & ~3], pw[k & 3]);
@@ -354,13 +371,13 @@ __device__ __forceinline__ void fft_wave(float2* v, float2* slot, const Twiddles
   constexpr int S1 = P::R0;
   constexpr int S2 = P::R0 * P::R1;
   constexpr bool REG = Twiddles<N>::REG;
-  stockham_pass<N, P::R0, 1, DIR, true, false, SWZ, ZIN>(v, slot, nullptr, tw.tab, t);
-  stockham_pass<N, P::R1, S1, DIR, false, false, SWZ>(v, slot, REG ? tw.w1 : nullptr, tw.tab, t);
+  stockham_pass<N, P::R0, 1, DIR, true, false, SWZ, ZIN>(v, slot, nullptr, t);
+  stockham_pass<N, P::R1, S1, DIR, false, false, SWZ>(v, slot, REG ? tw.w1 : nullptr, t);
   if constexpr (P::NP == 3) {
-    stockham_pass<N, P::R2, S2, DIR, false, true, SWZ, 0, NEED>(v, slot, REG ? tw.w2 : nullptr, tw.tab, t);
+    stockham_pass<N, P::R2, S2, DIR, false, true, SWZ, 0, NEED>(v, slot, REG ? tw.w2 : nullptr, t);
   } else {
-    stockham_pass<N, P::R2, S2, DIR, false, false, SWZ>(v, slot, nullptr, tw.tab, t);
-    stockham_pass<N, P::R3, S2 * P::R2, DIR, false, true, SWZ, 0, NEED>(v, slot, nullptr, tw.tab, t);
+    stockham_pass<N, P::R2, S2, DIR, false, false, SWZ>(v, slot, nullptr, t);
+    stockham_pass<N, P::R3, S2 * P::R2, DIR, false, true, SWZ, 0, NEED>(v, slot, nullptr, t);
   }
 }
 
