@@ -256,9 +256,11 @@ struct Twiddles {
   static constexpr int C1 = NB1 * (P::R1 - 1), C2 = NB2 * (P::R2 - 1);
   static constexpr bool REG = P::NP == 3 && (C1 + C2) <= 16;
   float2 w1[REG ? C1 : 1], w2[REG ? C2 : 1];
-  // REG: both twiddled passes' factors from the table W_N^m, held in registers;
-  // otherwise each pass computes its own (twiddle_at)
+  const float2* tab;  // the table W_N^m, m < N
+  // REG: both twiddled passes' factors from the table, held in registers;
+  // otherwise each pass computes its own (twiddle_at) or loads them from tab
   __device__ __forceinline__ void init(const float2* __restrict__ tw, int t) {
+    tab = tw;
     if constexpr (REG) {
 #pragma unroll
       for (int b = 0; b < NB1; b++) {
@@ -278,12 +280,14 @@ struct Twiddles {
 
 // One Stockham pass.  FIRST: input from registers (natural layout).  LAST:
 // output to registers (natural layout).  Otherwise through the LDS slot.
-// wr: this pass's register twiddles, or nullptr: the pass computes them.
+// wr: this pass's register twiddles, or nullptr: the pass computes them (TWC)
+// or loads them from tab, the table W_N^m = exp(-2 pi i m/N), m < N.
 // ZIN (first pass) / NEED (last pass): masks over the natural-layout register
 // slots m (element t + T*m) that are zero on input / needed on output; used when
 // one butterfly covers all E slots (R == E), ignored otherwise.
-template <int N, int R, int NS, int DIR, bool FIRST, bool LAST, bool SWZ, int ZIN = 0, int NEED = -1>
-__device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const float2* wr, int t) {
+template <int N, int R, int NS, int DIR, bool FIRST, bool LAST, bool SWZ, int ZIN = 0, int NEED = -1, bool TWC = true>
+__device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const float2* wr,
+                                              const float2* __restrict__ tab, int t) {
   constexpr int E = Plan<N>::E;
   constexpr int T = N / E;
   constexpr int NB = E / R;
@@ -300,21 +304,27 @@ __device__ __forceinline__ void stockham_pass(float2* v, float2* slot, const flo
       for (int k = 0; k < R; k++) a[b][k] = slot[lidx<N, SWZ>(t + T * b + k * (N / R))];
   }
   if constexpr (NS > 1) {
-    // computed twiddles are formed where they are used: the empty asm makes the
-    // index offset opaque, so they are not hoisted out of the callers' loops
-    // into registers held across the whole kernel
+    // computed or table twiddles are formed where they are used: the empty asm
+    // makes the index offset opaque, so they are not hoisted out of the callers'
+    // loops into registers held across the whole kernel (and the table stays in
+    // the global address space: global loads, not flat ones that also hold up
+    // the LDS waits)
     int z = 0;
     if (!wr) asm volatile("" : "+s"(z));
 #pragma unroll
     for (int b = 0; b < NB; b++) {
       const int jm = (t + T * b) % NS;
-      // w^k of w = W_N^{jm N/(NS R)}: computed for k < 4 and k a multiple of 4,
-      // the others as w^(k & ~3) * w^(k & 3) (6 evaluations for a radix-16 pass)
+      // w^k of w = W_N^{jm N/(NS R)}: computed or loaded for k < 4 and k a
+      // multiple of 4, the others as w^(k & ~3) * w^(k & 3) (6 evaluations or
+      // loads for a radix-16 pass)
       float2 pw[R];
       if (!wr) {
 #pragma unroll
         for (int k = 1; k < R; k++)
-          if ((k & 3) == 0 || k < 4) pw[k] = twiddle_at<N>(z + k * jm * (N / (NS * R)));
+          if ((k & 3) == 0 || k < 4) {
+            const int m = z + k * jm * (N / (NS * R));
+            pw[k] = TWC ? twiddle_at<N>(m) : tab[m];
+          }
 #pragma unroll
         for (int k = 5; k < R; k++)
           if (k & 3) pw[k] = cmul(pw[k & ~3], pw[k & 3]);
@@ -365,19 +375,19 @@ constexpr int gap_mask() {
 // Full length-N transform of the thread-distributed vector v (natural layout
 // in and out).  slot: this transform's LDS region of lds_slot<N, SWZ>() float2.
 // ZIN: slots known to be zero on input; NEED: slots needed on output.
-template <int N, int DIR, bool SWZ = kSwz<N>, int ZIN = 0, int NEED = -1>
+template <int N, int DIR, bool SWZ = kSwz<N>, int ZIN = 0, int NEED = -1, bool TWC = true>
 __device__ __forceinline__ void fft_wave(float2* v, float2* slot, const Twiddles<N>& tw, int t) {
   using P = Plan<N>;
   constexpr int S1 = P::R0;
   constexpr int S2 = P::R0 * P::R1;
   constexpr bool REG = Twiddles<N>::REG;
-  stockham_pass<N, P::R0, 1, DIR, true, false, SWZ, ZIN>(v, slot, nullptr, t);
-  stockham_pass<N, P::R1, S1, DIR, false, false, SWZ>(v, slot, REG ? tw.w1 : nullptr, t);
+  stockham_pass<N, P::R0, 1, DIR, true, false, SWZ, ZIN, -1, TWC>(v, slot, nullptr, tw.tab, t);
+  stockham_pass<N, P::R1, S1, DIR, false, false, SWZ, 0, -1, TWC>(v, slot, REG ? tw.w1 : nullptr, tw.tab, t);
   if constexpr (P::NP == 3) {
-    stockham_pass<N, P::R2, S2, DIR, false, true, SWZ, 0, NEED>(v, slot, REG ? tw.w2 : nullptr, t);
+    stockham_pass<N, P::R2, S2, DIR, false, true, SWZ, 0, NEED, TWC>(v, slot, REG ? tw.w2 : nullptr, tw.tab, t);
   } else {
-    stockham_pass<N, P::R2, S2, DIR, false, false, SWZ>(v, slot, nullptr, t);
-    stockham_pass<N, P::R3, S2 * P::R2, DIR, false, true, SWZ, 0, NEED>(v, slot, nullptr, t);
+    stockham_pass<N, P::R2, S2, DIR, false, false, SWZ, 0, -1, TWC>(v, slot, nullptr, tw.tab, t);
+    stockham_pass<N, P::R3, S2 * P::R2, DIR, false, true, SWZ, 0, NEED, TWC>(v, slot, nullptr, tw.tab, t);
   }
 }
 

@@ -213,6 +213,9 @@ template <int NX> struct ColCfg {
   // tiles grouped on one XCD: those sharing a 128-byte line (16 columns); pairs
   // of whole-line tiles at 512 (measured faster than no grouping)
   static constexpr int GS = TC >= 16 ? 2 : 16 / TC;
+  // large-size twiddles computed in the pass, or loaded from the table at 1024
+  // (measured: columns 147 vs 128 us per 32-probe launch)
+  static constexpr bool TWC = NX != 1024;
   // column slots use the PAD layout (registers are the scarcer resource here);
   // float2 per LDS column, rounded so the tile staging is at most 2-way
   static constexpr int STRIDE = ((lds_slot<NX, false>() + 31) / 32) * 32 + 2;
@@ -359,7 +362,7 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_sweep_cols(SweepArgs a) {
       float2 v[E];
 #pragma unroll
       for (int m = 0; m < E; m++) v[m] = col[lidx<NX, false>(t + T * m)];
-      fft_wave<NX, -1, false, 0, ~GAP>(v, col, tw, t);
+      fft_wave<NX, -1, false, 0, ~GAP, ColCfg<NX>::TWC>(v, col, tw, t);
       // propagate: P = px[ix]*py[iy] (1/(nx*ny) folded into px), zero outside k2max
       if (iy >= 0) {
         const float ky2 = a.ky2[iy];
@@ -415,7 +418,7 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_sweep_cols(SweepArgs a) {
 #pragma unroll
         for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
       }
-      fft_wave<NX, +1, false, GAP>(v, col, tw, t);
+      fft_wave<NX, +1, false, GAP, -1, ColCfg<NX>::TWC>(v, col, tw, t);
       xform_sync<NX>();
 #pragma unroll
       for (int m = 0; m < E; m++) col[lidx<NX, false>(t + T * m)] = v[m];
@@ -614,7 +617,7 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_adj_cols(AdjArgs A) {
     float2 v[E];
 #pragma unroll
     for (int m = 0; m < E; m++) v[m] = col[lidx<NX, false>(t + T * m)];
-    fft_wave<NX, -1, false>(v, col, tw, t);
+    fft_wave<NX, -1, false, 0, -1, ColCfg<NX>::TWC>(v, col, tw, t);
     if (iy >= 0) {
       const float ky2 = a.ky2[iy];
       const float2 py = a.py[iy];
@@ -639,7 +642,7 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_adj_cols(AdjArgs A) {
 #pragma unroll
       for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
     }
-    fft_wave<NX, +1, false>(v, col, tw, t);
+    fft_wave<NX, +1, false, 0, -1, ColCfg<NX>::TWC>(v, col, tw, t);
     xform_sync<NX>();
 #pragma unroll
     for (int m = 0; m < E; m++) col[lidx<NX, false>(t + T * m)] = v[m];
