@@ -312,7 +312,7 @@ def main():
     ap.add_argument("--tiles-per-block", type=int, default=0, help="with --group: column tiles per block")
     a = ap.parse_args()
     if a.gpus != 1:
-        print("bench.py: this build runs on one GPU only (no multi-GPU path); got --gpus %d" % a.gpus,
+        print("bench.py: the library runs on one GPU only (no multi-GPU path); got --gpus %d" % a.gpus,
               file=sys.stderr)
         sys.exit(2)
 

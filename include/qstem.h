@@ -33,7 +33,7 @@ extern "C" {
 
 #define QS_OK 0
 #define QS_ERR_ARG 1          /* invalid argument or inconsistent sizes */
-#define QS_ERR_UNSUPPORTED 2  /* size or option this build does not handle */
+#define QS_ERR_UNSUPPORTED 2  /* size or option the library does not handle */
 #define QS_ERR_HIP 3          /* HIP runtime failure (message in qs_last_error) */
 #define QS_ERR_STATE 4        /* call order violated (e.g. run before potential) */
 

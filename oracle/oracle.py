@@ -1,7 +1,8 @@
 """ctypes wrapper of the FP64 CPU oracle (oracle/_build/libqs_oracle.so).
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
-bench.py's cpu_baseline leg, as the checker.  The product path never uses it.
+bench.py's cpu_baseline leg, as the reference that results are compared with.
+The product path never uses it.
 """
 import ctypes
 import os

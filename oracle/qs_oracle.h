@@ -1,9 +1,9 @@
 /*
  * qs_oracle.h -- FP64 CPU restatement of QSTEM's STEM multislice path.
  *
- * TEST INFRASTRUCTURE ONLY.  This library is the checker for the HIP product
- * path: only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg
- * may load it.  The product library (libqstem_hip.so) never links or calls it.
+ * TEST INFRASTRUCTURE ONLY.  This library is the FP64 reference the HIP product
+ * path is compared with: only tests/, __graft_entry__.smoke() and bench.py's
+ * cpu_baseline leg may load it.  The product library (libqstem_hip.so) never links or calls it.
  *
  * Every function cites the reference routine it restates (paths relative to
  * the QSTEM source tree).  Arithmetic is double precision throughout, except

@@ -9,7 +9,7 @@ configuration is time-seeded, so only statistics can be compared:
   * the mean of the 20x20 ADF image at t = 46.86 A (0.0673).
 Differences kept on purpose: a 512x512 grid at the reference's dx = 0.0625 A
 (same 134 mrad band limit, 32 A instead of 25 A window) and neutral atoms
-(the reference adds ion offsets, which this build does not implement).
+(the reference adds ion offsets, which the library does not implement).
 
 What the files pin here: the scan geometry (window offsets, image
 orientation [scanXN][scanYN]) through the correlation of the ADF image with
