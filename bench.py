@@ -72,13 +72,14 @@ def synthetic_potential(cfg, potoff):
     return np.concatenate([V] * (cfg.slices // 2), axis=0)
 
 
-def uneven_workload(nconfig=16, steps=3):
+def uneven_workload(nconfig=16, steps=3, streams=0):
     """Uneven batch with frozen phonons: seeded amorphous carbon slab 80 x 80 x 100 A
     at 2.0 g/cm^3 (1.4 A minimum separation) with a 2 nm gold FCC sphere (a = 4.08 A)
     at its centre; nconfig Gaussian phonon configurations (sigma = 0.08 A); 100 slices
     of 1 A; 200 kV, 20 mrad; dx = 0.08 A on a 1088^2 potential grid; windows 256^2,
     512^2 and 1024^2 in the ratio 1/2 : 1/3 : 1/6 (6, 4 and 2 positions per
-    configuration), all advanced by one qs_run_groups call.  Returns the record."""
+    configuration), all advanced by one qs_run_groups call (streams: 0 = library
+    default, else 1 or 2 HIP streams).  Returns the record."""
     import torch
     from qstem_amd import Detector, Multislice, StemConfig, record_bins, run_groups
     from qstem_amd.specimen import carbon_gold_specimen, gaussian_phonons
@@ -104,6 +105,9 @@ def uneven_workload(nconfig=16, steps=3):
     ctxs = {1024: big, 512: make(512), 256: make(256)}
     for n in (512, 256):
         ctxs[n].share_trans(big)
+    if streams:
+        for ms in ctxs.values():
+            ms.set_streams(streams)
     per_cfg = {256: 6, 512: 4, 1024: 2}
     groups = []
     for n, ms in ctxs.items():
@@ -133,6 +137,7 @@ def uneven_workload(nconfig=16, steps=3):
     return {"metric": "item-slices/s, uneven batch (256^2/512^2/1024^2) x %d phonon configurations" % nconfig,
             "value": round(items * nslice / el, 1), "unit": "item-slices/s", "items": items,
             "items_per_size": {str(n): len(g[1]) for n, g in zip(ctxs, groups)}, "slices": nslice,
+            "streams": big.get_streams(),
             "ms_per_batch": round(el * 1e3, 2), "sequential_sizes_ms": round(el_seq * 1e3, 2),
             "atoms": int(len(atoms)), "specimen_s": round(t_spec, 2),
             "potential_build_s_per_config": round(t_pot / nconfig, 3),
@@ -310,6 +315,8 @@ def main():
                          "(calibrates the FETCH_SIZE / WRITE_SIZE counters); copyrate: plain device copy rates")
     ap.add_argument("--rows-per-wave", type=int, default=0, help="with --group: row-sweep steps per wavefront")
     ap.add_argument("--tiles-per-block", type=int, default=0, help="with --group: column tiles per block")
+    ap.add_argument("--streams", type=int, default=0, choices=[0, 1, 2],
+                    help="HIP streams for the probe groups of the timed runs (0 = library default)")
     a = ap.parse_args()
     if a.gpus != 1:
         print("bench.py: the library runs on one GPU only (no multi-GPU path); got --gpus %d" % a.gpus,
@@ -317,7 +324,7 @@ def main():
         sys.exit(2)
 
     if a.workload == "uneven":
-        print(json.dumps(uneven_workload()), flush=True)
+        print(json.dumps(uneven_workload(streams=a.streams)), flush=True)
         return
     if a.workload == "copy":  # counter calibration: a plain 512 MiB device-to-device copy, 3 times
         import torch
@@ -366,6 +373,9 @@ def main():
     t0 = time.perf_counter()
     psi0 = ms.make_probe()
     t_probe = time.perf_counter() - t0
+    if a.streams:
+        ms.set_streams(a.streams)
+    streams = ms.get_streams()
     ms.set_batch(a.group)
     ms.set_tuning(a.rows_per_wave, a.tiles_per_block)
     tuning = None
@@ -401,15 +411,24 @@ def main():
         ms.set_graph(False)
         return t1 - t0, kt
 
-    el_eager, (ms_rows, ms_cols, n_rows, n_cols) = timed(graph=False, profile=True)
+    # kernel durations for the roofline: eager launches on one stream, so that no other
+    # launch overlaps the timed ones (with two streams, two groups' sweeps share the GPU)
+    # and the library's default group size for one stream
+    tuned = ms.get_tuning()
+    ms.set_streams(1)
+    ms.set_batch(0)
+    _, (ms_rows, ms_cols, n_rows, n_cols) = timed(graph=False, profile=True)
+    ms.set_streams(streams)
+    ms.set_batch(tuned[2])
+    el_eager, _ = timed(graph=False, profile=False)
     el_graph, _ = timed(graph=True, profile=False) if not a.no_graph else (float("inf"), None)
     elapsed = min(el_eager, el_graph)
     probe_slices = a.steps * P * nslice
     value = probe_slices / elapsed
 
-    # per-probe GPU loop: same kernels, one probe per launch sequence
+    # per-probe GPU loop: same kernels, one probe per launch sequence on one stream
     # (library default launch shapes, which suit one probe per launch)
-    tuned = ms.get_tuning()
+    ms.set_streams(1)
     ms.set_batch(1)
     ms.set_tuning(0, 0)
     sl = ipos_all[:a.loop_probes]
@@ -421,6 +440,7 @@ def main():
     loop_rate = a.loop_probes * nslice / (time.perf_counter() - t0)
     ms.set_tuning(tuned[0], tuned[1])
     ms.set_batch(tuned[2])
+    ms.set_streams(streams)
 
     # roofline of the slice step (row sweep + column sweep, one logical kernel in two launches)
     N2 = cfg.nx * cfg.ny
@@ -455,8 +475,8 @@ def main():
         "data": "synthetic: perovskite-window projected potential (Gaussian atom columns), seeded positions",
         "config": {"workload": "batched plain STEM, perovskite 12x12x40 cells, 80 slices, 512x512 wave, "
                                "dx=0.030 A, 300 kV, 25 mrad, Cs=1 um, df=-5 nm, 64x64 scan, BF/ABF/ADF",
-                   "probes_per_launch": P, "probes_per_kernel_launch": G, "slices": nslice, "nx": cfg.nx,
-                   "potN": cfg.potNx},
+                   "probes_per_call": P, "probes_per_kernel_launch": tuned[2] or G, "streams": streams,
+                   "slices": nslice, "nx": cfg.nx, "potN": cfg.potNx},
         "launch_modes": {"eager_events": round(probe_slices / el_eager, 1),
                          "hip_graph": round(probe_slices / el_graph, 1) if el_graph < float("inf") else None,
                          "value_from": "hip_graph" if el_graph < el_eager else "eager_events"},
@@ -468,6 +488,7 @@ def main():
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
                      "kernel": "slice step = k_sweep_rows<512,1> + k_sweep_cols<512>",
+                     "launches": "eager, one stream, %d probes per launch (HIP events per launch)" % G,
                      "bytes_per_probe_slice": bytes_per_probe_slice},
     }
 

@@ -233,6 +233,13 @@ int qs_atom_gradients(qs_ctx *ctx, const qs_atom *atoms, int natom, const qs_pot
 int qs_set_batch(qs_ctx *ctx, int batch);
 int qs_get_batch(const qs_ctx *ctx);
 
+/* Streams of qs_run_stem and qs_run_groups: 2 (default) lets consecutive probe
+ * groups (qs_run_groups: contexts) run on two HIP streams, so that one group's
+ * launches fill the other's ramp-up and drain; 1 keeps every launch on the
+ * context stream.  Results do not depend on it.  No reference counterpart. */
+int qs_set_streams(qs_ctx *ctx, int nstreams);
+int qs_get_streams(const qs_ctx *ctx);
+
 /* Replay the slice loop of qs_run_stem as a HIP graph: captured on the first call
  * of a given shape (probe count, slices, bins, outputs, buffers) and relaunched
  * with one hipGraphLaunch while the shape repeats; any setter above drops it.

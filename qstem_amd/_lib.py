@@ -98,6 +98,8 @@ SIGNATURES = {
     "qs_atom_gradients": (_I, [_P, ctypes.POINTER(QsAtom), _I, ctypes.POINTER(QsPotParams), _FP]),
     "qs_set_batch": (_I, [_P, _I]),
     "qs_get_batch": (_I, [_P]),
+    "qs_set_streams": (_I, [_P, _I]),
+    "qs_get_streams": (_I, [_P]),
     "qs_set_graph": (_I, [_P, _I]),
     "qs_set_tuning": (_I, [_P, _I, _I]),
     "qs_get_tuning": (_I, [_P, _IP, _IP, _IP]),

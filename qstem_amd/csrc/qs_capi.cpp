@@ -124,6 +124,12 @@ struct qs_ctx {
   qs_params p{};
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  // second stream of qs_run_stem / qs_run_groups: consecutive probe groups (or the
+  // contexts of qs_run_groups) alternate between the two streams so that one group's
+  // launches fill the other's ramp-up and drain; each call joins it back into stream
+  int nstreams = 2;
+  hipStream_t stream2 = nullptr;
+  hipEvent_t evfork = nullptr, evjoin = nullptr;
   float last_ms = 0.f;
   std::string err;
   int batch = 0;
@@ -147,7 +153,7 @@ struct qs_ctx {
   // device
   DevBuf twx, twy, px, py, dkx2, dky2;
   DevBuf probe, trans, scratch, V;
-  DevBuf state, ipos, partial, det, diff, exitw, total;
+  DevBuf state, ipos, partial, partial2, det, diff, exitw, total;
   DevBuf pkx2, pky2, probe_partial;
   RowXform rx, ry;  // potNx and potNy row transforms
   // band limit of the transmission functions as band-compact sweeps (potNx and
@@ -283,12 +289,15 @@ int qs_create(const qs_params* p, qs_ctx** out) {
     return code;
   };
   if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
-      hipEventCreate(&c->ev1) != hipSuccess) {
+      hipEventCreate(&c->ev1) != hipSuccess || hipStreamCreate(&c->stream2) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evfork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evjoin, hipEventDisableTiming) != hipSuccess) {
     c->err = "hipStreamCreate/hipEventCreate failed (no HIP device?)";
     return bad(QS_ERR_HIP);
   }
   if (const char* e = getenv("QS_ROWS_PER_WAVE")) c->rows_per_wave = atoi(e);  // tuning overrides
   if (const char* e = getenv("QS_TILES_PER_BLOCK")) c->tiles_per_block = atoi(e);
+  if (const char* e = getenv("QS_STREAMS")) c->nstreams = atoi(e) == 1 ? 1 : 2;
   if (qs::set_col_lds_limits() != hipSuccess) {
     c->err = "hipFuncSetAttribute for the column sweep LDS size failed";
     return bad(QS_ERR_HIP);
@@ -429,7 +438,7 @@ void qs_destroy(qs_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   drop_graph(c);
   DevBuf* bufs[] = {&c->cfgidx, &c->twx, &c->twy, &c->px, &c->py, &c->dkx2, &c->dky2, &c->probe, &c->trans, &c->scratch, &c->V,
-                    &c->state, &c->ipos, &c->partial, &c->det, &c->diff, &c->exitw, &c->total, &c->pkx2, &c->pky2,
+                    &c->state, &c->ipos, &c->partial, &c->partial2, &c->det, &c->diff, &c->exitw, &c->total, &c->pkx2, &c->pky2,
                     &c->probe_partial, &c->rx.tw, &c->rx.chirp_f, &c->rx.vhat_f, &c->rx.chirp_b, &c->rx.vhat_b,
                     &c->ry.tw, &c->ry.chirp_f, &c->ry.vhat_f, &c->ry.chirp_b, &c->ry.vhat_b,
                     &c->atoms, &c->kinds, &c->lutptrs, &c->spl, &c->luttmp, &c->lutscr,
@@ -443,7 +452,10 @@ void qs_destroy(qs_ctx* c) {
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->evfork) (void)hipEventDestroy(c->evfork);
+  if (c->evjoin) (void)hipEventDestroy(c->evjoin);
   if (c->stream) (void)hipStreamDestroy(c->stream);
+  if (c->stream2) (void)hipStreamDestroy(c->stream2);
   delete c;
 }
 
@@ -870,6 +882,15 @@ int qs_set_batch(qs_ctx* c, int batch) {
   return QS_OK;
 }
 
+int qs_set_streams(qs_ctx* c, int nstreams) {
+  if (!c || (nstreams != 1 && nstreams != 2)) return fail(c, QS_ERR_ARG, "qs_set_streams: nstreams must be 1 or 2");
+  if (c->nstreams != nstreams) drop_graph(c);
+  c->nstreams = nstreams;
+  return QS_OK;
+}
+
+int qs_get_streams(const qs_ctx* c) { return c ? c->nstreams : QS_ERR_ARG; }
+
 int qs_set_graph(qs_ctx* c, int on) {
   if (!c) return QS_ERR_ARG;
   c->use_graph = on != 0;
@@ -989,11 +1010,14 @@ static int plan_run(qs_ctx* c, const char* who, const int* ipos, const int* conf
   }
   const int nx = p.nx, ny = p.ny;
   // probe group advanced through all slices together: the library default keeps the
-  // group's wave state (two sweeps per slice read and write it) inside the 256 MB
-  // Infinity Cache, measured best at ~180 MB for 512x512 waves
+  // wave state of the groups in flight (two sweeps per slice read and write it) inside
+  // the 256 MB Infinity Cache: one group of ~180 MB, or with two streams two groups of
+  // ~96 MB (measured best for 512x512 waves: 128 probes, 2 x 64 probes)
   const size_t per_probe = sizeof(float2) * (size_t)nx * c->kcp;
-  int Bdef = (int)std::max<size_t>(32, std::min<size_t>(1024, (size_t)(180u << 20) / per_probe));
+  const size_t budget = (size_t)(c->nstreams > 1 ? 96u : 180u) << 20;
+  int Bdef = (int)std::max<size_t>(32, std::min<size_t>(1024, budget / per_probe));
   Bdef = std::max(32, (Bdef / 16) * 16);
+  if (c->nstreams > 1 && nprobe > 1) Bdef = std::min(Bdef, (nprobe + 1) / 2);  // both streams get work
   r = RunPlan{};
   r.c = c;
   r.nprobe = nprobe;
@@ -1019,6 +1043,8 @@ static int plan_run(qs_ctx* c, const char* who, const int* ipos, const int* conf
     HIPCHK(c, hipMemcpyAsync(c->cfgidx.p, config, sizeof(int) * nprobe, hipMemcpyHostToDevice, c->stream));
   }
   HIPCHK(c, c->partial.ensure(sizeof(float) * (size_t)r.B * c->ntiles * (qs::kMaxDet + 1)));
+  if (c->nstreams > 1 && r.nprobe > r.B)
+    HIPCHK(c, c->partial2.ensure(sizeof(float) * (size_t)r.B * c->ntiles * (qs::kMaxDet + 1)));
   if (r.want_det) {
     const size_t nd = (size_t)nbins * p.ndet * nprobe;
     HIPCHK(c, c->det.ensure(sizeof(float) * nd));
@@ -1085,8 +1111,9 @@ struct LaunchMarks {
   }
 };
 
-// Launches of probe group [p0, p0 + B) at slice s; the exit-wave sweep follows the last slice.
-static hipError_t enqueue_slice(RunPlan& r, int p0, int s, hipStream_t st, LaunchMarks* lm) {
+// Launches of probe group [p0, p0 + B) at slice s on stream st, with detector partial
+// sums in `partial`; the exit-wave sweep follows the last slice.
+static hipError_t enqueue_slice(RunPlan& r, int p0, int s, hipStream_t st, float* partial, LaunchMarks* lm) {
   qs_ctx* c = r.c;
   const qs_params& p = c->p;
   SweepArgs& a = r.a;
@@ -1103,12 +1130,13 @@ static hipError_t enqueue_slice(RunPlan& r, int p0, int s, hipStream_t st, Launc
   const bool tot = last && r.intensity;
   a.rec_bin = (bin >= 0 || tot) ? (bin >= 0 ? bin : 0) : -1;
   a.diff = (last && r.diff) ? c->diff.as<float>() : nullptr;
+  a.partial = partial;
   if (lm && (e = lm->mark(1, true, st)) != hipSuccess) return e;
   qs::launch_sweep_cols(a, a.rec_bin >= 0, a.diff != nullptr, st);
   if (lm && (e = lm->mark(1, false, st)) != hipSuccess) return e;
   if (bin >= 0 || tot) {
     float* dbin = bin >= 0 ? c->det.as<float>() + (size_t)bin * p.ndet * r.nprobe : nullptr;
-    qs::launch_reduce_partials(c->partial.as<float>(), nb, c->ntiles, bin >= 0 ? p.ndet : 0, p.electronScale, dbin,
+    qs::launch_reduce_partials(partial, nb, c->ntiles, bin >= 0 ? p.ndet : 0, p.electronScale, dbin,
                                r.nprobe, tot ? c->total.as<float>() : nullptr, p0, st);
   }
   if (last && r.exit_wave) qs::launch_sweep_rows(a, 2, st);
@@ -1142,12 +1170,28 @@ int qs_run_stem_cfg(qs_ctx* c, const int* ipos, const int* config, int nprobe, i
   if (rc != QS_OK) return rc;
   LaunchMarks lm{c};
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
+  // probe group g runs on stream g % 2 (one stream for a single group or nstreams 1);
+  // the groups of one stream follow each other, and the launches of a pair of
+  // groups are issued slice by slice so that both streams stay fed
+  const int ngroup = (r.nprobe + r.B - 1) / r.B;
+  const int nst = (c->nstreams > 1 && ngroup > 1) ? 2 : 1;
   auto enqueue = [&]() -> hipError_t {
-    for (int p0 = 0; p0 < r.nprobe; p0 += r.B)
-      for (int s = 0; s < r.nslice; s++) {
-        const hipError_t e = enqueue_slice(r, p0, s, c->stream, &lm);
-        if (e != hipSuccess) return e;
-      }
+    hipError_t e = hipSuccess;
+    if (nst == 2) {  // fork: the second stream starts after the uploads on the first
+      if ((e = hipEventRecord(c->evfork, c->stream)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(c->stream2, c->evfork, 0)) != hipSuccess) return e;
+    }
+    for (int g = 0; g < ngroup; g += nst)
+      for (int s = 0; s < r.nslice; s++)
+        for (int k = 0; k < nst && g + k < ngroup; k++) {
+          hipStream_t st = k ? c->stream2 : c->stream;
+          float* part = (k ? c->partial2 : c->partial).as<float>();
+          if ((e = enqueue_slice(r, (g + k) * r.B, s, st, part, &lm)) != hipSuccess) return e;
+        }
+    if (nst == 2) {  // join: the first stream waits for the second
+      if ((e = hipEventRecord(c->evjoin, c->stream2)) != hipSuccess) return e;
+      if ((e = hipStreamWaitEvent(c->stream, c->evjoin, 0)) != hipSuccess) return e;
+    }
     return hipGetLastError();
   };
   if (c->use_graph && !c->profiling) {
@@ -1156,7 +1200,7 @@ int qs_run_stem_cfg(qs_ctx* c, const int* ipos, const int* config, int nprobe, i
                                   exit_wave != nullptr, int_intensity != nullptr, r.has_cfg,
                                   (long long)trans_base(c), (long long)c->state.p, (long long)c->det.p,
                                   (long long)c->diff.p, (long long)c->exitw.p, (long long)c->total.p,
-                                  (long long)c->partial.p, (long long)c->ipos.p, (long long)c->cfgidx.p,
+                                  (long long)c->partial.p, (long long)c->partial2.p, nst, (long long)c->ipos.p, (long long)c->cfgidx.p,
                                   (long long)c->probe.p, c->rows_per_wave, c->tiles_per_block};
     for (int s = 0; s < nslice; s++) key.push_back(bin_of_slice ? bin_of_slice[s] : -1);
     if (!c->gexec || key != c->gkey) {
@@ -1229,12 +1273,40 @@ int qs_run_groups(const qs_group* groups, int ngroup, const int* bin_of_slice, i
   // function is read by every window size while it is cache-resident
   int rounds = 0;
   for (auto& r : plans) rounds = std::max(rounds, (r.nprobe + r.B - 1) / r.B);
+  // with two streams each context's launches go to the stream with less work so far
+  // (work ~ items x rows x band columns), so that one stream's launches fill the
+  // other's ramp-up and drain
+  std::vector<int> on2(ngroup, 0);
+  bool two = false;
+  if (c0->nstreams > 1 && ngroup > 1) {
+    std::vector<int> order(ngroup);
+    for (int g = 0; g < ngroup; g++) order[g] = g;
+    auto work = [&](int g) { return (double)plans[g].nprobe * groups[g].ctx->p.nx * groups[g].ctx->kcp; };
+    std::sort(order.begin(), order.end(), [&](int u, int v) { return work(u) > work(v); });
+    double w[2] = {0.0, 0.0};
+    for (int g : order) {
+      on2[g] = w[1] < w[0] ? 1 : 0;
+      w[on2[g]] += work(g);
+    }
+    two = w[1] > 0.0;
+  }
   HIPCHK(c0, hipEventRecord(c0->ev0, c0->stream));
+  if (two) {
+    HIPCHK(c0, hipEventRecord(c0->evfork, c0->stream));
+    HIPCHK(c0, hipStreamWaitEvent(c0->stream2, c0->evfork, 0));
+  }
   for (int j = 0; j < rounds; j++)
     for (int s = 0; s < nslice; s++)
-      for (auto& r : plans)
-        if ((long)j * r.B < r.nprobe) HIPCHK(c0, enqueue_slice(r, j * r.B, s, c0->stream, nullptr));
+      for (int g = 0; g < ngroup; g++) {
+        RunPlan& r = plans[g];
+        if ((long)j * r.B < r.nprobe)
+          HIPCHK(c0, enqueue_slice(r, j * r.B, s, on2[g] ? c0->stream2 : c0->stream, r.c->partial.as<float>(), nullptr));
+      }
   HIPCHK(c0, hipGetLastError());
+  if (two) {
+    HIPCHK(c0, hipEventRecord(c0->evjoin, c0->stream2));
+    HIPCHK(c0, hipStreamWaitEvent(c0->stream, c0->evjoin, 0));
+  }
   HIPCHK(c0, hipEventRecord(c0->ev1, c0->stream));
   HIPCHK(c0, hipStreamSynchronize(c0->stream));
   for (auto& r : plans) {
@@ -1539,9 +1611,9 @@ int qs_atom_gradients(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_p
 // device and wave geometry for the life of the process.
 namespace {
 struct TuneKey {
-  int dev, nx, ny, kcp, potNy;
+  int dev, nx, ny, kcp, potNy, nstreams;
   bool operator<(const TuneKey& o) const {
-    return std::tie(dev, nx, ny, kcp, potNy) < std::tie(o.dev, o.nx, o.ny, o.kcp, o.potNy);
+    return std::tie(dev, nx, ny, kcp, potNy, nstreams) < std::tie(o.dev, o.nx, o.ny, o.kcp, o.potNy, o.nstreams);
   }
 };
 struct TuneVal {
@@ -1559,7 +1631,7 @@ int qs_autotune(qs_ctx* c, const int* ipos, int nprobe, int* batch, int* rows_pe
     return fail(c, QS_ERR_STATE, "qs_autotune: set the potential and the probe first");
   int dev = 0;
   HIPCHK(c, hipGetDevice(&dev));
-  const TuneKey key{dev, c->p.nx, c->p.ny, c->kcp, c->p.potNy};
+  const TuneKey key{dev, c->p.nx, c->p.ny, c->kcp, c->p.potNy, c->nstreams};
   TuneVal best{0, 0, 0, 0.f};
   bool cached = false;
   {

@@ -298,6 +298,13 @@ class Multislice:
     def set_batch(self, batch: int):
         self._check(self._lib.qs_set_batch(self._h, int(batch)), "qs_set_batch")
 
+    def set_streams(self, nstreams: int):
+        """1 or 2 HIP streams for the probe groups of run_stem (results are unchanged)."""
+        self._check(self._lib.qs_set_streams(self._h, int(nstreams)), "qs_set_streams")
+
+    def get_streams(self) -> int:
+        return self._lib.qs_get_streams(self._h)
+
     def set_graph(self, on: bool = True):
         """Replay run_stem's slice loop as a HIP graph (captured once per call shape)."""
         self._check(self._lib.qs_set_graph(self._h, 1 if on else 0), "qs_set_graph")

@@ -94,3 +94,55 @@ def test_share_trans_rejects_other_grids():
     with _ctx(cfg_a) as A, _ctx(cfg_b) as B:
         with pytest.raises(QstemError):
             B.share_trans(A)
+
+
+def test_two_streams_equal_one_stream():
+    """Probe groups alternating over two HIP streams (qs_set_streams 2) give the same
+    results as one stream, eager and as a HIP graph, for run_stem with every output
+    and for run_groups; an invalid stream count is rejected."""
+    from qstem_amd import QstemError, run_groups
+    cfg = small_config(nx=128, ny=128, pad=40, slices=5)
+    V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=60)
+    ipos = probe_positions(cfg, 11, seed=9)
+    bins = np.array([0, -1, 1, -1, 2], np.int32)
+    kw = dict(bins=bins, nbins=3, want_diff=True, want_exit=True, want_intensity=True)
+    res = {}
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        ms.make_probe()
+        ms.set_batch(3)  # four probe groups: two per stream, the last one partial
+        for ns in (1, 2):
+            ms.set_streams(ns)
+            assert ms.get_streams() == ns
+            for graph in (False, True, True):
+                ms.set_graph(graph)
+                res[(ns, graph, len(res))] = ms.run_stem(ipos, **kw)
+            ms.set_graph(False)
+        with pytest.raises(QstemError):
+            ms.set_streams(3)
+    ref = res[(1, False, 0)]
+    for key, out in res.items():
+        assert np.array_equal(out["exit"], ref["exit"]), key
+        assert rel_l2(out["det"], ref["det"]) < 1e-6, key
+        assert rel_l2(out["diff"], ref["diff"]) < 1e-6, key
+        assert np.allclose(out["intensity"], ref["intensity"], rtol=1e-6), key
+
+    cfg_a = small_config(nx=128, ny=128, pad=64, slices=3, dx=0.03)   # potN 192
+    cfg_b = small_config(nx=64, ny=64, pad=128, slices=3, dx=0.03)    # potN 192
+    Vg = smooth_potential(3, 192, 192, seed=61)
+    ipa = probe_positions(cfg_a, 6, seed=3)
+    ipb = probe_positions(cfg_b, 9, seed=4)
+    gb, gn = np.array([0, -1, 1], np.int32), 2
+    outs = {}
+    for ns in (1, 2):
+        with _ctx(cfg_a) as A, _ctx(cfg_b) as B:
+            A.set_potential(Vg)
+            B.share_trans(A)
+            A.make_probe()
+            B.make_probe()
+            A.set_streams(ns)
+            B.set_batch(4)
+            outs[ns] = run_groups([(A, ipa, None), (B, ipb, None)], bins=gb, nbins=gn, want_intensity=True)
+    for g in range(2):
+        assert rel_l2(outs[2][g]["det"], outs[1][g]["det"]) < 1e-6, g
+        assert np.allclose(outs[2][g]["intensity"], outs[1][g]["intensity"], rtol=1e-6), g
