@@ -143,7 +143,7 @@ struct qs_ctx {
   DevBuf cfgidx;                // per-probe configuration of the current run
   // HIP graph of the qs_run_stem slice loop, reused while the call shape matches
   bool use_graph = false;
-  hipGraphExec_t gexec = nullptr;
+  hipGraphExec_t gexec = nullptr, gexec2 = nullptr;  // one graph per stream
   std::vector<long long> gkey;
   // derived tables
   int band_a = 0, band_b = 0, kc = 0, kcp = 0, ntiles = 0, tc = 0;
@@ -422,7 +422,8 @@ int qs_create(const qs_params* p, qs_ctx** out) {
 
 static void drop_graph(qs_ctx* c) {
   if (c->gexec) (void)hipGraphExecDestroy(c->gexec);
-  c->gexec = nullptr;
+  if (c->gexec2) (void)hipGraphExecDestroy(c->gexec2);
+  c->gexec = c->gexec2 = nullptr;
   c->gkey.clear();
 }
 
@@ -1175,23 +1176,25 @@ int qs_run_stem_cfg(qs_ctx* c, const int* ipos, const int* config, int nprobe, i
   // groups are issued slice by slice so that both streams stay fed
   const int ngroup = (r.nprobe + r.B - 1) / r.B;
   const int nst = (c->nstreams > 1 && ngroup > 1) ? 2 : 1;
-  auto enqueue = [&]() -> hipError_t {
+  auto fork = [&]() -> hipError_t {  // the second stream starts after the uploads on the first
+    const hipError_t e = hipEventRecord(c->evfork, c->stream);
+    return e != hipSuccess ? e : hipStreamWaitEvent(c->stream2, c->evfork, 0);
+  };
+  auto join = [&]() -> hipError_t {  // the first stream waits for the second
+    const hipError_t e = hipEventRecord(c->evjoin, c->stream2);
+    return e != hipSuccess ? e : hipStreamWaitEvent(c->stream, c->evjoin, 0);
+  };
+  // launches of stream `only` (or of both streams, only = -1)
+  auto enqueue = [&](int only) -> hipError_t {
     hipError_t e = hipSuccess;
-    if (nst == 2) {  // fork: the second stream starts after the uploads on the first
-      if ((e = hipEventRecord(c->evfork, c->stream)) != hipSuccess) return e;
-      if ((e = hipStreamWaitEvent(c->stream2, c->evfork, 0)) != hipSuccess) return e;
-    }
     for (int g = 0; g < ngroup; g += nst)
       for (int s = 0; s < r.nslice; s++)
         for (int k = 0; k < nst && g + k < ngroup; k++) {
+          if (only >= 0 && k != only) continue;
           hipStream_t st = k ? c->stream2 : c->stream;
           float* part = (k ? c->partial2 : c->partial).as<float>();
           if ((e = enqueue_slice(r, (g + k) * r.B, s, st, part, &lm)) != hipSuccess) return e;
         }
-    if (nst == 2) {  // join: the first stream waits for the second
-      if ((e = hipEventRecord(c->evjoin, c->stream2)) != hipSuccess) return e;
-      if ((e = hipStreamWaitEvent(c->stream, c->evjoin, 0)) != hipSuccess) return e;
-    }
     return hipGetLastError();
   };
   if (c->use_graph && !c->profiling) {
@@ -1204,27 +1207,42 @@ int qs_run_stem_cfg(qs_ctx* c, const int* ipos, const int* config, int nprobe, i
                                   (long long)c->probe.p, c->rows_per_wave, c->tiles_per_block};
     for (int s = 0; s < nslice; s++) key.push_back(bin_of_slice ? bin_of_slice[s] : -1);
     if (!c->gexec || key != c->gkey) {
+      // one graph per stream, each the linear chain of that stream's launches (measured
+      // faster than one graph with two parallel branches)
       drop_graph(c);
-      HIPCHK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-      const hipError_t le = enqueue();
-      hipGraph_t g = nullptr;
-      const hipError_t ce = hipStreamEndCapture(c->stream, &g);
-      if (le != hipSuccess || ce != hipSuccess) {
-        if (g) (void)hipGraphDestroy(g);
-        return fail(c, QS_ERR_HIP, std::string("qs_run_stem: graph capture failed: ") +
-                                       hipGetErrorString(le != hipSuccess ? le : ce));
-      }
-      const hipError_t ie = hipGraphInstantiate(&c->gexec, g, nullptr, nullptr, 0);
-      (void)hipGraphDestroy(g);
-      if (ie != hipSuccess) {
-        c->gexec = nullptr;
-        return fail(c, QS_ERR_HIP, std::string("qs_run_stem: hipGraphInstantiate: ") + hipGetErrorString(ie));
+      for (int k = 0; k < nst; k++) {
+        hipStream_t st = k ? c->stream2 : c->stream;
+        hipGraphExec_t& ge = k ? c->gexec2 : c->gexec;
+        HIPCHK(c, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        const hipError_t le = enqueue(k);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(st, &g);
+        if (le != hipSuccess || ce != hipSuccess) {
+          if (g) (void)hipGraphDestroy(g);
+          drop_graph(c);
+          return fail(c, QS_ERR_HIP, std::string("qs_run_stem: graph capture failed: ") +
+                                         hipGetErrorString(le != hipSuccess ? le : ce));
+        }
+        const hipError_t ie = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) {
+          ge = nullptr;
+          drop_graph(c);
+          return fail(c, QS_ERR_HIP, std::string("qs_run_stem: hipGraphInstantiate: ") + hipGetErrorString(ie));
+        }
       }
       c->gkey = key;
     }
+    if (nst == 2) HIPCHK(c, fork());
     HIPCHK(c, hipGraphLaunch(c->gexec, c->stream));
+    if (nst == 2) {
+      HIPCHK(c, hipGraphLaunch(c->gexec2, c->stream2));
+      HIPCHK(c, join());
+    }
   } else {
-    HIPCHK(c, enqueue());
+    if (nst == 2) HIPCHK(c, fork());
+    HIPCHK(c, enqueue(-1));
+    if (nst == 2) HIPCHK(c, join());
   }
   HIPCHK(c, hipEventRecord(c->ev1, c->stream));
   rc = finish_run(r, c->stream);
