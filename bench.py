@@ -337,23 +337,28 @@ def main():
         return
     if a.workload == "copyrate":
         # plain device-to-device copy rate (bytes read + written per second) at 512 MiB
-        # and at the 184.5 MB of one 128-probe 512^2 state, which the Infinity Cache holds
+        # and at the 184.5 MB of one 128-probe 512^2 state, and an in-place pass (read and
+        # write the same buffer, as the sweeps do) over the 184.5 MB, which the 256 MB
+        # Infinity Cache holds
         import torch
         x = torch.ones(2 ** 27, device="cuda")
         y = torch.empty_like(x)
         rates = {}
-        for nbytes in (2 ** 29, 128 * 512 * 352 * 8):
+        state_bytes = 128 * 512 * 352 * 8
+        for name, nbytes, op in (("copy_536870912", 2 ** 29, "copy"), ("copy_%d" % state_bytes, state_bytes, "copy"),
+                                 ("inplace_%d" % state_bytes, state_bytes, "inplace")):
             xs, ys = x[: nbytes // 4], y[: nbytes // 4]
+            run = (lambda: ys.copy_(xs)) if op == "copy" else (lambda: xs.mul_(1.0))
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
             for _ in range(3):
-                ys.copy_(xs)
+                run()
             e0.record()
             for _ in range(20):
-                ys.copy_(xs)
+                run()
             e1.record()
             torch.cuda.synchronize()
-            rates[str(nbytes)] = round(2 * nbytes * 20 / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
-        print(json.dumps({"workload": "copyrate", "copy_GBps_read_plus_write": rates}), flush=True)
+            rates[name] = round(2 * nbytes * 20 / (e0.elapsed_time(e1) * 1e-3) / 1e9, 1)
+        print(json.dumps({"workload": "copyrate", "GBps_read_plus_write": rates}), flush=True)
         return
     if a.workload == "tem":
         print(json.dumps(tem_workload(nslice=a.tem_slices)), flush=True)
