@@ -263,6 +263,22 @@ struct TileStage {
       }
     }
   }
+  // the same rows as in(), into registers: q[i] = columns c, c+1 of row x0 + XS*i
+  __device__ __forceinline__ void in_regs(float4* q, int c0) const {
+    const unsigned co = (unsigned)c0 * 8u;
+#pragma unroll
+    for (int i = 0; i < IT; i++)
+      q[i] = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rs, voff + co, i * rstep, 0));
+  }
+  // columns c, c+1 of row x0 + XS*i from u[i], w[i]
+  __device__ __forceinline__ void out_regs(const float2* u, const float2* w, int c0) const {
+    const unsigned co = (unsigned)c0 * 8u;
+    using U4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(rs, 0u, 0, 0));
+#pragma unroll
+    for (int i = 0; i < IT; i++)
+      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, make_float4(u[i].x, u[i].y, w[i].x, w[i].y)), rs,
+                                             voff + co, i * rstep, 0);
+  }
   __device__ __forceinline__ void out(const float2* tile, int c0) const {
     const unsigned co = (unsigned)c0 * 8u;
     using U4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(rs, 0u, 0, 0));
@@ -278,6 +294,18 @@ struct TileStage {
     }
   }
 };
+
+// Column sweeps whose tile staging hands each thread exactly the elements t + T*m
+// of both tile columns (two columns per tile, one transform per block: 4096): the
+// loads go straight into the transform's registers and the results straight back
+// to memory, and the block needs a single LDS column slot for the exchanges (used
+// for the launches without detector sums or diffraction patterns).
+template <int NX>
+constexpr bool col_direct() {
+  using S = TileStage<NX>;
+  constexpr int T = NX / Plan<NX>::E;
+  return S::H == 1 && ColCfg<NX>::TC == 2 && ColCfg<NX>::BLK == T && S::XS == T && S::IT == Plan<NX>::E;
+}
 
 // Work item of block blockIdx.x such that work items GS*j ... GS*j+GS-1 (the
 // tiles that share the state's 128-byte lines when a block stages one tile)
@@ -339,6 +367,45 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_sweep_cols(SweepArgs a) {
   }
   auto kx2_of = [&](int m) -> float { return PXR ? kx2r[m] : a.kx2[t + T * m]; };
   auto px_of = [&](int m) -> float2 { return PXR ? pxr[m] : a.px[t + T * m]; };
+  if constexpr (col_direct<NX>() && !REC && !DIFF) {
+    // column c of the tile in v (element t + T*m in v[m]), LDS slot `tile` for the
+    // exchanges: column FFT, propagator and band mask, column IFFT
+    auto column = [&](float2 (&v)[E], int c) __attribute__((always_inline)) {
+      const int iy = c <= a.band_a ? c : (c < a.kc ? c - (a.band_a + 1) + (a.ny - a.band_b) : -1);
+      fft_wave<NX, -1, false, 0, ~GAP, ColCfg<NX>::TWC>(v, tile, tw, t);
+      if (iy >= 0) {
+        const float ky2 = a.ky2[iy];
+        const float2 py = a.py[iy];
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+          const float kx2 = kx2_of(m);
+          if (!((GAP >> m) & 1) && kx2 < a.k2max && (kx2 + ky2) < a.k2max) v[m] = cmul(cmul(v[m], py), px_of(m));
+          else v[m] = make_float2(0.f, 0.f);
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
+      }
+      fft_wave<NX, +1, false, GAP, -1, ColCfg<NX>::TWC>(v, tile, tw, t);
+    };
+    for (int tl = tile_first; tl < tile_end; tl++) {
+      if (tl > tile_first) __syncthreads();  // the previous tile's last slot reads are done
+      const int c0 = tl * TC;
+      float4 q[E];
+      stage.in_regs(q, c0);
+      float2 u[E], w[E];
+#pragma unroll
+      for (int m = 0; m < E; m++) {
+        u[m] = make_float2(q[m].x, q[m].y);
+        w[m] = make_float2(q[m].z, q[m].w);
+      }
+      column(u, c0);
+      xform_sync<NX>();  // the second column reuses the slot
+      column(w, c0 + 1);
+      stage.out_regs(u, w, c0);
+    }
+    return;
+  }
   for (int tl = tile_first; tl < tile_end; tl++) {
     if (tl > tile_first) __syncthreads();  // previous tile's LDS reads (stage-out, red) are done
     stage.in(tile, tl * TC);
@@ -534,7 +601,7 @@ constexpr int col_lds_bytes_t() { return (int)sizeof(float2) * ColCfg<N>::TC * C
 // 512^2: more tiles per block only lengthen each block's serial phases).
 template <int N>
 void launch_cols_t(const SweepArgs& a0, bool rec, bool diff, hipStream_t st) {
-  const size_t lds = col_lds_bytes_t<N>();
+  const size_t lds = col_direct<N>() && !rec && !diff ? sizeof(float2) * ColCfg<N>::STRIDE : col_lds_bytes_t<N>();
   SweepArgs a = a0;
   if (a.tpb <= 0) a.tpb = 1;
   dim3 grid((unsigned)(a.nprobe * ((a.ntiles + a.tpb - 1) / a.tpb)));
