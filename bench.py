@@ -145,13 +145,15 @@ def uneven_workload(nconfig=16, steps=3, streams=0):
             "adf_mean_per_size": {str(n): float(np.mean(o["det"][-1, 1])) for n, o in zip(ctxs, outs)}}
 
 
-def tem_workload(nslice=200, ckpt=14):
+def tem_workload(nslice=200):
     """Large TEM with the checkpointed adjoint: plane wave on a 4096^2 grid at
     dx = 390.5/4096 A (100 x 100 perovskite cells), 300 kV, 200 slices of a/2
     (synthetic projected potential of the two perovskite slice types, as in the
     batched workload); forward FP32 with the exit wave, then the adjoint of
-    L = sum (I - target)^2 with checkpoints every 14 slices.  Reports device times,
-    their ratio and the library's peak device memory per phase."""
+    L = sum (I - target)^2 with the library's checkpoint interval (every slice kept
+    when the states fit in half the free device memory), and with checkpoints every
+    ceil(sqrt(slices)) slices (least memory).  Reports device times, their ratio and the
+    library's peak device memory per phase."""
     import torch
     from qstem_amd import Multislice, StemConfig, device_memory
     n, a = 4096, 3.905
@@ -187,25 +189,37 @@ def tem_workload(nslice=200, ckpt=14):
         I = np.abs(np.fft.fft2(ex[0].astype(np.complex128)) / (n * n)) ** 2
         target = (0.9 * I).astype(np.float32)[None]
         torch.cuda.synchronize()
-        ad = []
-        for _ in range(2):
-            t0 = time.perf_counter()
-            out = ms.adjoint(ipos, target, ckpt_every=ckpt, want_grad_V=True, want_grad_probe=False)
-            torch.cuda.synchronize()
-            ad.append(time.perf_counter() - t0)
-            adm = ms.last_run_ms
+
+        def adjoint_runs(k):
+            wall = []
+            for _ in range(2):
+                t0 = time.perf_counter()
+                o = ms.adjoint(ipos, target, ckpt_every=k, want_grad_V=True, want_grad_probe=False)
+                torch.cuda.synchronize()
+                wall.append(time.perf_counter() - t0)
+            return o, ms.last_run_ms, min(wall), ms.last_ckpt_interval
+
+        out, adm, adw, k_def = adjoint_runs(0)
         _, peak_adj = device_memory(reset_peak=True)
         # the same adjoint in its two halves (forward pass kept, then the reverse pass)
-        ms.forward_save(ipos, target, ckpt_every=ckpt)
+        ms.forward_save(ipos, target)
         fsm = ms.last_run_ms
         ms.adjoint_saved(want_grad_V=True, want_grad_probe=False)
         rvm = ms.last_run_ms
+        _, adm_sq, _, k_sq = adjoint_runs(int(np.ceil(np.sqrt(nslice))))
+        ms.forward_save(ipos, target, ckpt_every=k_sq)
+        fsm_sq = ms.last_run_ms
+        ms.adjoint_saved(want_grad_V=True, want_grad_probe=False)
+        rvm_sq = ms.last_run_ms
     f_ms = min(fw)
     return {"workload": "TEM plane wave 4096^2, perovskite 100x100 cells, %d slices, 300 kV" % nslice,
             "forward_ms": round(f_ms, 2), "adjoint_ms": round(adm, 2), "adjoint_over_forward": round(adm / f_ms, 3),
             "forward_save_ms": round(fsm, 2), "reverse_ms": round(rvm, 2),
             "reverse_over_forward": round(rvm / f_ms, 3),
-            "adjoint_wall_s": round(min(ad), 3), "checkpoint_every": ckpt, "loss": out["loss"],
+            "adjoint_wall_s": round(adw, 3), "checkpoint_every": k_def, "loss": out["loss"],
+            "sqrt_checkpoints": {"checkpoint_every": k_sq, "adjoint_ms": round(adm_sq, 2),
+                                 "adjoint_over_forward": round(adm_sq / f_ms, 3), "forward_save_ms": round(fsm_sq, 2),
+                                 "reverse_ms": round(rvm_sq, 2), "reverse_over_forward": round(rvm_sq / f_ms, 3)},
             "exit_norm": float(np.sum(np.abs(ex[0]) ** 2) / (n * n)),
             "peak_device_GB": {"potential_setup": round(peak_setup / 1e9, 2), "forward": round(peak_fwd / 1e9, 2),
                                "adjoint": round(peak_adj / 1e9, 2)},
@@ -247,19 +261,21 @@ def adjoint_workload(steps=3):
         ms.run_stem(ipos, want_det=False, want_diff=True)
     torch.cuda.synchronize()
     fwd = (time.perf_counter() - t0) / steps
-    res = {}
-    for k_ck in (0, 1):
+    res, kused = {}, {}
+    k_sq = int(np.ceil(np.sqrt(S)))
+    for k_ck in (0, k_sq):  # the library's interval, then ceil(sqrt(S)) (least memory)
         t0 = time.perf_counter()
         for _ in range(steps):
             out = ms.adjoint(ipos, target, ckpt_every=k_ck)
         torch.cuda.synchronize()
         res[k_ck] = (time.perf_counter() - t0) / steps
+        kused[k_ck] = ms.last_ckpt_interval
     ms.close()
     return {"workload": "4D-STEM 32x32 scan, 256x256, 20 slices, L2 on diffraction intensities",
             "forward_ms": round(fwd * 1e3, 2), "adjoint_ms": round(res[0] * 1e3, 2),
-            "adjoint_over_forward": round(res[0] / fwd, 3), "ckpt_every": int(np.ceil(np.sqrt(S))),
-            "adjoint_ms_no_recompute": round(res[1] * 1e3, 2),
-            "adjoint_over_forward_no_recompute": round(res[1] / fwd, 3),
+            "adjoint_over_forward": round(res[0] / fwd, 3), "ckpt_every": kused[0],
+            "adjoint_ms_sqrt_ckpt": round(res[k_sq] * 1e3, 2), "sqrt_ckpt_every": k_sq,
+            "adjoint_over_forward_sqrt_ckpt": round(res[k_sq] / fwd, 3),
             "note": "adjoint = one call returning loss, dL/dV, dL/d(df, Cs): forward with checkpoints, "
                     "segment recompute, reverse sweeps"}
 

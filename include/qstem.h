@@ -203,7 +203,10 @@ int qs_run_groups(const qs_group *groups, int ngroup, const int *bin_of_slice, i
  * I_p(k) = |Psi_p(k) P(k)|^2/(nx ny)^2 is the diffraction intensity after the last slice
  * (fraction of the incident beam per pixel, FFTW order [nprobe][nx][ny]).  All
  * transmission slices are used, starting from the probe.  The forward waves are kept
- * every ckpt_every slices (0: ceil(sqrt(nslice))) and recomputed inside each segment.
+ * every ckpt_every slices and recomputed inside each segment; ckpt_every = 0 lets the
+ * library choose the smallest interval K whose ceil(nslice/K) + K wave states fit in
+ * half the free device memory (K = 1: every slice kept, nothing recomputed), at most
+ * ceil(sqrt(nslice)); qs_last_ckpt_interval reports the interval used.
  * Outputs (nullable): loss; grad_V [nslice][potNx][potNy] = dL/dV (needs the potential
  * set by qs_set_potential or qs_build_potential); grad_probe[2] = dL/d(df0), dL/d(Cs) in
  * the units of qs_params (Angstrom).  Device memory: ceil(nslice/K) + K wave states
@@ -223,6 +226,9 @@ int qs_adjoint(qs_ctx *ctx, const int *ipos, int nprobe, const float *target, in
  * allowed).  qs_last_run_ms reports each half's device time.  No reference
  * counterpart. */
 int qs_forward_save(qs_ctx *ctx, const int *ipos, int nprobe, const float *target, int ckpt_every, float *loss);
+/* Checkpoint interval of the last qs_adjoint / qs_forward_save (0 before the first).
+ * No reference counterpart. */
+int qs_last_ckpt_interval(const qs_ctx *ctx);
 int qs_adjoint_saved(qs_ctx *ctx, float *grad_V, float *grad_probe);
 /* dL/d(x, y, z) of the atoms of the last qs_build_potential (same atoms, same pp), from the
  * grad_V of the last qs_adjoint; z is 0 (the potential samples z by nearest slice). */

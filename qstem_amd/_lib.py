@@ -100,6 +100,7 @@ SIGNATURES = {
     "qs_get_batch": (_I, [_P]),
     "qs_set_streams": (_I, [_P, _I]),
     "qs_get_streams": (_I, [_P]),
+    "qs_last_ckpt_interval": (_I, [_P]),
     "qs_set_graph": (_I, [_P, _I]),
     "qs_set_tuning": (_I, [_P, _I, _I]),
     "qs_get_tuning": (_I, [_P, _IP, _IP, _IP]),

@@ -1381,6 +1381,20 @@ static qs::AdjArgs adj_args(qs_ctx* c, int nprobe) {
 // ckpt[j], written in place by the row sweep that produces it; X_{S-1} in
 // c->state), the reverse column sweep of the last slice (loss partials and the
 // adjoint seed in gx) and the loss.
+// Default checkpoint interval K for S slices of st_bytes wave states: the smallest K
+// whose ceil(S/K) + K states fit in half the device memory that is free (counting
+// this context's checkpoint buffers, which are reused) -- K = 1 keeps every slice and
+// recomputes nothing -- and at most ceil(sqrt(S)), the interval of least memory.
+static int ckpt_interval(const qs_ctx* c, int S, size_t st_bytes) {
+  const int kmax = (int)ceil(sqrt((double)S));
+  size_t free_b = 0, total_b = 0;
+  if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return kmax;
+  for (const DevBuf& b : c->ckpt) free_b += b.bytes;
+  for (int K = 1; K < kmax; K++)
+    if ((size_t)((S + K - 1) / K + K) * st_bytes <= free_b / 2) return K;
+  return kmax;
+}
+
 static int adj_forward(qs_ctx* c, const char* who, const int* ipos, int nprobe, const float* target, int ckpt_every,
                        float* loss_out) {
   if (!c) return QS_ERR_ARG;
@@ -1398,10 +1412,10 @@ static int adj_forward(qs_ctx* c, const char* who, const int* ipos, int nprobe, 
     if (x < 0 || y < 0 || x + p.nx > p.potNx || y + p.ny > p.potNy)
       return fail(c, QS_ERR_ARG, std::string(who) + ": window offset outside the transmission array");
   }
-  const int K = ckpt_every > 0 ? ckpt_every : (int)ceil(sqrt((double)S));
-  const int nseg = (S + K - 1) / K;
   const size_t st_elems = (size_t)nprobe * p.nx * c->kcp;
   const size_t st_bytes = sizeof(float2) * st_elems;
+  const int K = ckpt_every > 0 ? std::min(ckpt_every, S) : ckpt_interval(c, S, st_bytes);
+  const int nseg = (S + K - 1) / K;
   const size_t slice_elems = (size_t)p.potNx * p.potNy;
   HIPCHK(c, c->state.ensure(st_bytes));
   HIPCHK(c, c->gx.ensure(st_bytes));
@@ -1557,6 +1571,8 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
   }
   return QS_OK;
 }
+
+int qs_last_ckpt_interval(const qs_ctx* c) { return c ? c->adj_K : QS_ERR_ARG; }
 
 int qs_forward_save(qs_ctx* c, const int* ipos, int nprobe, const float* target, int ckpt_every, float* loss) {
   return adj_forward(c, "qs_forward_save", ipos, nprobe, target, ckpt_every, loss);

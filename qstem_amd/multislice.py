@@ -345,6 +345,11 @@ class Multislice:
     def last_run_ms(self) -> float:
         return float(self._lib.qs_last_run_ms(self._h))
 
+    @property
+    def last_ckpt_interval(self) -> int:
+        """Checkpoint interval used by the last adjoint / forward_save."""
+        return int(self._lib.qs_last_ckpt_interval(self._h))
+
     def set_configs(self, nconfig: int):
         """Hold nconfig frozen-phonon transmission functions (reload all, configuration 0 first)."""
         self._check(self._lib.qs_set_configs(self._h, int(nconfig)), "qs_set_configs")

@@ -77,8 +77,10 @@ def test_checkpoint_interval_does_not_change_gradients():
     with Multislice(cfg) as ms:
         ms.set_potential(V)
         ms.make_probe()
-        for k in (1, 2, 0, 7):
+        for k in (1, 2, 0, 7, 9):
             res.append(ms.adjoint(ipos, target, ckpt_every=k))
+            # 0: the library's choice, every slice kept when the states fit in memory
+            assert ms.last_ckpt_interval == (min(k, cfg.slices) if k > 0 else 1), k
     for r in res[1:]:
         assert r["loss"] == res[0]["loss"]
         g0 = res[0]["grad_V"]
