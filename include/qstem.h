@@ -240,9 +240,11 @@ int qs_set_batch(qs_ctx *ctx, int batch);
 int qs_get_batch(const qs_ctx *ctx);
 
 /* Streams of qs_run_stem and qs_run_groups: 2 (default) lets consecutive probe
- * groups (qs_run_groups: contexts) run on two HIP streams, so that one group's
- * launches fill the other's ramp-up and drain; 1 keeps every launch on the
- * context stream.  Results do not depend on it.  No reference counterpart. */
+ * groups (qs_run_groups: contexts, by the setting of the first group's context)
+ * run on two HIP streams, so that one group's launches fill the other's ramp-up
+ * and drain; 1 keeps every launch on the context stream.  Each call waits for
+ * both streams before it returns.  Results do not depend on it.  No reference
+ * counterpart. */
 int qs_set_streams(qs_ctx *ctx, int nstreams);
 int qs_get_streams(const qs_ctx *ctx);
 
