@@ -130,6 +130,7 @@ struct qs_ctx {
   int nstreams = 2;
   hipStream_t stream2 = nullptr;
   hipEvent_t evfork = nullptr, evjoin = nullptr;
+  hipEvent_t evgt[2] = {nullptr, nullptr}, evbl[2] = {nullptr, nullptr};  // adjoint: dL/dt buffers ready / consumed
   float last_ms = 0.f;
   std::string err;
   int batch = 0;
@@ -163,7 +164,7 @@ struct qs_ctx {
     bool on = false;
     int band_a = 0, band_b = 0, kc = 0, kcp = 0, ntiles = 0;
   } bl;
-  DevBuf blpx, blpy, blstate;
+  DevBuf blpx, blpy, blstate, blstate2;
   // potential builder
   std::vector<double> scat;  // 93 x 123 Rez table
   DevBuf atoms, kinds, lutptrs, spl, luttmp, lutscr;
@@ -177,7 +178,7 @@ struct qs_ctx {
   bool have_probe = false;
   // adjoint
   std::vector<DevBuf> ckpt;
-  DevBuf gx, target, lossp, gT, gprobe, gV, dsum, adj_ipos;
+  DevBuf gx, target, lossp, gT, gT2, gprobe, gV, dsum, adj_ipos;
   // forward pass kept by qs_forward_save for qs_adjoint_saved (checkpoints, the
   // adjoint seed in gx, the loss); cleared by every change of potential, probe or
   // optics (with the graph) and by the reverse pass itself
@@ -291,7 +292,11 @@ int qs_create(const qs_params* p, qs_ctx** out) {
   if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
       hipEventCreate(&c->ev1) != hipSuccess || hipStreamCreate(&c->stream2) != hipSuccess ||
       hipEventCreateWithFlags(&c->evfork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evjoin, hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->evjoin, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evgt[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evgt[1], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evbl[0], hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&c->evbl[1], hipEventDisableTiming) != hipSuccess) {
     c->err = "hipStreamCreate/hipEventCreate failed (no HIP device?)";
     return bad(QS_ERR_HIP);
   }
@@ -445,8 +450,8 @@ void qs_destroy(qs_ctx* c) {
                     &c->atoms, &c->kinds, &c->lutptrs, &c->spl, &c->luttmp, &c->lutscr,
                     &c->lx.tw, &c->lx.chirp_f, &c->lx.vhat_f, &c->lx.chirp_b, &c->lx.vhat_b,
                     &c->lz.tw, &c->lz.chirp_f, &c->lz.vhat_f, &c->lz.chirp_b, &c->lz.vhat_b,
-                    &c->gx, &c->target, &c->lossp, &c->gT, &c->gprobe, &c->gV, &c->dsum,
-                    &c->blpx, &c->blpy, &c->blstate, &c->adj_ipos};
+                    &c->gx, &c->target, &c->lossp, &c->gT, &c->gT2, &c->gprobe, &c->gV, &c->dsum,
+                    &c->blpx, &c->blpy, &c->blstate, &c->blstate2, &c->adj_ipos};
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : c->luts) b.release();
   for (DevBuf& b : c->ckpt) b.release();
@@ -455,6 +460,10 @@ void qs_destroy(qs_ctx* c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->evfork) (void)hipEventDestroy(c->evfork);
   if (c->evjoin) (void)hipEventDestroy(c->evjoin);
+  for (int k = 0; k < 2; k++) {
+    if (c->evgt[k]) (void)hipEventDestroy(c->evgt[k]);
+    if (c->evbl[k]) (void)hipEventDestroy(c->evbl[k]);
+  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   delete c;
@@ -471,13 +480,16 @@ static constexpr size_t kBandScratchBytes = size_t(1) << 30;
 // FFT / propagator 1/(nx ny) inside the disk k^2 < k2max / column IFFT, row IFFT
 // written back to `tr` (rows mode 2) or, with gv given, turned into
 // dL/dV = sigma Im(g conj(exp(i sigma V))) of the slices' potential vpot (mode 4).
-static int bl_sweeps(qs_ctx* c, float2* tr, int nslice, const float* vpot, float* gv, float sigma) {
+static int bl_sweeps(qs_ctx* c, float2* tr, int nslice, const float* vpot, float* gv, float sigma,
+                     hipStream_t st = nullptr, DevBuf* scratch = nullptr) {
   const qs_params& p = c->p;
   const int nx = p.potNx, ny = p.potNy;
   const size_t slice = (size_t)nx * ny;
   const size_t per = sizeof(float2) * (size_t)nx * c->bl.kcp;
   const int q = (int)std::max<size_t>(1, std::min<size_t>((size_t)nslice, kBandScratchBytes / per));
-  HIPCHK(c, c->blstate.ensure(per * q));
+  if (!st) st = c->stream;
+  if (!scratch) scratch = &c->blstate;
+  HIPCHK(c, scratch->ensure(per * q));
   qs::SweepArgs a{};
   a.nx = nx;
   a.ny = ny;
@@ -495,7 +507,7 @@ static int bl_sweeps(qs_ctx* c, float2* tr, int nslice, const float* vpot, float
   a.ky2 = c->pky2.as<float>();
   a.k2max = bl_k2max(p);
   a.rec_bin = -1;
-  a.state = c->blstate.as<float2>();
+  a.state = scratch->as<float2>();
   a.gv_sigma = sigma;
   for (int s0 = 0; s0 < nslice; s0 += q) {
     a.nprobe = std::min(q, nslice - s0);
@@ -505,9 +517,9 @@ static int bl_sweeps(qs_ctx* c, float2* tr, int nslice, const float* vpot, float
       a.vpot = vpot + (size_t)s0 * slice;
       a.gv = gv + (size_t)s0 * slice;
     }
-    qs::launch_sweep_rows(a, 3, c->stream);
-    qs::launch_sweep_cols(a, false, false, c->stream);
-    qs::launch_sweep_rows(a, gv ? 4 : 2, c->stream);
+    qs::launch_sweep_rows(a, 3, st);
+    qs::launch_sweep_cols(a, false, false, st);
+    qs::launch_sweep_rows(a, gv ? 4 : 2, st);
   }
   HIPCHK(c, hipGetLastError());
   return QS_OK;
@@ -1499,6 +1511,12 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
   const float mm0 = 1.0F + p.v0 / 511.0F;
   const double sigma = mm0 * wavelength(p.v0);
   HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * gslices));
+  // band-limited dL/dV slice by slice: the band limit of slice s runs on the second
+  // stream while the reverse sweeps of slice s-1 run on the first, from two dL/dt
+  // buffers used in turn
+  const bool overlap = grad_each && p.bandlimittrans && c->nstreams > 1 && S > 1;
+  if (overlap) HIPCHK(c, c->gT2.ensure(sizeof(float2) * slice_elems));
+  bool pending[2] = {false, false};  // a band limit on the second stream still reads buffer k
   if (grad_V) HIPCHK(c, c->gV.ensure(sizeof(float) * slice_elems * S));
   HIPCHK(c, c->gprobe.ensure(sizeof(float2) * (size_t)p.nx * p.ny));
   qs::AdjArgs A = adj_args(c, nprobe);
@@ -1521,17 +1539,27 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
       prev = w.state;
     }
     for (int s = s1; s >= s0; s--) {
+      const int b = overlap ? (s & 1) : 0;
       A.f.trans = trans_base(c) + (size_t)s * slice_elems;
-      A.gT = c->gT.as<float>() + (gslices > 1 ? 2 * (size_t)s * slice_elems : 0);
+      A.gT = (b ? c->gT2 : c->gT).as<float>() + (gslices > 1 ? 2 * (size_t)s * slice_elems : 0);
       A.xprev = nullptr;
       if (s < S - 1) qs::launch_adj_cols(A, false, c->stream);
       if (s > 0) A.xprev = (s - 1 >= s0) ? c->ckpt[nseg + (s - 1 - s0)].as<float2>() : c->ckpt[j].as<float2>();
+      if (pending[b]) HIPCHK(c, hipStreamWaitEvent(c->stream, c->evbl[b], 0));  // slice s+2's band limit read it
       if (grad_each && !whole) qs::launch_cplx_fill_zero(A.gT, 2 * slice_elems, c->stream);
       qs::launch_adj_rows(A, s == 0, c->stream);
       if (grad_each) {
         const float* vs = c->V.as<float>() + (size_t)s * slice_elems;
         float* gs = c->gV.as<float>() + (size_t)s * slice_elems;
-        if (p.bandlimittrans) {
+        if (overlap) {
+          HIPCHK(c, hipEventRecord(c->evgt[b], c->stream));
+          HIPCHK(c, hipStreamWaitEvent(c->stream2, c->evgt[b], 0));
+          const int rc = bl_sweeps(c, reinterpret_cast<float2*>(A.gT), 1, vs, gs, (float)sigma, c->stream2,
+                                   b ? &c->blstate2 : &c->blstate);
+          if (rc != QS_OK) return rc;
+          HIPCHK(c, hipEventRecord(c->evbl[b], c->stream2));
+          pending[b] = true;
+        } else if (p.bandlimittrans) {
           const int rc = bl_sweeps(c, reinterpret_cast<float2*>(A.gT), 1, vs, gs, (float)sigma);
           if (rc != QS_OK) return rc;
         } else {
@@ -1541,6 +1569,10 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
     }
   }
   HIPCHK(c, hipGetLastError());
+  if (overlap) {  // join: the first stream waits for the last band limits
+    HIPCHK(c, hipEventRecord(c->evjoin, c->stream2));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->evjoin, 0));
+  }
   // dL/dV of every slice at once: band limit (self-adjoint) of dL/dt, then sigma Im(g conj(u))
   if (grad_V && !grad_each) {
     int rc = bandlimit_buf(c, c->gT.as<float2>(), S);
