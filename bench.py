@@ -145,15 +145,17 @@ def uneven_workload(nconfig=16, steps=3, streams=0):
             "adf_mean_per_size": {str(n): float(np.mean(o["det"][-1, 1])) for n, o in zip(ctxs, outs)}}
 
 
-def tem_workload(nslice=200):
+def tem_workload(nslice=200, streams=0):
     """Large TEM with the checkpointed adjoint: plane wave on a 4096^2 grid at
     dx = 390.5/4096 A (100 x 100 perovskite cells), 300 kV, 200 slices of a/2
     (synthetic projected potential of the two perovskite slice types, as in the
     batched workload); forward FP32 with the exit wave, then the adjoint of
     L = sum (I - target)^2 with the library's checkpoint interval (every slice kept
     when the states fit in half the free device memory), and with checkpoints every
-    ceil(sqrt(slices)) slices (least memory).  Reports device times, their ratio and the
-    library's peak device memory per phase."""
+    ceil(sqrt(slices)) slices (least memory); streams: 0 = library default, else 1 or 2
+    HIP streams (with two, the band limit of dL/dt overlaps the next slice's reverse
+    sweeps).  Reports device times, their ratio and the library's peak device memory
+    per phase."""
     import torch
     from qstem_amd import Multislice, StemConfig, device_memory
     n, a = 4096, 3.905
@@ -173,6 +175,8 @@ def tem_workload(nslice=200):
         V[sl::2] = P
     ipos = np.zeros((1, 2), np.int32)
     with Multislice(cfg) as ms:
+        if streams:
+            ms.set_streams(streams)
         device_memory(reset_peak=True)
         t0 = time.perf_counter()
         ms.set_potential(V)
@@ -207,6 +211,7 @@ def tem_workload(nslice=200):
         ms.adjoint_saved(want_grad_V=True, want_grad_probe=False)
         rvm = ms.last_run_ms
         _, adm_sq, _, k_sq = adjoint_runs(int(np.ceil(np.sqrt(nslice))))
+        ms_streams = ms.get_streams()
         ms.forward_save(ipos, target, ckpt_every=k_sq)
         fsm_sq = ms.last_run_ms
         ms.adjoint_saved(want_grad_V=True, want_grad_probe=False)
@@ -216,7 +221,7 @@ def tem_workload(nslice=200):
             "forward_ms": round(f_ms, 2), "adjoint_ms": round(adm, 2), "adjoint_over_forward": round(adm / f_ms, 3),
             "forward_save_ms": round(fsm, 2), "reverse_ms": round(rvm, 2),
             "reverse_over_forward": round(rvm / f_ms, 3),
-            "adjoint_wall_s": round(adw, 3), "checkpoint_every": k_def, "loss": out["loss"],
+            "adjoint_wall_s": round(adw, 3), "checkpoint_every": k_def, "streams": ms_streams, "loss": out["loss"],
             "sqrt_checkpoints": {"checkpoint_every": k_sq, "adjoint_ms": round(adm_sq, 2),
                                  "adjoint_over_forward": round(adm_sq / f_ms, 3), "forward_save_ms": round(fsm_sq, 2),
                                  "reverse_ms": round(rvm_sq, 2), "reverse_over_forward": round(rvm_sq / f_ms, 3)},
@@ -377,7 +382,7 @@ def main():
         print(json.dumps({"workload": "copyrate", "GBps_read_plus_write": rates}), flush=True)
         return
     if a.workload == "tem":
-        print(json.dumps(tem_workload(nslice=a.tem_slices)), flush=True)
+        print(json.dumps(tem_workload(nslice=a.tem_slices, streams=a.streams)), flush=True)
         return
 
     from qstem_amd import Multislice, record_bins
