@@ -168,6 +168,7 @@ struct qs_ctx {
   // potential builder
   std::vector<double> scat;  // 93 x 123 Rez table
   DevBuf atoms, kinds, lutptrs, spl, luttmp, lutscr;
+  DevBuf reach, tcount, tstart, tlist;  // gather form: per-atom reach, atoms binned by pixel tile
   std::vector<DevBuf> luts;
   RowXform lx, lz;  // LUT transform lengths
   int trans_slices = 0;
@@ -448,6 +449,7 @@ void qs_destroy(qs_ctx* c) {
                     &c->probe_partial, &c->rx.tw, &c->rx.chirp_f, &c->rx.vhat_f, &c->rx.chirp_b, &c->rx.vhat_b,
                     &c->ry.tw, &c->ry.chirp_f, &c->ry.vhat_f, &c->ry.chirp_b, &c->ry.vhat_b,
                     &c->atoms, &c->kinds, &c->lutptrs, &c->spl, &c->luttmp, &c->lutscr,
+                    &c->reach, &c->tcount, &c->tstart, &c->tlist,
                     &c->lx.tw, &c->lx.chirp_f, &c->lx.vhat_f, &c->lx.chirp_b, &c->lx.vhat_b,
                     &c->lz.tw, &c->lz.chirp_f, &c->lz.vhat_f, &c->lz.chirp_b, &c->lz.vhat_b,
                     &c->gx, &c->target, &c->lossp, &c->gT, &c->gT2, &c->gprobe, &c->gV, &c->dsum,
@@ -728,8 +730,40 @@ int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_
     sa.iRadX = (int)ceil(R / p.resolutionX); sa.iRadY = (int)ceil(R / p.resolutionY); sa.iRadZ = (int)ceil(R / dz);
     sa.Nr = Nr; sa.Nz = Nz; sa.nzSub = nzp;
     sa.dr = pp->lut_dr_fix ? p.resolutionX : p.resolutionX / 2;
-    qs::launch_scatter(c->atoms.as<qs::QsAtomDev>(), natom, c->lutptrs.as<const float*>(), c->kinds.as<int>(), sa,
-                       c->V.as<float>(), c->stream);
+    if (!pp->periodicXY) {
+      // gather: per-atom reach, count and list the atoms of each pixel tile, then one
+      // thread per voxel sums them (one store per voxel instead of an atomic per sample)
+      const int tiles_x = (p.potNx + qs::kPotTile - 1) / qs::kPotTile;
+      const int tiles_y = (p.potNy + qs::kPotTile - 1) / qs::kPotTile;
+      const int ntile = tiles_x * tiles_y;
+      HIPCHK(c, c->reach.ensure(sizeof(qs::AtomReach) * natom));
+      HIPCHK(c, c->tcount.ensure(sizeof(int) * ntile));
+      HIPCHK(c, c->tstart.ensure(sizeof(int) * (ntile + 1)));
+      qs::AtomReach* reach = c->reach.as<qs::AtomReach>();
+      qs::launch_atom_reach(c->atoms.as<qs::QsAtomDev>(), natom, c->kinds.as<int>(), sa, reach, c->stream);
+      HIPCHK(c, hipMemsetAsync(c->tcount.p, 0, sizeof(int) * ntile, c->stream));
+      qs::launch_bin_atoms(reach, natom, tiles_y, c->tcount.as<int>(), nullptr, false, c->stream);
+      std::vector<int> cnt(ntile), start(ntile + 1, 0);
+      HIPCHK(c, hipMemcpyAsync(cnt.data(), c->tcount.p, sizeof(int) * ntile, hipMemcpyDeviceToHost, c->stream));
+      HIPCHK(c, hipStreamSynchronize(c->stream));
+      long long total = 0;
+      for (int t = 0; t < ntile; t++) {
+        start[t] = (int)total;
+        total += cnt[t];
+        if (total > 0x7fffffffLL) return fail(c, QS_ERR_ARG, "qs_build_potential: too many atom-tile pairs");
+      }
+      start[ntile] = (int)total;
+      HIPCHK(c, c->tlist.ensure(sizeof(int) * (size_t)std::max<long long>(1, total)));
+      HIPCHK(c, hipMemcpyAsync(c->tstart.p, start.data(), sizeof(int) * (ntile + 1), hipMemcpyHostToDevice, c->stream));
+      HIPCHK(c, hipMemcpyAsync(c->tcount.p, start.data(), sizeof(int) * ntile, hipMemcpyHostToDevice, c->stream));
+      qs::launch_bin_atoms(reach, natom, tiles_y, c->tcount.as<int>(), c->tlist.as<int>(), true, c->stream);
+      qs::launch_gather(reach, c->tstart.as<int>(), c->tlist.as<int>(), c->lutptrs.as<const float*>(), sa,
+                        c->V.as<float>(), c->stream);
+      HIPCHK(c, hipStreamSynchronize(c->stream));  // the host arrays above are copied from
+    } else {
+      qs::launch_scatter(c->atoms.as<qs::QsAtomDev>(), natom, c->lutptrs.as<const float*>(), c->kinds.as<int>(), sa,
+                         c->V.as<float>(), c->stream);
+    }
   }
   HIPCHK(c, hipGetLastError());
   if (V_out) HIPCHK(c, hipMemcpyAsync(V_out, c->V.p, sizeof(float) * nV, hipMemcpyDeviceToHost, c->stream));

@@ -104,6 +104,24 @@ void launch_lut_extract(const float2* tmp, const LutArgs& g, float* lut, hipStre
 void launch_scatter(const QsAtomDev* atoms, int natom, const float* const* luts, const int* kind,
                     const ScatterArgs& s, float* V, hipStream_t st);
 
+// Gather form of the scatter for non-periodic boxes: per-atom reach, atoms binned by
+// the kPotTile x kPotTile pixel tiles their box reaches (tile tx * tiles_y + ty), and
+// one thread per pixel summing kPotZc slices.
+constexpr int kPotTile = 16, kPotZc = 16;
+struct AtomReach {
+  float atomX, atomY;          // position relative to the potential array (Angstrom)
+  int iax0, iax1, iay0, iay1;  // pixel box (inclusive)
+  int iAtomZ, iaz0, iaz1;      // slices iAtomZ + [iaz0, iaz1]; iAtomZ = INT_MIN: reaches nothing
+  int iOffsZ0;                 // LUT z offset at iaz0
+  int kind;                    // LUT index
+};
+void launch_atom_reach(const QsAtomDev* atoms, int natom, const int* kind, const ScatterArgs& s, AtomReach* reach,
+                       hipStream_t st);
+void launch_bin_atoms(const AtomReach* reach, int natom, int tiles_y, int* count_or_cursor, int* list, bool fill,
+                      hipStream_t st);
+void launch_gather(const AtomReach* reach, const int* tstart, const int* list, const float* const* luts,
+                   const ScatterArgs& s, float* V, hipStream_t st);
+
 bool supported_size(int n);
 int col_tile(int nx);
 

@@ -7,9 +7,12 @@
 //      Debye-Waller factor, summed over q_y (k_lut_fk, FP64),
 //   2. a 2D inverse DFT of the (kz, kx) map (Stockham/Bluestein row kernels),
 //   3. z-integration over nzPerSlice sub-samples, clamp >= 0, scale (k_lut_extract),
-//   4. one block per atom adds r-linear / z-nearest LUT samples into the slices
-//      it reaches (k_scatter), with float atomics.
+//   4. r-linear / z-nearest LUT samples of every atom summed into the slices it
+//      reaches: by pixel tile and slice block, one store per voxel (k_gather), or
+//      for periodic boxes one block per atom with float atomics (k_scatter).
 #include <hip/hip_runtime.h>
+
+#include <climits>
 
 #include "qs_internal.hpp"
 
@@ -84,54 +87,73 @@ __global__ void k_lut_extract(const float2* __restrict__ tmp, LutArgs g, float* 
 
 // One block per atom (stemlib.cpp:883-1213).  Float arithmetic follows the
 // reference statement by statement where it decides sample indices.
-__global__ __launch_bounds__(256) void k_scatter(const QsAtomDev* __restrict__ atoms, int natom,
-                                                 const float* const* __restrict__ luts,
-                                                 const int* __restrict__ kind, ScatterArgs s, float* __restrict__ V) {
-  const int ia = blockIdx.x;
-  if (ia >= natom) return;
-  const QsAtomDev at = atoms[ia];
-  if (at.Znum == 0) return;  // vacancy
+// Per-atom set-up of getAtomPotential3D: the pixel box, the slices reached and the
+// z offset into the LUT; false when the atom reaches no pixel or slice.
+__device__ __forceinline__ bool atom_setup(const QsAtomDev& at, const ScatterArgs& s, AtomReach& r) {
+  if (at.Znum == 0) return false;  // vacancy
   const float dzs = s.sliceThickness;
   const float c = dzs * s.slices;
   // :886-889, then :945 (the -dz and +dz are kept to reproduce float rounding)
   float atomZ = (float)((double)(at.z + s.czOffset) - (0.5 * (double)dzs * (double)(1 - s.centerSlices)));
   atomZ -= dzs;
-  if (atomZ - s.atomRadius > c) return;
-  if (atomZ + s.atomRadius + dzs < 0) return;
+  if (atomZ - s.atomRadius > c) return false;
+  if (atomZ + s.atomRadius + dzs < 0) return false;
   atomZ += dzs;
-  const float atomX = at.x - s.potOffsetX, atomY = at.y - s.potOffsetY;
-  const int iAtomX = (int)floor(atomX / s.dx), iAtomY = (int)floor(atomY / s.dy);
-  const int iAtomZ = (int)floor(atomZ / dzs + 0.5);
-  int iax0, iax1, iay0, iay1;
+  r.atomX = at.x - s.potOffsetX;
+  r.atomY = at.y - s.potOffsetY;
+  const int iAtomX = (int)floor(r.atomX / s.dx), iAtomY = (int)floor(r.atomY / s.dy);
+  r.iAtomZ = (int)floor(atomZ / dzs + 0.5);
   if (!s.periodic) {
-    iax0 = iAtomX - s.iRadX < 0 ? 0 : iAtomX - s.iRadX;
-    iax1 = iAtomX + s.iRadX >= s.potNx ? s.potNx - 1 : iAtomX + s.iRadX;
-    iay0 = iAtomY - s.iRadY < 0 ? 0 : iAtomY - s.iRadY;
-    iay1 = iAtomY + s.iRadY >= s.potNy ? s.potNy - 1 : iAtomY + s.iRadY;
-    if (!(iax0 < s.potNx && iax1 >= 0 && iay0 < s.potNy && iay1 >= 0)) return;
+    r.iax0 = iAtomX - s.iRadX < 0 ? 0 : iAtomX - s.iRadX;
+    r.iax1 = iAtomX + s.iRadX >= s.potNx ? s.potNx - 1 : iAtomX + s.iRadX;
+    r.iay0 = iAtomY - s.iRadY < 0 ? 0 : iAtomY - s.iRadY;
+    r.iay1 = iAtomY + s.iRadY >= s.potNy ? s.potNy - 1 : iAtomY + s.iRadY;
+    if (!(r.iax0 < s.potNx && r.iax1 >= 0 && r.iay0 < s.potNy && r.iay1 >= 0)) return false;
   } else {  // :1106-1109, upper bounds exclusive
-    iax0 = iAtomX - s.iRadX; iax1 = iAtomX + s.iRadX - 1;
-    iay0 = iAtomY - s.iRadY; iay1 = iAtomY + s.iRadY - 1;
+    r.iax0 = iAtomX - s.iRadX; r.iax1 = iAtomX + s.iRadX - 1;
+    r.iay0 = iAtomY - s.iRadY; r.iay1 = iAtomY + s.iRadY - 1;
   }
-  const int iaz0 = iAtomZ - s.iRadZ < 0 ? -iAtomZ : -s.iRadZ;
-  const int iaz1 = iAtomZ + s.iRadZ >= s.slices ? s.slices - iAtomZ - 1 : s.iRadZ;
-  if (!(iAtomZ + iaz0 < s.slices && iAtomZ + iaz1 >= 0)) return;
-  const gfloat* lut = (const gfloat*)luts[kind[ia]];
+  r.iaz0 = r.iAtomZ - s.iRadZ < 0 ? -r.iAtomZ : -s.iRadZ;
+  r.iaz1 = r.iAtomZ + s.iRadZ >= s.slices ? s.slices - r.iAtomZ - 1 : s.iRadZ;
+  if (!(r.iAtomZ + r.iaz0 < s.slices && r.iAtomZ + r.iaz1 >= 0)) return false;
   const float zrel = atomZ / dzs;
-  const double dOffsZ = (double)((float)((float)(iAtomZ + iaz0) - zrel) * (float)s.nzSub);
-  const int iOffsZ0 = (int)(dOffsZ + 0.5) * s.Nr;
-  const int limHi = s.Nr * (s.Nz - 1), limLo = -s.Nr * (s.Nz - 1), step = s.nzSub * s.Nr;
-  const int nbx = iax1 - iax0 + 1, nby = iay1 - iay0 + 1;
+  const double dOffsZ = (double)((float)((float)(r.iAtomZ + r.iaz0) - zrel) * (float)s.nzSub);
+  r.iOffsZ0 = (int)(dOffsZ + 0.5) * s.Nr;
+  return true;
+}
+
+// LUT sample of pixel distance bin (ir, ddr) at z offset iOffsZ (:1170-1190).
+__device__ __forceinline__ double lut_sample(const gfloat* lut, const ScatterArgs& s, int ir, double ddr, int iOffsZ) {
+  const int limHi = s.Nr * (s.Nz - 1), limLo = -s.Nr * (s.Nz - 1);
+  double pv = 0;
+  if (iOffsZ < 0) {
+    if (iOffsZ > limLo) pv = (1 - ddr) * lut[ir - iOffsZ + s.Nr] + ddr * lut[ir + 1 - iOffsZ + s.Nr];
+  } else if (iOffsZ < limHi) {
+    pv = (1 - ddr) * lut[ir + iOffsZ] + ddr * lut[ir + 1 + iOffsZ];
+  }
+  return pv;
+}
+
+__global__ __launch_bounds__(256) void k_scatter(const QsAtomDev* __restrict__ atoms, int natom,
+                                                 const float* const* __restrict__ luts,
+                                                 const int* __restrict__ kind, ScatterArgs s, float* __restrict__ V) {
+  const int ia = blockIdx.x;
+  if (ia >= natom) return;
+  AtomReach r;
+  if (!atom_setup(atoms[ia], s, r)) return;
+  const gfloat* lut = (const gfloat*)luts[kind[ia]];
+  const int step = s.nzSub * s.Nr;
+  const int nbx = r.iax1 - r.iax0 + 1, nby = r.iay1 - r.iay0 + 1;
   const double dr = (double)s.dr;
   const size_t slice = (size_t)s.potNx * s.potNy;
   for (int pidx = threadIdx.x; pidx < nbx * nby; pidx += blockDim.x) {
-    const int iax = iax0 + pidx / nby, iay = iay0 + pidx % nby;
-    double x2 = (float)(iax * s.dx - atomX);
+    const int iax = r.iax0 + pidx / nby, iay = r.iay0 + pidx % nby;
+    double x2 = (float)(iax * s.dx - r.atomX);
     x2 *= x2;
-    double y2 = (float)(iay * s.dy - atomY);
+    double y2 = (float)(iay * s.dy - r.atomY);
     y2 *= y2;
-    const double r = sqrt(x2 + y2);
-    double ddr = r / dr;
+    const double rr = sqrt(x2 + y2);
+    double ddr = rr / dr;
     const int ir = (int)floor(ddr);
     if (ir >= s.Nr - 1) continue;
     ddr -= ir;
@@ -139,20 +161,91 @@ __global__ __launch_bounds__(256) void k_scatter(const QsAtomDev* __restrict__ a
     // modulo keeps atoms further outside the array in bounds as well)
     const int ixw = s.periodic ? (((iax % s.potNx) + s.potNx) % s.potNx) : iax;
     const int iyw = s.periodic ? (((iay % s.potNy) + s.potNy) % s.potNy) : iay;
-    float* cell = V + (size_t)(iAtomZ + iaz0) * slice + (size_t)ixw * s.potNy + iyw;
-    int iOffsZ = iOffsZ0;
-    for (int iaz = iaz0; iaz <= iaz1; iaz++) {
-      double pv = 0;
-      if (iOffsZ < 0) {
-        if (iOffsZ > limLo) pv = (1 - ddr) * lut[ir - iOffsZ + s.Nr] + ddr * lut[ir + 1 - iOffsZ + s.Nr];
-      } else if (iOffsZ < limHi) {
-        pv = (1 - ddr) * lut[ir + iOffsZ] + ddr * lut[ir + 1 + iOffsZ];
-      }
+    float* cell = V + (size_t)(r.iAtomZ + r.iaz0) * slice + (size_t)ixw * s.potNy + iyw;
+    int iOffsZ = r.iOffsZ0;
+    for (int iaz = r.iaz0; iaz <= r.iaz1; iaz++) {
+      const double pv = lut_sample(lut, s, ir, ddr, iOffsZ);
       if (pv != 0) atomicAdd(cell, (float)pv);
       cell += slice;
       iOffsZ += step;
     }
   }
+}
+
+// ---------------------------------------------------------------------------------
+// Gather form (non-periodic boxes): atoms binned by the kPotTile x kPotTile pixel tiles
+// their box reaches; one thread per pixel of a tile sums, for kPotZc consecutive slices
+// in registers, the same LUT samples the scatter adds, and stores each voxel once.
+
+// Set-up of every atom; reach[ia].iAtomZ = INT_MIN marks atoms that reach nothing.
+__global__ __launch_bounds__(256) void k_atom_reach(const QsAtomDev* __restrict__ atoms, int natom,
+                                                    const int* __restrict__ kind, ScatterArgs s,
+                                                    AtomReach* __restrict__ reach) {
+  const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= natom) return;
+  AtomReach r;
+  if (!atom_setup(atoms[ia], s, r)) r.iAtomZ = INT_MIN;
+  r.kind = kind[ia];
+  reach[ia] = r;
+}
+
+// Per tile, the number of atoms whose box reaches it (fill = false), or the atom
+// indices at the tile's next free list positions (fill = true).
+__global__ __launch_bounds__(256) void k_bin_atoms(const AtomReach* __restrict__ reach, int natom, int tiles_y,
+                                                   int* __restrict__ count_or_cursor, int* __restrict__ list,
+                                                   bool fill) {
+  const int ia = blockIdx.x * blockDim.x + threadIdx.x;
+  if (ia >= natom) return;
+  const AtomReach r = reach[ia];
+  if (r.iAtomZ == INT_MIN) return;
+  for (int tx = r.iax0 / kPotTile; tx <= r.iax1 / kPotTile; tx++)
+    for (int ty = r.iay0 / kPotTile; ty <= r.iay1 / kPotTile; ty++) {
+      const int pos = atomicAdd(count_or_cursor + tx * tiles_y + ty, 1);
+      if (fill) list[pos] = ia;
+    }
+}
+
+__global__ __launch_bounds__(kPotTile * kPotTile) void k_gather(const AtomReach* __restrict__ reach,
+                                                               const int* __restrict__ tstart,
+                                                               const int* __restrict__ list,
+                                                               const float* const* __restrict__ luts, ScatterArgs s,
+                                                               int tiles_y, float* __restrict__ V) {
+  const int tile = blockIdx.x;  // tile (tx, ty) = tx * tiles_y + ty
+  const int ix = (tile / tiles_y) * kPotTile + threadIdx.x / kPotTile;
+  const int iy = (tile % tiles_y) * kPotTile + threadIdx.x % kPotTile;
+  const int z0 = blockIdx.y * kPotZc;
+  const int step = s.nzSub * s.Nr;
+  const double dr = (double)s.dr;
+  float acc[kPotZc];
+#pragma unroll
+  for (int zz = 0; zz < kPotZc; zz++) acc[zz] = 0.f;
+  const int b = tstart[tile], e = tstart[tile + 1];
+  for (int k = b; k < e; k++) {
+    const AtomReach r = reach[list[k]];  // the same atom in every lane
+    if (r.iAtomZ + r.iaz1 < z0 || r.iAtomZ + r.iaz0 >= z0 + kPotZc) continue;
+    if (ix < r.iax0 || ix > r.iax1 || iy < r.iay0 || iy > r.iay1) continue;
+    double x2 = (float)(ix * s.dx - r.atomX);
+    x2 *= x2;
+    double y2 = (float)(iy * s.dy - r.atomY);
+    y2 *= y2;
+    const double rr = sqrt(x2 + y2);
+    double ddr = rr / dr;
+    const int ir = (int)floor(ddr);
+    if (ir >= s.Nr - 1) continue;
+    ddr -= ir;
+    const gfloat* lut = (const gfloat*)luts[r.kind];
+#pragma unroll
+    for (int zz = 0; zz < kPotZc; zz++) {
+      const int iaz = z0 + zz - r.iAtomZ;
+      if (iaz < r.iaz0 || iaz > r.iaz1) continue;
+      acc[zz] += (float)lut_sample(lut, s, ir, ddr, r.iOffsZ0 + (iaz - r.iaz0) * step);
+    }
+  }
+  if (ix >= s.potNx || iy >= s.potNy) return;
+  const size_t slice = (size_t)s.potNx * s.potNy;
+#pragma unroll
+  for (int zz = 0; zz < kPotZc; zz++)
+    if (z0 + zz < s.slices) V[(size_t)(z0 + zz) * slice + (size_t)ix * s.potNy + iy] = acc[zz];
 }
 
 static unsigned gridn(size_t n) {
@@ -172,6 +265,25 @@ void launch_lut_extract(const float2* tmp, const LutArgs& g, float* lut, hipStre
 void launch_scatter(const QsAtomDev* atoms, int natom, const float* const* luts, const int* kind,
                     const ScatterArgs& s, float* V, hipStream_t st) {
   if (natom > 0) hipLaunchKernelGGL(k_scatter, dim3(natom), dim3(256), 0, st, atoms, natom, luts, kind, s, V);
+}
+
+void launch_atom_reach(const QsAtomDev* atoms, int natom, const int* kind, const ScatterArgs& s, AtomReach* reach,
+                       hipStream_t st) {
+  if (natom > 0) hipLaunchKernelGGL(k_atom_reach, dim3((natom + 255) / 256), dim3(256), 0, st, atoms, natom, kind, s, reach);
+}
+
+void launch_bin_atoms(const AtomReach* reach, int natom, int tiles_y, int* count_or_cursor, int* list, bool fill,
+                      hipStream_t st) {
+  if (natom > 0)
+    hipLaunchKernelGGL(k_bin_atoms, dim3((natom + 255) / 256), dim3(256), 0, st, reach, natom, tiles_y,
+                       count_or_cursor, list, fill);
+}
+
+void launch_gather(const AtomReach* reach, const int* tstart, const int* list, const float* const* luts,
+                   const ScatterArgs& s, float* V, hipStream_t st) {
+  const int tiles_x = (s.potNx + kPotTile - 1) / kPotTile, tiles_y = (s.potNy + kPotTile - 1) / kPotTile;
+  hipLaunchKernelGGL(k_gather, dim3((unsigned)(tiles_x * tiles_y), (unsigned)((s.slices + kPotZc - 1) / kPotZc)),
+                     dim3(kPotTile * kPotTile), 0, st, reach, tstart, list, luts, s, tiles_y, V);
 }
 
 }  // namespace qs
