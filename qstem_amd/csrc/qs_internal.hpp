@@ -105,8 +105,9 @@ void launch_scatter(const QsAtomDev* atoms, int natom, const float* const* luts,
                     const ScatterArgs& s, float* V, hipStream_t st);
 
 // Gather form of the scatter for non-periodic boxes: per-atom reach, atoms binned by
-// the kPotTile x kPotTile pixel tiles their box reaches (tile tx * tiles_y + ty), and
-// one thread per pixel summing kPotZc slices.
+// the kPotTile x kPotTile pixel tiles their box reaches (tile tx * tiles_y + ty) and
+// the blocks of kPotZc slices they reach (bin tile * zblocks + zb), and one thread per
+// pixel summing the kPotZc slices of its block.
 constexpr int kPotTile = 16, kPotZc = 16;
 struct AtomReach {
   float atomX, atomY;          // position relative to the potential array (Angstrom)
@@ -117,8 +118,8 @@ struct AtomReach {
 };
 void launch_atom_reach(const QsAtomDev* atoms, int natom, const int* kind, const ScatterArgs& s, AtomReach* reach,
                        hipStream_t st);
-void launch_bin_atoms(const AtomReach* reach, int natom, int tiles_y, int* count_or_cursor, int* list, bool fill,
-                      hipStream_t st);
+void launch_bin_atoms(const AtomReach* reach, int natom, int tiles_y, int zblocks, int* count_or_cursor, int* list,
+                      bool fill, hipStream_t st);
 void launch_gather(const AtomReach* reach, const int* tstart, const int* list, const float* const* luts,
                    const ScatterArgs& s, float* V, hipStream_t st);
 

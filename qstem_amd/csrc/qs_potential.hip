@@ -174,8 +174,9 @@ __global__ __launch_bounds__(256) void k_scatter(const QsAtomDev* __restrict__ a
 
 // ---------------------------------------------------------------------------------
 // Gather form (non-periodic boxes): atoms binned by the kPotTile x kPotTile pixel tiles
-// their box reaches; one thread per pixel of a tile sums, for kPotZc consecutive slices
-// in registers, the same LUT samples the scatter adds, and stores each voxel once.
+// their box reaches and the blocks of kPotZc slices they reach; one thread per pixel of
+// a tile sums, for the kPotZc slices of its block in registers, the same LUT samples the
+// scatter adds, and stores each voxel once.
 
 // Set-up of every atom; reach[ia].iAtomZ = INT_MIN marks atoms that reach nothing.
 __global__ __launch_bounds__(256) void k_atom_reach(const QsAtomDev* __restrict__ atoms, int natom,
@@ -189,20 +190,23 @@ __global__ __launch_bounds__(256) void k_atom_reach(const QsAtomDev* __restrict_
   reach[ia] = r;
 }
 
-// Per tile, the number of atoms whose box reaches it (fill = false), or the atom
-// indices at the tile's next free list positions (fill = true).
+// Per bin (pixel tile, block of kPotZc slices), the number of atoms that reach it
+// (fill = false), or the atom indices at the bin's next free list positions (fill =
+// true).  Bin ((tx * tiles_y + ty) * zblocks + zb).
 __global__ __launch_bounds__(256) void k_bin_atoms(const AtomReach* __restrict__ reach, int natom, int tiles_y,
-                                                   int* __restrict__ count_or_cursor, int* __restrict__ list,
-                                                   bool fill) {
+                                                   int zblocks, int* __restrict__ count_or_cursor,
+                                                   int* __restrict__ list, bool fill) {
   const int ia = blockIdx.x * blockDim.x + threadIdx.x;
   if (ia >= natom) return;
   const AtomReach r = reach[ia];
   if (r.iAtomZ == INT_MIN) return;
+  const int zb0 = (r.iAtomZ + r.iaz0) / kPotZc, zb1 = (r.iAtomZ + r.iaz1) / kPotZc;
   for (int tx = r.iax0 / kPotTile; tx <= r.iax1 / kPotTile; tx++)
-    for (int ty = r.iay0 / kPotTile; ty <= r.iay1 / kPotTile; ty++) {
-      const int pos = atomicAdd(count_or_cursor + tx * tiles_y + ty, 1);
-      if (fill) list[pos] = ia;
-    }
+    for (int ty = r.iay0 / kPotTile; ty <= r.iay1 / kPotTile; ty++)
+      for (int zb = zb0; zb <= zb1; zb++) {
+        const int pos = atomicAdd(count_or_cursor + (tx * tiles_y + ty) * zblocks + zb, 1);
+        if (fill) list[pos] = ia;
+      }
 }
 
 __global__ __launch_bounds__(kPotTile * kPotTile) void k_gather(const AtomReach* __restrict__ reach,
@@ -214,31 +218,38 @@ __global__ __launch_bounds__(kPotTile * kPotTile) void k_gather(const AtomReach*
   const int ix = (tile / tiles_y) * kPotTile + threadIdx.x / kPotTile;
   const int iy = (tile % tiles_y) * kPotTile + threadIdx.x % kPotTile;
   const int z0 = blockIdx.y * kPotZc;
+  const int bin = tile * gridDim.y + blockIdx.y;
+  __shared__ AtomReach sh[kPotTile * kPotTile];  // the bin's atoms, a block-sized batch at a time
   const int step = s.nzSub * s.Nr;
   const double dr = (double)s.dr;
   float acc[kPotZc];
 #pragma unroll
   for (int zz = 0; zz < kPotZc; zz++) acc[zz] = 0.f;
-  const int b = tstart[tile], e = tstart[tile + 1];
-  for (int k = b; k < e; k++) {
-    const AtomReach r = reach[list[k]];  // the same atom in every lane
-    if (r.iAtomZ + r.iaz1 < z0 || r.iAtomZ + r.iaz0 >= z0 + kPotZc) continue;
-    if (ix < r.iax0 || ix > r.iax1 || iy < r.iay0 || iy > r.iay1) continue;
-    double x2 = (float)(ix * s.dx - r.atomX);
-    x2 *= x2;
-    double y2 = (float)(iy * s.dy - r.atomY);
-    y2 *= y2;
-    const double rr = sqrt(x2 + y2);
-    double ddr = rr / dr;
-    const int ir = (int)floor(ddr);
-    if (ir >= s.Nr - 1) continue;
-    ddr -= ir;
-    const gfloat* lut = (const gfloat*)luts[r.kind];
+  const int b = tstart[bin], e = tstart[bin + 1];
+  for (int k0 = b; k0 < e; k0 += kPotTile * kPotTile) {
+    const int n = min(kPotTile * kPotTile, e - k0);
+    __syncthreads();  // the previous batch is read
+    if ((int)threadIdx.x < n) sh[threadIdx.x] = reach[list[k0 + threadIdx.x]];
+    __syncthreads();
+    for (int k = 0; k < n; k++) {
+      const AtomReach r = sh[k];  // the same atom in every lane
+      if (ix < r.iax0 || ix > r.iax1 || iy < r.iay0 || iy > r.iay1) continue;
+      double x2 = (float)(ix * s.dx - r.atomX);
+      x2 *= x2;
+      double y2 = (float)(iy * s.dy - r.atomY);
+      y2 *= y2;
+      const double rr = sqrt(x2 + y2);
+      double ddr = rr / dr;
+      const int ir = (int)floor(ddr);
+      if (ir >= s.Nr - 1) continue;
+      ddr -= ir;
+      const gfloat* lut = (const gfloat*)luts[r.kind];
 #pragma unroll
-    for (int zz = 0; zz < kPotZc; zz++) {
-      const int iaz = z0 + zz - r.iAtomZ;
-      if (iaz < r.iaz0 || iaz > r.iaz1) continue;
-      acc[zz] += (float)lut_sample(lut, s, ir, ddr, r.iOffsZ0 + (iaz - r.iaz0) * step);
+      for (int zz = 0; zz < kPotZc; zz++) {
+        const int iaz = z0 + zz - r.iAtomZ;
+        if (iaz < r.iaz0 || iaz > r.iaz1) continue;
+        acc[zz] += (float)lut_sample(lut, s, ir, ddr, r.iOffsZ0 + (iaz - r.iaz0) * step);
+      }
     }
   }
   if (ix >= s.potNx || iy >= s.potNy) return;
@@ -272,10 +283,10 @@ void launch_atom_reach(const QsAtomDev* atoms, int natom, const int* kind, const
   if (natom > 0) hipLaunchKernelGGL(k_atom_reach, dim3((natom + 255) / 256), dim3(256), 0, st, atoms, natom, kind, s, reach);
 }
 
-void launch_bin_atoms(const AtomReach* reach, int natom, int tiles_y, int* count_or_cursor, int* list, bool fill,
-                      hipStream_t st) {
+void launch_bin_atoms(const AtomReach* reach, int natom, int tiles_y, int zblocks, int* count_or_cursor, int* list,
+                      bool fill, hipStream_t st) {
   if (natom > 0)
-    hipLaunchKernelGGL(k_bin_atoms, dim3((natom + 255) / 256), dim3(256), 0, st, reach, natom, tiles_y,
+    hipLaunchKernelGGL(k_bin_atoms, dim3((natom + 255) / 256), dim3(256), 0, st, reach, natom, tiles_y, zblocks,
                        count_or_cursor, list, fill);
 }
 
