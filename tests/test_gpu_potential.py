@@ -30,10 +30,14 @@ def _cfg(nx, pad, dx, slices, dz):
     return cfg
 
 
-@pytest.mark.parametrize("periodic", [0, 1])
-def test_potential_matches_oracle(oracle, periodic):
+@pytest.mark.parametrize("periodic,pad,slices,dz", [(0, 32, 8, 3.905 / 2), (1, 32, 8, 3.905 / 2),
+                                                  (0, 42, 20, 0.5)])
+def test_potential_matches_oracle(oracle, periodic, pad, slices, dz):
+    """Scatter (periodic) and gather (non-periodic) forms; the last case has a grid (170)
+    and a slice count (20) that are not multiples of the gather's 16-pixel tiles and
+    16-slice blocks."""
     from qstem_amd import Multislice
-    cfg = _cfg(128, 32, 0.05, 8, 3.905 / 2)
+    cfg = _cfg(128, pad, 0.05, slices, dz)
     atoms = perovskite_atoms((2, 2, 2), jitter=0.05, seed=3)
     offx, offy = -0.7, -0.4
     with Multislice(cfg) as ms:
