@@ -4,7 +4,7 @@
  * Drop-in boundary for QSTEM's STEM hot path.  The reference exposes it as
  * C++ free functions over a process-global MULS and per-thread WAVEFUNCs
  * (stem3/stemlib.h:36-62); this ABI replaces them with an opaque context
- * that owns the device buffers and one HIP stream.  Plain pointers and sizes
+ * that owns the device buffers and two HIP streams (qs_set_streams).  Plain pointers and sizes
  * only; every call returns a status code instead of the reference's
  * printf+exit(0).
  *
@@ -115,7 +115,9 @@ int qs_set_scattering_table(qs_ctx *ctx, const double *table);
 /* make3DSlices + getAtomPotential3D + initSTEMSlices: builds the projected potential of
  * the given atoms into qs_params.slices slices on the device, then the transmission
  * functions.  Neutral atoms and the 3D potential only (QS_ERR_UNSUPPORTED otherwise).
- * V_out (nullable) receives V [slices][potNx][potNy]. */
+ * V_out (nullable) receives V [slices][potNx][potNy].  Each voxel is the FP32 sum of the
+ * atoms' LUT samples, in an unspecified order: gathered per voxel from atoms binned by
+ * pixel tile (periodicXY = 0) or added by float atomics (periodicXY = 1). */
 int qs_build_potential(qs_ctx *ctx, const qs_atom *atoms, int natom, const qs_pot_params *pp, float *V_out);
 
 /* Replaces the probe optics of the context (Cs, C5, df0, astigmatism, a_nm/phi_nm,
