@@ -1251,8 +1251,9 @@ int qs_run_stem_cfg(qs_ctx* c, const int* ipos, const int* config, int nprobe, i
                                   exit_wave != nullptr, int_intensity != nullptr, r.has_cfg,
                                   (long long)trans_base(c), (long long)c->state.p, (long long)c->det.p,
                                   (long long)c->diff.p, (long long)c->exitw.p, (long long)c->total.p,
-                                  (long long)c->partial.p, (long long)c->partial2.p, nst, (long long)c->ipos.p, (long long)c->cfgidx.p,
-                                  (long long)c->probe.p, c->rows_per_wave, c->tiles_per_block};
+                                  (long long)c->partial.p, (long long)c->partial2.p, nst, (long long)c->ipos.p,
+                                  (long long)c->cfgidx.p, (long long)c->probe.p, c->rows_per_wave,
+                                  c->tiles_per_block};
     for (int s = 0; s < nslice; s++) key.push_back(bin_of_slice ? bin_of_slice[s] : -1);
     if (!c->gexec || key != c->gkey) {
       // one graph per stream, each the linear chain of that stream's launches (measured
