@@ -116,8 +116,8 @@ int qs_set_scattering_table(qs_ctx *ctx, const double *table);
  * the given atoms into qs_params.slices slices on the device, then the transmission
  * functions.  Neutral atoms and the 3D potential only (QS_ERR_UNSUPPORTED otherwise).
  * V_out (nullable) receives V [slices][potNx][potNy].  Each voxel is the FP32 sum of the
- * atoms' LUT samples, in an unspecified order: gathered per voxel from atoms binned by
- * pixel tile (periodicXY = 0) or added by float atomics (periodicXY = 1). */
+ * atoms' LUT samples, gathered per voxel from atoms binned by pixel tile, in an
+ * unspecified order. */
 int qs_build_potential(qs_ctx *ctx, const qs_atom *atoms, int natom, const qs_pot_params *pp, float *V_out);
 
 /* Replaces the probe optics of the context (Cs, C5, df0, astigmatism, a_nm/phi_nm,

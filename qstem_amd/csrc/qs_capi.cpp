@@ -730,42 +730,37 @@ int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_
     sa.iRadX = (int)ceil(R / p.resolutionX); sa.iRadY = (int)ceil(R / p.resolutionY); sa.iRadZ = (int)ceil(R / dz);
     sa.Nr = Nr; sa.Nz = Nz; sa.nzSub = nzp;
     sa.dr = pp->lut_dr_fix ? p.resolutionX : p.resolutionX / 2;
-    if (!pp->periodicXY) {
-      // gather: per-atom reach, count and list the atoms of each bin (pixel tile, block
-      // of slices), then one thread per voxel sums them (one store per voxel instead of
-      // an atomic per sample)
-      const int tiles_x = (p.potNx + qs::kPotTile - 1) / qs::kPotTile;
-      const int tiles_y = (p.potNy + qs::kPotTile - 1) / qs::kPotTile;
-      const int zblocks = (p.slices + qs::kPotZc - 1) / qs::kPotZc;
-      const int ntile = tiles_x * tiles_y * zblocks;  // bins: pixel tile x block of slices
-      HIPCHK(c, c->reach.ensure(sizeof(qs::AtomReach) * natom));
-      HIPCHK(c, c->tcount.ensure(sizeof(int) * ntile));
-      HIPCHK(c, c->tstart.ensure(sizeof(int) * (ntile + 1)));
-      qs::AtomReach* reach = c->reach.as<qs::AtomReach>();
-      qs::launch_atom_reach(c->atoms.as<qs::QsAtomDev>(), natom, c->kinds.as<int>(), sa, reach, c->stream);
-      HIPCHK(c, hipMemsetAsync(c->tcount.p, 0, sizeof(int) * ntile, c->stream));
-      qs::launch_bin_atoms(reach, natom, tiles_y, zblocks, c->tcount.as<int>(), nullptr, false, c->stream);
-      std::vector<int> cnt(ntile), start(ntile + 1, 0);
-      HIPCHK(c, hipMemcpyAsync(cnt.data(), c->tcount.p, sizeof(int) * ntile, hipMemcpyDeviceToHost, c->stream));
-      HIPCHK(c, hipStreamSynchronize(c->stream));
-      long long total = 0;
-      for (int t = 0; t < ntile; t++) {
-        start[t] = (int)total;
-        total += cnt[t];
-        if (total > 0x7fffffffLL) return fail(c, QS_ERR_ARG, "qs_build_potential: too many atom-tile pairs");
-      }
-      start[ntile] = (int)total;
-      HIPCHK(c, c->tlist.ensure(sizeof(int) * (size_t)std::max<long long>(1, total)));
-      HIPCHK(c, hipMemcpyAsync(c->tstart.p, start.data(), sizeof(int) * (ntile + 1), hipMemcpyHostToDevice, c->stream));
-      HIPCHK(c, hipMemcpyAsync(c->tcount.p, start.data(), sizeof(int) * ntile, hipMemcpyHostToDevice, c->stream));
-      qs::launch_bin_atoms(reach, natom, tiles_y, zblocks, c->tcount.as<int>(), c->tlist.as<int>(), true, c->stream);
-      qs::launch_gather(reach, c->tstart.as<int>(), c->tlist.as<int>(), c->lutptrs.as<const float*>(), sa,
-                        c->V.as<float>(), c->stream);
-      HIPCHK(c, hipStreamSynchronize(c->stream));  // the host arrays above are copied from
-    } else {
-      qs::launch_scatter(c->atoms.as<qs::QsAtomDev>(), natom, c->lutptrs.as<const float*>(), c->kinds.as<int>(), sa,
-                         c->V.as<float>(), c->stream);
+    // gather: per-atom reach, count and list the atoms of each bin (pixel tile, block
+    // of slices), then one thread per voxel sums them (one store per voxel instead of
+    // an atomic per sample)
+    const int tiles_x = (p.potNx + qs::kPotTile - 1) / qs::kPotTile;
+    const int tiles_y = (p.potNy + qs::kPotTile - 1) / qs::kPotTile;
+    const int zblocks = (p.slices + qs::kPotZc - 1) / qs::kPotZc;
+    const int ntile = tiles_x * tiles_y * zblocks;  // bins: pixel tile x block of slices
+    HIPCHK(c, c->reach.ensure(sizeof(qs::AtomReach) * natom));
+    HIPCHK(c, c->tcount.ensure(sizeof(int) * ntile));
+    HIPCHK(c, c->tstart.ensure(sizeof(int) * (ntile + 1)));
+    qs::AtomReach* reach = c->reach.as<qs::AtomReach>();
+    qs::launch_atom_reach(c->atoms.as<qs::QsAtomDev>(), natom, c->kinds.as<int>(), sa, reach, c->stream);
+    HIPCHK(c, hipMemsetAsync(c->tcount.p, 0, sizeof(int) * ntile, c->stream));
+    qs::launch_bin_atoms(reach, natom, sa, zblocks, c->tcount.as<int>(), nullptr, false, c->stream);
+    std::vector<int> cnt(ntile), start(ntile + 1, 0);
+    HIPCHK(c, hipMemcpyAsync(cnt.data(), c->tcount.p, sizeof(int) * ntile, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipStreamSynchronize(c->stream));
+    long long total = 0;
+    for (int t = 0; t < ntile; t++) {
+      start[t] = (int)total;
+      total += cnt[t];
+      if (total > 0x7fffffffLL) return fail(c, QS_ERR_ARG, "qs_build_potential: too many atom-tile pairs");
     }
+    start[ntile] = (int)total;
+    HIPCHK(c, c->tlist.ensure(sizeof(int) * (size_t)std::max<long long>(1, total)));
+    HIPCHK(c, hipMemcpyAsync(c->tstart.p, start.data(), sizeof(int) * (ntile + 1), hipMemcpyHostToDevice, c->stream));
+    HIPCHK(c, hipMemcpyAsync(c->tcount.p, start.data(), sizeof(int) * ntile, hipMemcpyHostToDevice, c->stream));
+    qs::launch_bin_atoms(reach, natom, sa, zblocks, c->tcount.as<int>(), c->tlist.as<int>(), true, c->stream);
+    qs::launch_gather(reach, c->tstart.as<int>(), c->tlist.as<int>(), c->lutptrs.as<const float*>(), sa,
+                      c->V.as<float>(), c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // the host arrays above are copied from
   }
   HIPCHK(c, hipGetLastError());
   if (V_out) HIPCHK(c, hipMemcpyAsync(V_out, c->V.p, sizeof(float) * nV, hipMemcpyDeviceToHost, c->stream));

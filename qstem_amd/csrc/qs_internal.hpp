@@ -101,10 +101,7 @@ struct ScatterArgs {
 
 void launch_lut_fk(const double* spl, const LutArgs& g, double B, float2* out, hipStream_t st);
 void launch_lut_extract(const float2* tmp, const LutArgs& g, float* lut, hipStream_t st);
-void launch_scatter(const QsAtomDev* atoms, int natom, const float* const* luts, const int* kind,
-                    const ScatterArgs& s, float* V, hipStream_t st);
-
-// Gather form of the scatter for non-periodic boxes: per-atom reach, atoms binned by
+// Sum over atoms in gather form: per-atom reach, atoms binned by
 // the kPotTile x kPotTile pixel tiles their box reaches (tile tx * tiles_y + ty) and
 // the blocks of kPotZc slices they reach (bin tile * zblocks + zb), and one thread per
 // pixel summing the kPotZc slices of its block.
@@ -118,8 +115,8 @@ struct AtomReach {
 };
 void launch_atom_reach(const QsAtomDev* atoms, int natom, const int* kind, const ScatterArgs& s, AtomReach* reach,
                        hipStream_t st);
-void launch_bin_atoms(const AtomReach* reach, int natom, int tiles_y, int zblocks, int* count_or_cursor, int* list,
-                      bool fill, hipStream_t st);
+void launch_bin_atoms(const AtomReach* reach, int natom, const ScatterArgs& s, int zblocks, int* count_or_cursor,
+                      int* list, bool fill, hipStream_t st);
 void launch_gather(const AtomReach* reach, const int* tstart, const int* list, const float* const* luts,
                    const ScatterArgs& s, float* V, hipStream_t st);
 

@@ -8,8 +8,8 @@
 //   2. a 2D inverse DFT of the (kz, kx) map (Stockham/Bluestein row kernels),
 //   3. z-integration over nzPerSlice sub-samples, clamp >= 0, scale (k_lut_extract),
 //   4. r-linear / z-nearest LUT samples of every atom summed into the slices it
-//      reaches: by pixel tile and slice block, one store per voxel (k_gather), or
-//      for periodic boxes one block per atom with float atomics (k_scatter).
+//      reaches, gathered per voxel from atoms binned by pixel tile and block of
+//      slices (k_atom_reach, k_bin_atoms, k_gather): one store per voxel.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -122,6 +122,8 @@ __device__ __forceinline__ bool atom_setup(const QsAtomDev& at, const ScatterArg
   return true;
 }
 
+__device__ __forceinline__ int pmod(int a, int n) { return ((a % n) + n) % n; }
+
 // LUT sample of pixel distance bin (ir, ddr) at z offset iOffsZ (:1170-1190).
 __device__ __forceinline__ double lut_sample(const gfloat* lut, const ScatterArgs& s, int ir, double ddr, int iOffsZ) {
   const int limHi = s.Nr * (s.Nz - 1), limLo = -s.Nr * (s.Nz - 1);
@@ -134,49 +136,12 @@ __device__ __forceinline__ double lut_sample(const gfloat* lut, const ScatterArg
   return pv;
 }
 
-__global__ __launch_bounds__(256) void k_scatter(const QsAtomDev* __restrict__ atoms, int natom,
-                                                 const float* const* __restrict__ luts,
-                                                 const int* __restrict__ kind, ScatterArgs s, float* __restrict__ V) {
-  const int ia = blockIdx.x;
-  if (ia >= natom) return;
-  AtomReach r;
-  if (!atom_setup(atoms[ia], s, r)) return;
-  const gfloat* lut = (const gfloat*)luts[kind[ia]];
-  const int step = s.nzSub * s.Nr;
-  const int nbx = r.iax1 - r.iax0 + 1, nby = r.iay1 - r.iay0 + 1;
-  const double dr = (double)s.dr;
-  const size_t slice = (size_t)s.potNx * s.potNy;
-  for (int pidx = threadIdx.x; pidx < nbx * nby; pidx += blockDim.x) {
-    const int iax = r.iax0 + pidx / nby, iay = r.iay0 + pidx % nby;
-    double x2 = (float)(iax * s.dx - r.atomX);
-    x2 *= x2;
-    double y2 = (float)(iay * s.dy - r.atomY);
-    y2 *= y2;
-    const double rr = sqrt(x2 + y2);
-    double ddr = rr / dr;
-    const int ir = (int)floor(ddr);
-    if (ir >= s.Nr - 1) continue;
-    ddr -= ir;
-    // periodic wrap (the reference adds 2*potN before the modulo; a full
-    // modulo keeps atoms further outside the array in bounds as well)
-    const int ixw = s.periodic ? (((iax % s.potNx) + s.potNx) % s.potNx) : iax;
-    const int iyw = s.periodic ? (((iay % s.potNy) + s.potNy) % s.potNy) : iay;
-    float* cell = V + (size_t)(r.iAtomZ + r.iaz0) * slice + (size_t)ixw * s.potNy + iyw;
-    int iOffsZ = r.iOffsZ0;
-    for (int iaz = r.iaz0; iaz <= r.iaz1; iaz++) {
-      const double pv = lut_sample(lut, s, ir, ddr, iOffsZ);
-      if (pv != 0) atomicAdd(cell, (float)pv);
-      cell += slice;
-      iOffsZ += step;
-    }
-  }
-}
-
 // ---------------------------------------------------------------------------------
-// Gather form (non-periodic boxes): atoms binned by the kPotTile x kPotTile pixel tiles
-// their box reaches and the blocks of kPotZc slices they reach; one thread per pixel of
-// a tile sums, for the kPotZc slices of its block in registers, the same LUT samples the
-// scatter adds, and stores each voxel once.
+// Gather form of getAtomPotential3D's additions: atoms binned by the kPotTile x
+// kPotTile pixel tiles their box reaches (wrapped for periodic boxes) and the blocks
+// of kPotZc slices they reach; one thread per pixel of a tile sums, for the kPotZc
+// slices of its block in registers, the LUT samples of every atom (every image of it
+// inside a periodic box) and stores each voxel once.
 
 // Set-up of every atom; reach[ia].iAtomZ = INT_MIN marks atoms that reach nothing.
 __global__ __launch_bounds__(256) void k_atom_reach(const QsAtomDev* __restrict__ atoms, int natom,
@@ -193,22 +158,49 @@ __global__ __launch_bounds__(256) void k_atom_reach(const QsAtomDev* __restrict_
 // Per bin (pixel tile, block of kPotZc slices), the number of atoms that reach it
 // (fill = false), or the atom indices at the bin's next free list positions (fill =
 // true).  Bin ((tx * tiles_y + ty) * zblocks + zb).
-__global__ __launch_bounds__(256) void k_bin_atoms(const AtomReach* __restrict__ reach, int natom, int tiles_y,
-                                                   int zblocks, int* __restrict__ count_or_cursor,
+// Tiles of a grid of n pixels reached by the pixel box [lo, hi] (periodic boxes may
+// leave [0, n) and are wrapped): up to two disjoint ranges t0[i]..t1[i]; returns how many.
+__device__ __forceinline__ int tile_ranges(int lo, int hi, int n, int* t0, int* t1) {
+  const int tiles = (n + kPotTile - 1) / kPotTile;
+  if (hi - lo + 1 >= n) {
+    t0[0] = 0; t1[0] = tiles - 1;
+    return 1;
+  }
+  const int a = ((lo % n) + n) % n, b = ((hi % n) + n) % n;
+  if (a <= b) {
+    t0[0] = a / kPotTile; t1[0] = b / kPotTile;
+    return 1;
+  }
+  if (a / kPotTile <= b / kPotTile) {  // the two wrapped pieces share a tile
+    t0[0] = 0; t1[0] = tiles - 1;
+    return 1;
+  }
+  t0[0] = 0; t1[0] = b / kPotTile;
+  t0[1] = a / kPotTile; t1[1] = tiles - 1;
+  return 2;
+}
+
+__global__ __launch_bounds__(256) void k_bin_atoms(const AtomReach* __restrict__ reach, int natom, ScatterArgs s,
+                                                   int tiles_y, int zblocks, int* __restrict__ count_or_cursor,
                                                    int* __restrict__ list, bool fill) {
   const int ia = blockIdx.x * blockDim.x + threadIdx.x;
   if (ia >= natom) return;
   const AtomReach r = reach[ia];
   if (r.iAtomZ == INT_MIN) return;
   const int zb0 = (r.iAtomZ + r.iaz0) / kPotZc, zb1 = (r.iAtomZ + r.iaz1) / kPotZc;
-  for (int tx = r.iax0 / kPotTile; tx <= r.iax1 / kPotTile; tx++)
-    for (int ty = r.iay0 / kPotTile; ty <= r.iay1 / kPotTile; ty++)
-      for (int zb = zb0; zb <= zb1; zb++) {
-        const int pos = atomicAdd(count_or_cursor + (tx * tiles_y + ty) * zblocks + zb, 1);
-        if (fill) list[pos] = ia;
-      }
+  int x0[2], x1[2], y0[2], y1[2];
+  const int nxr = tile_ranges(r.iax0, r.iax1, s.potNx, x0, x1), nyr = tile_ranges(r.iay0, r.iay1, s.potNy, y0, y1);
+  for (int i = 0; i < nxr; i++)
+    for (int tx = x0[i]; tx <= x1[i]; tx++)
+      for (int j = 0; j < nyr; j++)
+        for (int ty = y0[j]; ty <= y1[j]; ty++)
+          for (int zb = zb0; zb <= zb1; zb++) {
+            const int pos = atomicAdd(count_or_cursor + (tx * tiles_y + ty) * zblocks + zb, 1);
+            if (fill) list[pos] = ia;
+          }
 }
 
+template <bool PERIODIC>
 __global__ __launch_bounds__(kPotTile * kPotTile) void k_gather(const AtomReach* __restrict__ reach,
                                                                const int* __restrict__ tstart,
                                                                const int* __restrict__ list,
@@ -233,22 +225,30 @@ __global__ __launch_bounds__(kPotTile * kPotTile) void k_gather(const AtomReach*
     __syncthreads();
     for (int k = 0; k < n; k++) {
       const AtomReach r = sh[k];  // the same atom in every lane
-      if (ix < r.iax0 || ix > r.iax1 || iy < r.iay0 || iy > r.iay1) continue;
-      double x2 = (float)(ix * s.dx - r.atomX);
-      x2 *= x2;
-      double y2 = (float)(iy * s.dy - r.atomY);
-      y2 *= y2;
-      const double rr = sqrt(x2 + y2);
-      double ddr = rr / dr;
-      const int ir = (int)floor(ddr);
-      if (ir >= s.Nr - 1) continue;
-      ddr -= ir;
       const gfloat* lut = (const gfloat*)luts[r.kind];
+      // samples of box pixel (ax, ay) (unwrapped coordinates) into this pixel's sums
+      auto add = [&](int ax, int ay) __attribute__((always_inline)) {
+        double x2 = (float)(ax * s.dx - r.atomX);
+        x2 *= x2;
+        double y2 = (float)(ay * s.dy - r.atomY);
+        y2 *= y2;
+        const double rr = sqrt(x2 + y2);
+        double ddr = rr / dr;
+        const int ir = (int)floor(ddr);
+        if (ir >= s.Nr - 1) return;
+        ddr -= ir;
 #pragma unroll
-      for (int zz = 0; zz < kPotZc; zz++) {
-        const int iaz = z0 + zz - r.iAtomZ;
-        if (iaz < r.iaz0 || iaz > r.iaz1) continue;
-        acc[zz] += (float)lut_sample(lut, s, ir, ddr, r.iOffsZ0 + (iaz - r.iaz0) * step);
+        for (int zz = 0; zz < kPotZc; zz++) {
+          const int iaz = z0 + zz - r.iAtomZ;
+          if (iaz < r.iaz0 || iaz > r.iaz1) continue;
+          acc[zz] += (float)lut_sample(lut, s, ir, ddr, r.iOffsZ0 + (iaz - r.iaz0) * step);
+        }
+      };
+      if constexpr (!PERIODIC) {  // the pixel itself, when inside the box
+        if (ix >= r.iax0 && ix <= r.iax1 && iy >= r.iay0 && iy <= r.iay1) add(ix, iy);
+      } else {  // every image ix + i*potNx, iy + j*potNy inside the box
+        for (int ax = r.iax0 + pmod(ix - r.iax0, s.potNx); ax <= r.iax1; ax += s.potNx)
+          for (int ay = r.iay0 + pmod(iy - r.iay0, s.potNy); ay <= r.iay1; ay += s.potNy) add(ax, ay);
       }
     }
   }
@@ -273,28 +273,27 @@ void launch_lut_extract(const float2* tmp, const LutArgs& g, float* lut, hipStre
   hipLaunchKernelGGL(k_lut_extract, dim3(gridn((size_t)(g.nx / 2) * (g.nz / 2))), dim3(256), 0, st, tmp, g, lut);
 }
 
-void launch_scatter(const QsAtomDev* atoms, int natom, const float* const* luts, const int* kind,
-                    const ScatterArgs& s, float* V, hipStream_t st) {
-  if (natom > 0) hipLaunchKernelGGL(k_scatter, dim3(natom), dim3(256), 0, st, atoms, natom, luts, kind, s, V);
-}
-
 void launch_atom_reach(const QsAtomDev* atoms, int natom, const int* kind, const ScatterArgs& s, AtomReach* reach,
                        hipStream_t st) {
   if (natom > 0) hipLaunchKernelGGL(k_atom_reach, dim3((natom + 255) / 256), dim3(256), 0, st, atoms, natom, kind, s, reach);
 }
 
-void launch_bin_atoms(const AtomReach* reach, int natom, int tiles_y, int zblocks, int* count_or_cursor, int* list,
-                      bool fill, hipStream_t st) {
+void launch_bin_atoms(const AtomReach* reach, int natom, const ScatterArgs& s, int zblocks, int* count_or_cursor,
+                      int* list, bool fill, hipStream_t st) {
+  const int tiles_y = (s.potNy + kPotTile - 1) / kPotTile;
   if (natom > 0)
-    hipLaunchKernelGGL(k_bin_atoms, dim3((natom + 255) / 256), dim3(256), 0, st, reach, natom, tiles_y, zblocks,
+    hipLaunchKernelGGL(k_bin_atoms, dim3((natom + 255) / 256), dim3(256), 0, st, reach, natom, s, tiles_y, zblocks,
                        count_or_cursor, list, fill);
 }
 
 void launch_gather(const AtomReach* reach, const int* tstart, const int* list, const float* const* luts,
                    const ScatterArgs& s, float* V, hipStream_t st) {
   const int tiles_x = (s.potNx + kPotTile - 1) / kPotTile, tiles_y = (s.potNy + kPotTile - 1) / kPotTile;
-  hipLaunchKernelGGL(k_gather, dim3((unsigned)(tiles_x * tiles_y), (unsigned)((s.slices + kPotZc - 1) / kPotZc)),
-                     dim3(kPotTile * kPotTile), 0, st, reach, tstart, list, luts, s, tiles_y, V);
+  const dim3 grid((unsigned)(tiles_x * tiles_y), (unsigned)((s.slices + kPotZc - 1) / kPotZc));
+  if (s.periodic)
+    hipLaunchKernelGGL(k_gather<true>, grid, dim3(kPotTile * kPotTile), 0, st, reach, tstart, list, luts, s, tiles_y, V);
+  else
+    hipLaunchKernelGGL(k_gather<false>, grid, dim3(kPotTile * kPotTile), 0, st, reach, tstart, list, luts, s, tiles_y, V);
 }
 
 }  // namespace qs
