@@ -666,6 +666,46 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_adj_cols(AdjArgs A) {
   const int pg = a.probe0 + p;
   const int c0 = tile_id * TC;
   const size_t off = (size_t)pg * a.nx * a.kcp;
+  if constexpr (col_direct<NX>() && !LAST) {
+    // as the forward column sweep at this size: the staging loads are the transform's
+    // registers, one LDS column slot for the exchanges
+    const int t = threadIdx.x % T;
+    Twiddles<NX> tw;
+    tw.init(a.twx, t);
+    const TileStage<NX> st(A.gx + off, a.nx, a.kcp);
+    float4 q[E];
+    st.in_regs(q, c0);
+    float2 u[E], w[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      u[m] = make_float2(q[m].x, q[m].y);
+      w[m] = make_float2(q[m].z, q[m].w);
+    }
+    auto column = [&](float2 (&v)[E], int c) __attribute__((always_inline)) {
+      const int iy = c <= a.band_a ? c : (c < a.kc ? c - (a.band_a + 1) + (a.ny - a.band_b) : -1);
+      fft_wave<NX, -1, false, 0, -1, ColCfg<NX>::TWC>(v, tile, tw, t);
+      if (iy >= 0) {
+        const float ky2 = a.ky2[iy];
+        const float2 py = a.py[iy];
+#pragma unroll
+        for (int m = 0; m < E; m++) {
+          const int ix = t + T * m;
+          const float kx2 = a.kx2[ix];
+          const bool in = kx2 < a.k2max && (kx2 + ky2) < a.k2max;
+          v[m] = in ? cmulc(cmulc(v[m], py), a.px[ix]) : make_float2(0.f, 0.f);
+        }
+      } else {
+#pragma unroll
+        for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
+      }
+      fft_wave<NX, +1, false, 0, -1, ColCfg<NX>::TWC>(v, tile, tw, t);
+    };
+    column(u, c0);
+    xform_sync<NX>();  // the second column reuses the slot
+    column(w, c0 + 1);
+    st.out_regs(u, w, c0);
+    return;
+  }
   if (threadIdx.x == 0) red = 0.f;
   TileStage<NX>((LAST ? A.xprev : A.gx) + off, a.nx, a.kcp).in(tile, c0);
   __syncthreads();
@@ -809,7 +849,7 @@ void launch_adj_rows_t(const AdjArgs& a, bool first, hipStream_t st) {
 
 template <int N>
 void launch_adj_cols_t(const AdjArgs& a, bool last, hipStream_t st) {
-  const size_t lds = col_lds_bytes_t<N>();
+  const size_t lds = col_direct<N>() && !last ? sizeof(float2) * ColCfg<N>::STRIDE : col_lds_bytes_t<N>();
   dim3 grid((unsigned)(a.f.nprobe * a.f.ntiles));
   if (last) hipLaunchKernelGGL((k_adj_cols<N, true>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
   else hipLaunchKernelGGL((k_adj_cols<N, false>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
