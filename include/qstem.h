@@ -158,6 +158,22 @@ int qs_device_memory(long long *current_bytes, long long *peak_bytes, int reset_
 int qs_run_stem(qs_ctx *ctx, const int *ipos, int nprobe, int start, const int *bin_of_slice, int nslice, int nbins,
                 float *det, float *diff, float *exit_wave, float *int_intensity);
 
+/* qs_run_stem without allocation: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under HIP stream capture, tight serving
+ * loops, harnesses that forbid hipMalloc).  Same results as qs_run_stem with
+ * start = 0, configuration 0 and no diff or exit-wave output (det and int_intensity
+ * as there, host pointers, nullable); the context's transmission functions and
+ * probe must be set.  Every device buffer of the call (window offsets, wave state,
+ * detector partial sums and outputs) is carved from a static device array of
+ * QS_NOALLOC_SCRATCH_BYTES that each GPU holds until the process ends; a call that
+ * needs more (about nprobe * nx * 2/3 ny * 8 bytes) fails with QS_ERR_ARG.  The call
+ * calls no hipMalloc or hipFree and runs on the context's stream, one probe group
+ * after another.  All contexts on one GPU share that GPU's array: calls that
+ * overlap on one GPU race. */
+#define QS_NOALLOC_SCRATCH_BYTES (64LL << 20)
+int qs_run_stem_noalloc(qs_ctx *ctx, const int *ipos, int nprobe, const int *bin_of_slice, int nslice, int nbins,
+                        float *det, float *int_intensity);
+
 /* Frozen-phonon configurations as a batch axis.  The context holds nconfig
  * transmission functions (one per phonon configuration, same slices); the
  * potential setters (qs_set_potential, qs_set_trans, qs_build_potential) fill

@@ -88,6 +88,7 @@ SIGNATURES = {
     "qs_device_memory": (_I, [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), _I]),
     "qs_run_stem": (_I, [_P, _IP, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
     "qs_run_stem_cfg": (_I, [_P, _IP, _IP, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
+    "qs_run_stem_noalloc": (_I, [_P, _IP, _I, _IP, _I, _I, _FP, _FP]),
     "qs_set_configs": (_I, [_P, _I]),
     "qs_select_config": (_I, [_P, _I]),
     "qs_share_trans": (_I, [_P, _P]),

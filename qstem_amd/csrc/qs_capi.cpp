@@ -122,6 +122,7 @@ struct LutArgsHost {
 
 struct qs_ctx {
   qs_params p{};
+  int device = 0;  // HIP device current at qs_create
   hipStream_t stream = nullptr;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   // second stream of qs_run_stem / qs_run_groups: consecutive probe groups (or the
@@ -264,6 +265,40 @@ static float bl_k2max(const qs_params& p) {
   return k2max * k2max;
 }
 
+// ---------------------------------------------------------------------------------
+// Run without allocation.  The wave state, window offsets, detector partial sums
+// and outputs of qs_run_stem_noalloc live in this array; each device holds its own
+// copy (until the process ends), whose address hipGetSymbolAddress resolves once
+// per device in qs_create.  Host code uses only that address, never the name.
+static __device__ __align__(256) unsigned char scratch[QS_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+std::map<int, void*> g_scratch_ptr;        // device -> address of its copy of scratch
+std::map<int, std::string> g_scratch_err;  // device -> why the address could not be resolved
+}  // namespace
+
+// Resolves the scratch address of device dev (the current device); once per device.
+static void resolve_scratch(int dev) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (g_scratch_ptr.count(dev) || g_scratch_err.count(dev)) return;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || !ptr)
+    g_scratch_err[dev] = std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e);
+  else
+    g_scratch_ptr[dev] = ptr;
+}
+
+static unsigned char* scratch_of(int dev, std::string& why) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  auto it = g_scratch_ptr.find(dev);
+  if (it != g_scratch_ptr.end()) return static_cast<unsigned char*>(it->second);
+  auto e = g_scratch_err.find(dev);
+  why = e != g_scratch_err.end() ? e->second : "scratch address of this device not resolved";
+  return nullptr;
+}
+
 extern "C" {
 
 const char* qs_last_error(const qs_ctx* ctx) { return ctx ? ctx->err.c_str() : g_create_error.c_str(); }
@@ -301,6 +336,11 @@ int qs_create(const qs_params* p, qs_ctx** out) {
     c->err = "hipStreamCreate/hipEventCreate failed (no HIP device?)";
     return bad(QS_ERR_HIP);
   }
+  if (hipGetDevice(&c->device) != hipSuccess) {
+    c->err = "hipGetDevice failed";
+    return bad(QS_ERR_HIP);
+  }
+  resolve_scratch(c->device);  // address of qs_run_stem_noalloc's scratch on this device (once per device)
   if (const char* e = getenv("QS_ROWS_PER_WAVE")) c->rows_per_wave = atoi(e);  // tuning overrides
   if (const char* e = getenv("QS_TILES_PER_BLOCK")) c->tiles_per_block = atoi(e);
   if (const char* e = getenv("QS_STREAMS")) c->nstreams = atoi(e) == 1 ? 1 : 2;
@@ -1026,13 +1066,17 @@ struct RunPlan {
   const int* bins = nullptr;
   bool want_det = false, has_cfg = false;
   float *det = nullptr, *diff = nullptr, *exit_wave = nullptr, *intensity = nullptr;
+  // device buffers of the outputs: the context's own, or carved from the static
+  // scratch array by qs_run_stem_noalloc
+  float *det_d = nullptr, *diff_d = nullptr, *total_d = nullptr;
+  float2* exit_d = nullptr;
   size_t slice_elems = 0;
   SweepArgs a{};
 };
 
-static int plan_run(qs_ctx* c, const char* who, const int* ipos, const int* config, int nprobe, int start,
-                    const int* bin_of_slice, int nslice, int nbins, float* det, float* diff, float* exit_wave,
-                    float* int_intensity, RunPlan& r) {
+// Argument checks shared by the run entry points.
+static int check_run(qs_ctx* c, const char* who, const int* ipos, const int* config, int nprobe, int start,
+                     const int* bin_of_slice, int nslice, int nbins, const float* det) {
   const qs_params& p = c->p;
   const std::string w(who);
   if (!ipos || nprobe < 1 || nslice < 1 || (start != 0 && start != 1)) return fail(c, QS_ERR_ARG, w + ": bad arguments");
@@ -1052,16 +1096,68 @@ static int plan_run(qs_ctx* c, const char* who, const int* ipos, const int* conf
     for (int s = 0; s < nslice; s++)
       if (bin_of_slice[s] >= nbins) return fail(c, QS_ERR_ARG, w + ": bin index >= nbins");
   }
-  const int nx = p.nx, ny = p.ny;
-  // probe group advanced through all slices together: the library default keeps the
-  // wave state of the groups in flight (two sweeps per slice read and write it) inside
-  // the 256 MB Infinity Cache: one group of ~180 MB, or with two streams two groups of
-  // ~96 MB (measured best for 512x512 waves: 128 probes, 2 x 64 probes)
-  const size_t per_probe = sizeof(float2) * (size_t)nx * c->kcp;
-  const size_t budget = (size_t)(c->nstreams > 1 ? 96u : 180u) << 20;
+  return QS_OK;
+}
+
+// Probe-group size: the library default keeps the wave state of the groups in
+// flight (two sweeps per slice read and write it) inside the 256 MB Infinity
+// Cache: one group of ~180 MB, or with two streams two groups of ~96 MB
+// (measured best for 512x512 waves: 128 probes, 2 x 64 probes).
+static int group_size(const qs_ctx* c, int nprobe, int nstreams) {
+  const size_t per_probe = sizeof(float2) * (size_t)c->p.nx * c->kcp;
+  const size_t budget = (size_t)(nstreams > 1 ? 96u : 180u) << 20;
   int Bdef = (int)std::max<size_t>(32, std::min<size_t>(1024, budget / per_probe));
   Bdef = std::max(32, (Bdef / 16) * 16);
-  if (c->nstreams > 1 && nprobe > 1) Bdef = std::min(Bdef, (nprobe + 1) / 2);  // both streams get work
+  if (nstreams > 1 && nprobe > 1) Bdef = std::min(Bdef, (nprobe + 1) / 2);  // both streams get work
+  return c->batch > 0 ? std::min(c->batch, nprobe) : std::min(Bdef, nprobe);
+}
+
+// Kernel arguments of a run whose device buffers (state, ipos, partial, outputs) are set in r.
+static void fill_sweep_args(qs_ctx* c, RunPlan& r, const int* d_ipos, const int* d_cfg, float2* d_state,
+                            float* d_partial) {
+  const qs_params& p = c->p;
+  SweepArgs& a = r.a;
+  a.nx = p.nx;
+  a.ny = p.ny;
+  a.kc = c->kc;
+  a.kcp = c->kcp;
+  a.band_a = c->band_a;
+  a.band_b = c->band_b;
+  a.ntiles = c->ntiles;
+  a.potNy = p.potNy;
+  a.rw = c->rows_per_wave;
+  a.tpb = c->tiles_per_block;
+  a.probe = c->probe.as<float2>();
+  a.ipos = reinterpret_cast<const int2*>(d_ipos);
+  a.cfg = d_cfg;
+  a.cfg_stride = (long long)trans_slices_of(c) * (long long)r.slice_elems;
+  a.state = d_state;
+  a.out_wave = r.exit_d;
+  a.twx = c->twx.as<float2>();
+  a.twy = c->twy.as<float2>();
+  a.px = c->px.as<float2>();
+  a.py = c->py.as<float2>();
+  a.kx2 = c->dkx2.as<float>();
+  a.ky2 = c->dky2.as<float>();
+  a.k2max = c->k2max;
+  a.ndet = p.ndet;
+  for (int d = 0; d < p.ndet; d++) {
+    a.det_sx[d] = (int)p.det_shiftx[d];
+    a.det_sy[d] = (int)p.det_shifty[d];
+    a.det_k2in[d] = c->det_k2in[d];
+    a.det_k2out[d] = c->det_k2out[d];
+  }
+  a.partial = d_partial;
+  a.diff_scale = (float)pow((double)p.nx * p.ny, 1.5);
+}
+
+static int plan_run(qs_ctx* c, const char* who, const int* ipos, const int* config, int nprobe, int start,
+                    const int* bin_of_slice, int nslice, int nbins, float* det, float* diff, float* exit_wave,
+                    float* int_intensity, RunPlan& r) {
+  const qs_params& p = c->p;
+  const int rc = check_run(c, who, ipos, config, nprobe, start, bin_of_slice, nslice, nbins, det);
+  if (rc != QS_OK) return rc;
+  const int nx = p.nx, ny = p.ny;
   r = RunPlan{};
   r.c = c;
   r.nprobe = nprobe;
@@ -1069,7 +1165,7 @@ static int plan_run(qs_ctx* c, const char* who, const int* ipos, const int* conf
   r.nslice = nslice;
   r.nbins = nbins;
   r.bins = bin_of_slice;
-  r.B = c->batch > 0 ? std::min(c->batch, nprobe) : std::min(Bdef, nprobe);
+  r.B = group_size(c, nprobe, c->nstreams);
   r.want_det = det && p.ndet > 0;
   r.det = det;
   r.diff = diff;
@@ -1100,40 +1196,13 @@ static int plan_run(qs_ctx* c, const char* who, const int* ipos, const int* conf
   }
   if (exit_wave) HIPCHK(c, c->exitw.ensure(sizeof(float2) * (size_t)nprobe * nx * ny));
   if (int_intensity) HIPCHK(c, c->total.ensure(sizeof(float) * nprobe));
+  r.det_d = c->det.as<float>();
+  r.diff_d = c->diff.as<float>();
+  r.total_d = c->total.as<float>();
+  r.exit_d = c->exitw.as<float2>();
 
-  SweepArgs& a = r.a;
-  a.nx = nx;
-  a.ny = ny;
-  a.kc = c->kc;
-  a.kcp = c->kcp;
-  a.band_a = c->band_a;
-  a.band_b = c->band_b;
-  a.ntiles = c->ntiles;
-  a.potNy = p.potNy;
-  a.rw = c->rows_per_wave;
-  a.tpb = c->tiles_per_block;
-  a.probe = c->probe.as<float2>();
-  a.ipos = c->ipos.as<int2>();
-  a.cfg = config ? c->cfgidx.as<int>() : nullptr;
-  a.cfg_stride = (long long)trans_slices_of(c) * (long long)r.slice_elems;
-  a.state = c->state.as<float2>();
-  a.out_wave = c->exitw.as<float2>();
-  a.twx = c->twx.as<float2>();
-  a.twy = c->twy.as<float2>();
-  a.px = c->px.as<float2>();
-  a.py = c->py.as<float2>();
-  a.kx2 = c->dkx2.as<float>();
-  a.ky2 = c->dky2.as<float>();
-  a.k2max = c->k2max;
-  a.ndet = p.ndet;
-  for (int d = 0; d < p.ndet; d++) {
-    a.det_sx[d] = (int)p.det_shiftx[d];
-    a.det_sy[d] = (int)p.det_shifty[d];
-    a.det_k2in[d] = c->det_k2in[d];
-    a.det_k2out[d] = c->det_k2out[d];
-  }
-  a.partial = c->partial.as<float>();
-  a.diff_scale = (float)pow((double)nx * ny, 1.5);
+  fill_sweep_args(c, r, c->ipos.as<int>(), config ? c->cfgidx.as<int>() : nullptr, c->state.as<float2>(),
+                  c->partial.as<float>());
   return QS_OK;
 }
 
@@ -1173,15 +1242,15 @@ static hipError_t enqueue_slice(RunPlan& r, int p0, int s, hipStream_t st, float
   const int bin = (r.want_det && r.bins) ? r.bins[s] : -1;
   const bool tot = last && r.intensity;
   a.rec_bin = (bin >= 0 || tot) ? (bin >= 0 ? bin : 0) : -1;
-  a.diff = (last && r.diff) ? c->diff.as<float>() : nullptr;
+  a.diff = (last && r.diff) ? r.diff_d : nullptr;
   a.partial = partial;
   if (lm && (e = lm->mark(1, true, st)) != hipSuccess) return e;
   qs::launch_sweep_cols(a, a.rec_bin >= 0, a.diff != nullptr, st);
   if (lm && (e = lm->mark(1, false, st)) != hipSuccess) return e;
   if (bin >= 0 || tot) {
-    float* dbin = bin >= 0 ? c->det.as<float>() + (size_t)bin * p.ndet * r.nprobe : nullptr;
+    float* dbin = bin >= 0 ? r.det_d + (size_t)bin * p.ndet * r.nprobe : nullptr;
     qs::launch_reduce_partials(partial, nb, c->ntiles, bin >= 0 ? p.ndet : 0, p.electronScale, dbin,
-                               r.nprobe, tot ? c->total.as<float>() : nullptr, p0, st);
+                               r.nprobe, tot ? r.total_d : nullptr, p0, st);
   }
   if (last && r.exit_wave) qs::launch_sweep_rows(a, 2, st);
   return hipSuccess;
@@ -1193,12 +1262,12 @@ static int finish_run(RunPlan& r, hipStream_t st) {
   const qs_params& p = c->p;
   const size_t n2 = (size_t)r.nprobe * p.nx * p.ny;
   if (r.want_det)
-    HIPCHK(c, hipMemcpyAsync(r.det, c->det.p, sizeof(float) * (size_t)r.nbins * p.ndet * r.nprobe,
+    HIPCHK(c, hipMemcpyAsync(r.det, r.det_d, sizeof(float) * (size_t)r.nbins * p.ndet * r.nprobe,
                              hipMemcpyDeviceToHost, st));
-  if (r.diff) HIPCHK(c, hipMemcpyAsync(r.diff, c->diff.p, sizeof(float) * n2, hipMemcpyDeviceToHost, st));
-  if (r.exit_wave) HIPCHK(c, hipMemcpyAsync(r.exit_wave, c->exitw.p, sizeof(float2) * n2, hipMemcpyDeviceToHost, st));
+  if (r.diff) HIPCHK(c, hipMemcpyAsync(r.diff, r.diff_d, sizeof(float) * n2, hipMemcpyDeviceToHost, st));
+  if (r.exit_wave) HIPCHK(c, hipMemcpyAsync(r.exit_wave, r.exit_d, sizeof(float2) * n2, hipMemcpyDeviceToHost, st));
   if (r.intensity)
-    HIPCHK(c, hipMemcpyAsync(r.intensity, c->total.p, sizeof(float) * r.nprobe, hipMemcpyDeviceToHost, st));
+    HIPCHK(c, hipMemcpyAsync(r.intensity, r.total_d, sizeof(float) * r.nprobe, hipMemcpyDeviceToHost, st));
   HIPCHK(c, hipStreamSynchronize(st));
   c->state_valid = true;
   c->state_probes = r.nprobe;
@@ -1305,6 +1374,72 @@ int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bi
                 float* det, float* diff, float* exit_wave, float* int_intensity) {
   return qs_run_stem_cfg(c, ipos, nullptr, nprobe, start, bin_of_slice, nslice, nbins, det, diff, exit_wave,
                          int_intensity);
+}
+
+// qs_run_stem from the probe, with every device buffer of the call in the static scratch array.
+int qs_run_stem_noalloc(qs_ctx* c, const int* ipos, int nprobe, const int* bin_of_slice, int nslice, int nbins,
+                        float* det, float* int_intensity) {
+  if (!c) return QS_ERR_ARG;
+  const char* who = "qs_run_stem_noalloc";
+  int rc = check_run(c, who, ipos, nullptr, nprobe, 0, bin_of_slice, nslice, nbins, det);
+  if (rc != QS_OK) return rc;
+  const qs_params& p = c->p;
+  std::string why;
+  unsigned char* base = scratch_of(c->device, why);
+  if (!base) return fail(c, QS_ERR_HIP, std::string(who) + ": " + why);
+  RunPlan r;
+  r.c = c;
+  r.nprobe = nprobe;
+  r.start = 0;
+  r.nslice = nslice;
+  r.nbins = nbins;
+  r.bins = bin_of_slice;
+  r.B = group_size(c, nprobe, 1);
+  r.want_det = det && p.ndet > 0;
+  r.det = det;
+  r.intensity = int_intensity;
+  r.slice_elems = (size_t)p.potNx * p.potNy;
+  // layout of the scratch array for this call, 256-byte aligned pieces
+  size_t off = 0;
+  auto carve = [&](size_t bytes) {
+    const size_t o = off;
+    off += (bytes + 255) / 256 * 256;
+    return o;
+  };
+  const size_t state_elems = (size_t)nprobe * p.nx * c->kcp;
+  const size_t o_ipos = carve(sizeof(int) * 2 * (size_t)nprobe);
+  const size_t o_state = carve(sizeof(float2) * state_elems);
+  const size_t o_part = carve(sizeof(float) * (size_t)r.B * c->ntiles * (qs::kMaxDet + 1));
+  const size_t nd = r.want_det ? (size_t)nbins * p.ndet * nprobe : 0;
+  const size_t o_det = carve(sizeof(float) * nd);
+  const size_t o_tot = carve(int_intensity ? sizeof(float) * (size_t)nprobe : 0);
+  if (off > (size_t)QS_NOALLOC_SCRATCH_BYTES)
+    return fail(c, QS_ERR_ARG, std::string(who) + ": " + std::to_string(nprobe) + " probes need " +
+                                   std::to_string(off) + " bytes of scratch, more than the " +
+                                   std::to_string((long long)QS_NOALLOC_SCRATCH_BYTES) + " the library holds");
+  int* d_ipos = reinterpret_cast<int*>(base + o_ipos);
+  float2* d_state = reinterpret_cast<float2*>(base + o_state);
+  float* d_part = reinterpret_cast<float*>(base + o_part);
+  r.det_d = r.want_det ? reinterpret_cast<float*>(base + o_det) : nullptr;
+  r.total_d = int_intensity ? reinterpret_cast<float*>(base + o_tot) : nullptr;
+  hipStream_t st = c->stream;
+  HIPCHK(c, hipEventRecord(c->ev0, st));
+  // the array keeps the previous call's contents: the state's padding columns must
+  // start at zero, the detector bins start from the caller's values (in/out)
+  qs::launch_fill_zero(d_state, state_elems, st);
+  HIPCHK(c, hipMemcpyAsync(d_ipos, ipos, sizeof(int) * 2 * (size_t)nprobe, hipMemcpyHostToDevice, st));
+  if (r.want_det) HIPCHK(c, hipMemcpyAsync(r.det_d, det, sizeof(float) * nd, hipMemcpyHostToDevice, st));
+  fill_sweep_args(c, r, d_ipos, nullptr, d_state, d_part);
+  for (int p0 = 0; p0 < nprobe; p0 += r.B)
+    for (int s = 0; s < nslice; s++) HIPCHK(c, enqueue_slice(r, p0, s, st, d_part, nullptr));
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, st));
+  if (r.want_det) HIPCHK(c, hipMemcpyAsync(det, r.det_d, sizeof(float) * nd, hipMemcpyDeviceToHost, st));
+  if (int_intensity)
+    HIPCHK(c, hipMemcpyAsync(int_intensity, r.total_d, sizeof(float) * nprobe, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  return QS_OK;
 }
 
 int qs_run_groups(const qs_group* groups, int ngroup, const int* bin_of_slice, int nslice, int nbins) {

@@ -407,6 +407,31 @@ class Multislice:
         return out
 
 
+    def run_stem_noalloc(self, ipos: np.ndarray, nslice: Optional[int] = None, bins: Optional[Sequence[int]] = None,
+                         nbins: int = 0, want_intensity: bool = False) -> dict:
+        """run_stem from the probe through qs_run_stem_noalloc: every device buffer
+        of the call lives in the library's static scratch array (no allocation).
+        Returns 'det' [nbins][ndet][nprobe] (and 'intensity')."""
+        ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
+        n = ipos.shape[0]
+        nslice = self._nslice if nslice is None else nslice
+        if bins is None:
+            bins, nbins = record_bins(nslice)
+        bins = np.ascontiguousarray(bins, dtype=np.int32)
+        ndet = len(self.cfg.detectors)
+        det = np.zeros((nbins, ndet, n), dtype=np.float32) if ndet > 0 else None
+        inten = np.empty(n, dtype=np.float32) if want_intensity else None
+        rc = self._lib.qs_run_stem_noalloc(self._h, _iptr(ipos), n, _iptr(bins), int(nslice), int(nbins), _fptr(det),
+                                           _fptr(inten))
+        self._check(rc, "qs_run_stem_noalloc")
+        out = {}
+        if det is not None:
+            out["det"] = det
+        if inten is not None:
+            out["intensity"] = inten
+        return out
+
+
 def run_groups(groups, bins=None, nbins: int = 0, nslice: Optional[int] = None, want_intensity: bool = False):
     """Uneven batch (qs_run_groups): groups = [(Multislice, ipos, config or None), ...]
     whose contexts share one set of transmission functions (share_trans).  Every
