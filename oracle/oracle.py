@@ -162,34 +162,81 @@ def rez_table():
     return np.fromfile(path, dtype="<f8").reshape(93, 123)
 
 
-def atom_luts(g, Zs, Bs, sliceThickness):
-    """LUT per (Z, B) pair; applies the s-grid taper of the reference first."""
-    tab = rez_table().copy()
-    sgrid = np.ascontiguousarray(tab[0])
-    rows = np.ascontiguousarray(tab[1:])
+def rez_offs_table():
+    """Ion-offset table (scatParOffs): row 0 the s-grid, row Z df_e/dq."""
+    path = os.path.join(os.path.dirname(HERE), "qstem_amd", "data", "rez_scatparoffs_f64.bin")
+    return np.fromfile(path, dtype="<f8").reshape(93, 123)
+
+
+def _tapered(tab, g):
+    sgrid = np.ascontiguousarray(tab[0].copy())
+    rows = np.ascontiguousarray(tab[1:].copy())
     load().qo_taper_scatpar(_p(sgrid), _p(rows), rows.shape[0], _D(g.smax2))
+    return sgrid, rows
+
+
+def lut_norm_fix(g, sliceThickness):
+    """Scale of the LUT under the normalisation switch (slice sum -> f_e(0))."""
+    f = load().qo_lut_norm_fix
+    f.restype = _D
+    return float(f(ctypes.byref(g), _F(sliceThickness)))
+
+
+def atom_luts(g, Zs, Bs, sliceThickness, norm_fix=False, clamp=True):
+    """LUT per (Z, B) pair; applies the s-grid taper of the reference first."""
+    sgrid, rows = _tapered(rez_table(), g)
+    scale = lut_norm_fix(g, sliceThickness) if norm_fix else 1.0
     luts = []
     for Z, B in zip(Zs, Bs):
         frow = np.ascontiguousarray(rows[Z - 1])
         lut = np.zeros((g.Nz_lut, g.Nr), np.float32)
-        load().qo_atom_lut(ctypes.byref(g), _p(sgrid), _p(frow), _D(B), _F(sliceThickness), _p(lut, _F))
+        load().qo_atom_lut2(ctypes.byref(g), _p(sgrid), _p(frow), _p(sgrid), _p(frow), _D(g.dkx), _D(B),
+                            _F(sliceThickness), _D(scale), int(clamp), _p(lut, _F))
+        luts.append(lut)
+    return luts
+
+
+def ion_luts(g, Zs, Bs, sliceThickness, compat_q5=True, norm_fix=False, clamp=True, resY=None, atomRadius=None):
+    """Ion-offset LUT per (Z, B) pair (getAtomPotentialOffset3D): the offset
+    table's spline; its q_y sum reads the neutral table's values with that
+    spline when compat_q5 (the reference), the offset table's otherwise.  The
+    q_y step is dky of resY (= dkx for square pixels)."""
+    sg_o, rows_o = _tapered(rez_offs_table(), g)
+    sg_n, rows_n = _tapered(rez_table(), g)
+    dky = g.dkx
+    if resY is not None:
+        ny = 2 * 2 * int(np.ceil(np.float32(atomRadius) / np.float32(resY)))
+        dky = 0.5 * 2 / float(np.float32(ny * np.float32(resY)))
+    scale = lut_norm_fix(g, sliceThickness) if norm_fix else 1.0
+    luts = []
+    for Z, B in zip(Zs, Bs):
+        fo = np.ascontiguousarray(rows_o[Z - 1])
+        sgy, fy = (sg_n, np.ascontiguousarray(rows_n[Z - 1])) if compat_q5 else (sg_o, fo)
+        lut = np.zeros((g.Nz_lut, g.Nr), np.float32)
+        load().qo_atom_lut2(ctypes.byref(g), _p(sg_o), _p(fo), _p(sgy), _p(fy), _D(dky), _D(B), _F(sliceThickness),
+                            _D(scale), int(clamp), _p(lut, _F))
         luts.append(lut)
     return luts
 
 
 def make_slices(cfg, g, luts, kinds, atoms, potOffsetX, potOffsetY, czOffset=0.0, centerSlices=0,
-                atomRadius=5.0, periodic=0):
-    """atoms: structured array or list of (x, y, z, dw, occ, q, Znum), z-sorted."""
+                atomRadius=5.0, periodic=0, offs_luts=None):
+    """atoms: structured array or list of (x, y, z, dw, occ, q, Znum), z-sorted.
+    offs_luts[k] (None entries allowed): ion-offset LUT of kind k for charged atoms."""
     arr = (QoAtom * len(atoms))()
     for i, a in enumerate(atoms):
         arr[i] = QoAtom(*[float(v) for v in a[:6]], int(a[6]))
     lut_ptrs = (ctypes.POINTER(_F) * len(luts))(*[_p(l, _F) for l in luts])
+    offs_ptrs = None
+    if offs_luts is not None:
+        offs_ptrs = (ctypes.POINTER(_F) * len(offs_luts))(
+            *[_p(l, _F) if l is not None else ctypes.POINTER(_F)() for l in offs_luts])
     kinds = np.ascontiguousarray(kinds, dtype=np.int32)
     V = np.zeros((cfg.slices, cfg.potNx, cfg.potNy), np.float32)
     p = params(cfg)
-    load().qo_make_slices(ctypes.byref(p), ctypes.byref(g), lut_ptrs, _p(kinds, _I), arr, len(atoms),
-                          _F(potOffsetX), _F(potOffsetY), _F(czOffset), centerSlices, _F(atomRadius), periodic,
-                          _p(V, _F))
+    load().qo_make_slices_q(ctypes.byref(p), ctypes.byref(g), lut_ptrs, offs_ptrs, _p(kinds, _I), arr, len(atoms),
+                            _F(potOffsetX), _F(potOffsetY), _F(czOffset), centerSlices, _F(atomRadius), periodic,
+                            _p(V, _F))
     return V
 
 

@@ -15,11 +15,18 @@
  *   qo_multislice   empty specimen = free propagation (analytic in k);
  *                   detector covering all k = electronScale*sum|psi|^2/N^2;
  *                   reference fixture mulswav_16_2.img norm (statistical)    tests/test_oracle_multislice.py
- *   qo_atom_lut     DC ratio between elements ~ f_e(0) ratio; clamp >= 0    tests/test_oracle_potential.py
- *   qo_make_slices  projected sum over slices vs analytic 2D projection of
- *                   the same LUT; translation invariance                      tests/test_oracle_potential.py
- * Parity unpinned: the absolute LUT scale 47.8658*dkx*dkz/nz and the exact
- * z-sampling offsets of make3DSlices have no reference-produced fixture.
+ *   qo_atom_lut     slab sums of the continuous radial transform of f_e    tests/test_oracle_potential.py
+ *   qo_atom_lut2    absolute level: slice sum = 47.8658 f_e(0)/(4 L_z) with
+ *                   L_z the LUT z period (clamp off, to 3.5 %); norm switch
+ *                   -> f_e(0); clamp only adds                               tests/test_oracle_potential_level.py
+ *                   ion offsets: same path with the ion table (equality with
+ *                   the neutral LUT given one table), charge term linear,
+ *                   Q5 level near the neutral one; offset values and Q5
+ *                   pinned statistically by the reference's charged-specimen
+ *                   output                                                   tests/test_gpu_reference_fixtures.py
+ *   qo_make_slices  translation invariance, x<->y symmetry, peak slice       tests/test_oracle_potential.py
+ * Parity unpinned: the exact z-sampling offsets of make3DSlices (rounding of
+ * negative offsets, the +Nr row) have no reference-produced fixture.
  */
 #include "qs_oracle.h"
 
@@ -486,6 +493,32 @@ void qo_taper_scatpar(double *sgrid, double *rows, int nrows, double smax) {
  * restatement keeps everything double. */
 void qo_atom_lut(const qo_lut_geom *g, const double *sgrid, const double *frow, double B,
                  float sliceThickness, float *lut) {
+  qo_atom_lut2(g, sgrid, frow, sgrid, frow, g->dkx, B, sliceThickness, 1.0, 1, lut);
+}
+
+/* Scale that turns the reference's LUT normalisation 47.8658 dkx dkz / nz
+ * (stemlib.cpp:1516, :1720) into 4 dkx dkz dz/nzPerSlice: the slice-summed
+ * projected potential of an atom then integrates to f_e(0) (the phase integral
+ * gamma*lambda*f_e(0)) at every slice thickness, instead of to
+ * 47.8658 f_e(0) / (4 L_z) with L_z = nz dz / nzPerSlice = 2 ceil(R/dz) dz the
+ * LUT's z period.  The 4 undoes the radial factor 1/4 of reading the LUT (radial
+ * spacing dx) at dr = dx/2 (stemlib.cpp:576).  Not a reference routine. */
+double qo_lut_norm_fix(const qo_lut_geom *g, float sliceThickness) {
+  return 4.0 * g->nz * ((double)sliceThickness / g->nzPerSlice) / 47.8658;
+}
+
+/* General LUT builder: getAtomPotential3D (stemlib.cpp:1419-1532) when
+ * (sgrid_y, frow_y) = (sgrid, frow) and dky = dkx; getAtomPotentialOffset3D
+ * (stemlib.cpp:1629-1725) with (sgrid, frow) the ion-offset table.  The spline
+ * is fitted to (sgrid, frow) and evaluates the (kx, kz) term; the q_y sum
+ * evaluates seval(sgrid_y, frow_y, <that spline's b, c, d>) with step dky, which
+ * for the ion offsets with frow_y the neutral row is the reference's quirk Q5
+ * (stemlib.cpp:1672).  `scale` multiplies the result (1 = reference); clamp = 1
+ * sets negative slab sums to zero as the reference does (stemlib.cpp:1508); the
+ * tests turn it off to compare the LUT's integral with f_e(0) exactly. */
+void qo_atom_lut2(const qo_lut_geom *g, const double *sgrid, const double *frow, const double *sgrid_y,
+                  const double *frow_y, double dky, double B, float sliceThickness, double scale, int clamp,
+                  float *lut) {
   const int nsf = 123, nx = g->nx, nz = g->nz, nzp = g->nzPerSlice;
   const double smax2 = g->smax2 * g->smax2, dkx = g->dkx, dkz = g->dkz;
   double b[123], c[123], d[123];
@@ -501,9 +534,9 @@ void qo_atom_lut(const qo_lut_geom *g, const double *sgrid, const double *frow, 
       if (s2 < smax2) {
         double f = qo_seval(sgrid, frow, b, c, d, nsf, sqrt(s2)) * exp(-s2 * B * 0.25);
         for (int iy = 1; iy < nx; iy++) {
-          double s3 = dkx * iy;
+          double s3 = dky * iy;
           s3 = s3 * s3 + s2;
-          if (s3 < smax2) f += 2 * qo_seval(sgrid, frow, b, c, d, nsf, sqrt(s3)) * exp(-s3 * B * 0.25);
+          if (s3 < smax2) f += 2 * qo_seval(sgrid_y, frow_y, b, c, d, nsf, sqrt(s3)) * exp(-s3 * B * 0.25);
           else break;
         }
         f *= dkx;
@@ -518,8 +551,8 @@ void qo_atom_lut(const qo_lut_geom *g, const double *sgrid, const double *frow, 
       double zs = 0;
       for (int iiz = -izOffset; iiz <= izOffset; iiz++)
         if (iz + izOffset + iiz < nz / 2) zs += creal(tmp[ix + (size_t)(iz + izOffset + iiz) * nx]);
-      if (zs < 0) zs = 0;
-      lut[ix + (size_t)iz * (nx / 2)] = (float)(47.8658 * dkx * dkz / nz * zs);
+      if (clamp && zs < 0) zs = 0;
+      lut[ix + (size_t)iz * (nx / 2)] = (float)(47.8658 * dkx * dkz / nz * zs * scale);
     }
   free(tmp);
 }
@@ -530,6 +563,18 @@ void qo_make_slices(const qo_params *p, const qo_lut_geom *g, const float *const
                     const int *kind_of_atom, const qo_atom *atoms, int natom,
                     float potOffsetX, float potOffsetY, float czOffset, int centerSlices,
                     float atomRadius, int periodic, float *V) {
+  qo_make_slices_q(p, g, luts, NULL, kind_of_atom, atoms, natom, potOffsetX, potOffsetY, czOffset, centerSlices,
+                   atomRadius, periodic, V);
+}
+
+/* qo_make_slices with ion charges (USE_Q_POT_OFFSETS, stemlib.cpp:77): an atom of
+ * charge q != 0 adds q times the same r-linear / z-nearest sample of its
+ * element's offset LUT offs_luts[kind] (stemlib.cpp:966-979, :993-1006,
+ * :1150-1163, :1176-1189).  offs_luts may be NULL (no charged atoms). */
+void qo_make_slices_q(const qo_params *p, const qo_lut_geom *g, const float *const *luts,
+                      const float *const *offs_luts, const int *kind_of_atom, const qo_atom *atoms, int natom,
+                      float potOffsetX, float potOffsetY, float czOffset, int centerSlices,
+                      float atomRadius, int periodic, float *V) {
   const int nx = p->potNx, ny = p->potNy, nslc = p->slices;
   const float dx = p->resolutionX, dy = p->resolutionY, dzs = p->sliceThickness;
   const float dr = p->resolutionX / 2;
@@ -565,6 +610,8 @@ void qo_make_slices(const qo_params *p, const qo_lut_geom *g, const float *const
     const int iaz1 = iAtomZ + iRadZ >= nslc ? nslc - iAtomZ - 1 : iRadZ;
     if (!(iAtomZ + iaz0 < nslc && iAtomZ + iaz1 >= 0)) continue;
     const float *lut = luts[kind_of_atom[ia]];
+    const float q = atoms[ia].q;
+    const float *olut = (q != 0 && offs_luts) ? offs_luts[kind_of_atom[ia]] : NULL;
     const float zrel = atomZ / dzs;
     const double dOffsZ = (float)((float)(iAtomZ + iaz0) - zrel) * nzSub;
     const int iOffsZ0 = (int)(dOffsZ + 0.5) * Nr;
@@ -585,9 +632,13 @@ void qo_make_slices(const qo_params *p, const qo_lut_geom *g, const float *const
         for (int iaz = iaz0; iaz <= iaz1; iaz++) {
           double pv = 0;
           if (iOffsZ < 0) {
-            if (iOffsZ > limLo) pv = (1 - ddr) * lut[ir - iOffsZ + Nr] + ddr * lut[ir + 1 - iOffsZ + Nr];
+            if (iOffsZ > limLo) {
+              pv = (1 - ddr) * lut[ir - iOffsZ + Nr] + ddr * lut[ir + 1 - iOffsZ + Nr];
+              if (olut) pv += q * ((1 - ddr) * olut[ir - iOffsZ + Nr] + ddr * olut[ir + 1 - iOffsZ + Nr]);
+            }
           } else if (iOffsZ < limHi) {
             pv = (1 - ddr) * lut[ir + iOffsZ] + ddr * lut[ir + 1 + iOffsZ];
+            if (olut) pv += q * ((1 - ddr) * olut[ir + iOffsZ] + ddr * olut[ir + 1 + iOffsZ]);
           }
           float *cell = V + ((size_t)(iAtomZ + iaz) * nx + ixw) * ny + iyw;
           *cell = (float)(*cell + pv);

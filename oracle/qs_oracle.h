@@ -94,6 +94,10 @@ void qo_lut_geometry(float atomRadius, float resX, float sliceThickness, qo_lut_
 void qo_taper_scatpar(double *sgrid, double *rows, int nrows, double smax);
 void qo_atom_lut(const qo_lut_geom *g, const double *sgrid, const double *frow, double B,
                  float sliceThickness, float *lut /* [Nz_lut][Nr] */);
+void qo_atom_lut2(const qo_lut_geom *g, const double *sgrid, const double *frow, const double *sgrid_y,
+                  const double *frow_y, double dky, double B, float sliceThickness, double scale, int clamp,
+                  float *lut);
+double qo_lut_norm_fix(const qo_lut_geom *g, float sliceThickness);
 typedef struct qo_atom { float x, y, z, dw, occ, q; int Znum; } qo_atom;
 /* 3D potential scatter for non-periodic (periodic=0) or periodic (1) xy; V is [slices][potNx][potNy].
  * luts[k] is the LUT for element kind k; kind_of_atom[i] indexes it. Atoms must be z-sorted. */
@@ -101,6 +105,11 @@ void qo_make_slices(const qo_params *p, const qo_lut_geom *g, const float *const
                     const int *kind_of_atom, const qo_atom *atoms, int natom,
                     float potOffsetX, float potOffsetY, float czOffset, int centerSlices,
                     float atomRadius, int periodic, float *V);
+
+void qo_make_slices_q(const qo_params *p, const qo_lut_geom *g, const float *const *luts,
+                      const float *const *offs_luts, const int *kind_of_atom, const qo_atom *atoms, int natom,
+                      float potOffsetX, float potOffsetY, float czOffset, int centerSlices,
+                      float atomRadius, int periodic, float *V);
 
 void qo_make_slices_d(const qo_params *p, const qo_lut_geom *g, const float *const *luts,
                       const int *kind_of_atom, const double *xyz, const int *Znum, int natom,
