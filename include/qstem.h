@@ -103,21 +103,35 @@ typedef struct qs_pot_params {
   int tds;                      /* frozen phonons: B := 0 in the LUT (stemlib.cpp:914) */
   int compat_q6;                /* cache the LUT by Z only, first B in z order wins (stemlib.cpp:1422) */
   int lut_dr_fix;               /* 0: sample the LUT with dr = resolutionX/2 as make3DSlices does
-                                 * (stemlib.cpp:701) although its radial spacing is resolutionX
-                                 * (dkx, stemlib.cpp:1383), i.e. atoms radially compressed 2x (Q10);
-                                 * 1: sample at the LUT's own spacing */
+                                 * (stemlib.cpp:576); the LUT is built with f_e(s) read at s = |k|
+                                 * (a potential stretched 2x in r and z) on a radial spacing of
+                                 * resolutionX, so this read restores the radial shape (Q10);
+                                 * 1: sample at the LUT's own spacing (atoms 2x wider, 4x the
+                                 * projected integral), a comparison variant */
+  int compat_q5;                /* ion-offset LUT of charged atoms: 1 = the reference's q_y sum
+                                 * over the neutral table's nodes with the ion table's spline
+                                 * (stemlib.cpp:1672, quirk Q5); 0 = over the ion table */
+  int lut_norm_fix;             /* 1: scale the LUTs by 4 L_z / 47.8658 (L_z = 2 ceil(R/dz) dz, the
+                                 * LUT's z period) so that an atom's slice-summed projected
+                                 * potential integrates to f_e(0) at every slice thickness; the
+                                 * reference's 47.8658 dkx dkz / nz (stemlib.cpp:1516) gives
+                                 * 47.8658 f_e(0) / (4 L_z) (with dr = resolutionX/2); 0 = reference */
 } qs_pot_params;
 
 /* Rez et al. electron scattering factors: 93 rows x 123 doubles, row 0 = s grid,
  * row Z = f_e(s) (stem3/scatfactsRez.h:22-25).  Required before qs_build_potential. */
 int qs_set_scattering_table(qs_ctx *ctx, const double *table);
+/* Ion-offset scattering factors df_e/dq, same layout (stem3/scatfactsRez.h:122,
+ * scatParOffs).  Required before qs_build_potential with charged atoms. */
+int qs_set_ion_scattering_table(qs_ctx *ctx, const double *table);
 
-/* make3DSlices + getAtomPotential3D + initSTEMSlices: builds the projected potential of
- * the given atoms into qs_params.slices slices on the device, then the transmission
- * functions.  Neutral atoms and the 3D potential only (QS_ERR_UNSUPPORTED otherwise).
- * V_out (nullable) receives V [slices][potNx][potNy].  Each voxel is the FP32 sum of the
- * atoms' LUT samples, gathered per voxel from atoms binned by pixel tile, in an
- * unspecified order. */
+/* make3DSlices + getAtomPotential3D (+ getAtomPotentialOffset3D for charged atoms, the
+ * reference's USE_Q_POT_OFFSETS build, stemlib.cpp:77, :1548-1741) + initSTEMSlices:
+ * builds the projected potential of the given atoms into qs_params.slices slices on
+ * the device, then the transmission functions.  3D potential only.  V_out (nullable)
+ * receives V [slices][potNx][potNy].  Each voxel is the FP32 sum of the atoms' LUT
+ * samples (plus q times the offset LUT's sample for charged atoms), gathered per
+ * voxel from atoms binned by pixel tile, in an unspecified order. */
 int qs_build_potential(qs_ctx *ctx, const qs_atom *atoms, int natom, const qs_pot_params *pp, float *V_out);
 
 /* Replaces the probe optics of the context (Cs, C5, df0, astigmatism, a_nm/phi_nm,
@@ -251,7 +265,9 @@ int qs_forward_save(qs_ctx *ctx, const int *ipos, int nprobe, const float *targe
 int qs_last_ckpt_interval(const qs_ctx *ctx);
 int qs_adjoint_saved(qs_ctx *ctx, float *grad_V, float *grad_probe);
 /* dL/d(x, y, z) of the atoms of the last qs_build_potential (same atoms, same pp), from the
- * grad_V of the last qs_adjoint; z is 0 (the potential samples z by nearest slice). */
+ * grad_V of the last qs_adjoint; z is 0 (the potential samples z by nearest slice).  Fails
+ * with QS_ERR_ARG when the atoms' count, Z or charge or pp differ from that build, and with
+ * QS_ERR_STATE when the potential was set otherwise since. */
 int qs_atom_gradients(qs_ctx *ctx, const qs_atom *atoms, int natom, const qs_pot_params *pp, float *grad_xyz);
 
 /* Probes advanced together through all slices before the next group starts

@@ -58,7 +58,8 @@ class QsAtom(ctypes.Structure):
 
 class QsPotParams(ctypes.Structure):
     _fields_ = [("atomRadius", _F), ("potOffsetX", _F), ("potOffsetY", _F), ("czOffset", _F),
-                ("centerSlices", _I), ("periodicXY", _I), ("tds", _I), ("compat_q6", _I), ("lut_dr_fix", _I)]
+                ("centerSlices", _I), ("periodicXY", _I), ("tds", _I), ("compat_q6", _I), ("lut_dr_fix", _I),
+                ("compat_q5", _I), ("lut_norm_fix", _I)]
 
 
 class QsGroup(ctypes.Structure):
@@ -82,6 +83,7 @@ SIGNATURES = {
     "qs_make_probe": (_I, [_P, _FP]),
     "qs_set_optics": (_I, [_P, ctypes.POINTER(QsParams)]),
     "qs_set_scattering_table": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
+    "qs_set_ion_scattering_table": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "qs_build_potential": (_I, [_P, ctypes.POINTER(QsAtom), _I, ctypes.POINTER(QsPotParams), _FP]),
     "qs_record_bins": (_I, [_I, _I, _I, _I, _IP]),
     "qs_set_incident_wave": (_I, [_P, _FP]),

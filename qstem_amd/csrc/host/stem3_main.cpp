@@ -50,12 +50,21 @@ std::string dirname_of(const std::string& p) {
   return k == std::string::npos ? std::string(".") : p.substr(0, k);
 }
 
-std::string find_data_file(const char* argv0) {
-  if (const char* e = getenv("QSTEM_DATA")) return std::string(e) + "/rez_scatpar_f64.bin";
+std::string find_data_file(const char* argv0, const char* name) {
+  if (const char* e = getenv("QSTEM_DATA")) return std::string(e) + "/" + name;
   char buf[4096];
   ssize_t n = readlink("/proc/self/exe", buf, sizeof buf - 1);
   std::string exe = n > 0 ? std::string(buf, n) : std::string(argv0);
-  return dirname_of(dirname_of(exe)) + "/data/rez_scatpar_f64.bin";
+  return dirname_of(dirname_of(exe)) + "/data/" + name;
+}
+
+std::vector<double> read_table(const char* argv0, const char* name) {
+  std::vector<double> table(93 * 123);
+  const std::string tp = find_data_file(argv0, name);
+  FILE* f = fopen(tp.c_str(), "rb");
+  if (!f || fread(table.data(), 8, table.size(), f) != table.size()) die("cannot read scattering table " + tp);
+  fclose(f);
+  return table;
 }
 
 }  // namespace
@@ -282,12 +291,12 @@ int main(int argc, char** argv) {
   qs_ctx* ctx = nullptr;
   if (qs_create(&P, &ctx) != QS_OK) die(std::string("qs_create: ") + qs_last_error(nullptr));
   {
-    std::vector<double> table(93 * 123);
-    const std::string tp = find_data_file(argv[0]);
-    FILE* f = fopen(tp.c_str(), "rb");
-    if (!f || fread(table.data(), 8, table.size(), f) != table.size()) die("cannot read scattering table " + tp);
-    fclose(f);
+    // Rez et al. neutral and ion-offset tables (stem3/scatfactsRez.h): the reference's
+    // default build adds ion offsets (USE_Q_POT_OFFSETS 1, stemlib.cpp:77)
+    const std::vector<double> table = read_table(argv[0], "rez_scatpar_f64.bin");
     if (qs_set_scattering_table(ctx, table.data()) != QS_OK) die(qs_last_error(ctx));
+    const std::vector<double> ions = read_table(argv[0], "rez_scatparoffs_f64.bin");
+    if (qs_set_ion_scattering_table(ctx, ions.data()) != QS_OK) die(qs_last_error(ctx));
   }
   mkdir(folder.c_str(), 0755);
   const int nprobe = scanXN * scanYN;
@@ -333,6 +342,7 @@ int main(int argc, char** argv) {
       pp.periodicXY = periodicXY;
       pp.tds = tds;
       pp.compat_q6 = 1;
+      pp.compat_q5 = 1;  // the reference's ion-offset LUT (stemlib.cpp:1672)
       std::vector<float> V;
       if (savePotential) V.resize((size_t)slices * potNx * potNy);
       if (qs_build_potential(ctx, atoms.data(), (int)atoms.size(), &pp, savePotential ? V.data() : nullptr) != QS_OK)

@@ -101,7 +101,8 @@ __global__ void k_probe_grad(ProbeArgs a, const float2* __restrict__ G, double c
 // dL/d(atom x, y): chain rule through the r-linear LUT lookup of the scatter
 // (the z lookup is nearest-sample, piecewise constant, so dL/dz = 0).
 __global__ __launch_bounds__(256) void k_atom_grad(const QsAtomDev* __restrict__ atoms, int natom,
-                                                   const float* const* __restrict__ luts, const int* __restrict__ kind,
+                                                   const float* const* __restrict__ luts,
+                                                   const float* const* __restrict__ oluts, const int* __restrict__ kind,
                                                    ScatterArgs s, const float* __restrict__ gV, float* __restrict__ grad) {
   __shared__ double shx[256], shy[256];
   const int ia = blockIdx.x;
@@ -133,6 +134,8 @@ __global__ __launch_bounds__(256) void k_atom_grad(const QsAtomDev* __restrict__
   if (!(iAtomZ + iaz0 < s.slices && iAtomZ + iaz1 >= 0)) active = false;
   if (active) {
     const gfloat* lut = (const gfloat*)luts[kind[ia]];
+    const gfloat* olut = (at.q != 0.f && oluts) ? (const gfloat*)oluts[kind[ia]] : nullptr;
+    const double q = at.q;
     const float zrel = atomZ / dzs;
     const double dOffsZ = (double)((float)((float)(iAtomZ + iaz0) - zrel) * (float)s.nzSub);
     const int iOffsZ0 = (int)(dOffsZ + 0.5) * s.Nr;
@@ -156,9 +159,13 @@ __global__ __launch_bounds__(256) void k_atom_grad(const QsAtomDev* __restrict__
       for (int iaz = iaz0; iaz <= iaz1; iaz++) {
         double slope = 0;
         if (iOffsZ < 0) {
-          if (iOffsZ > limLo) slope = (lut[ir + 1 - iOffsZ + s.Nr] - lut[ir - iOffsZ + s.Nr]) / dr;
+          if (iOffsZ > limLo) {
+            slope = (lut[ir + 1 - iOffsZ + s.Nr] - lut[ir - iOffsZ + s.Nr]) / dr;
+            if (olut) slope += q * (olut[ir + 1 - iOffsZ + s.Nr] - olut[ir - iOffsZ + s.Nr]) / dr;
+          }
         } else if (iOffsZ < limHi) {
           slope = (lut[ir + 1 + iOffsZ] - lut[ir + iOffsZ]) / dr;
+          if (olut) slope += q * (olut[ir + 1 + iOffsZ] - olut[ir + iOffsZ]) / dr;
         }
         dVdr_g += slope * (double)(*g);
         g += slice;
@@ -214,9 +221,10 @@ void launch_probe_grad(const ProbeArgs& a, const float2* G, double cscale, doubl
   hipLaunchKernelGGL(k_probe_grad, dim3(gridn(n) > 1024 ? 1024 : gridn(n)), dim3(256), 0, st, a, G, cscale, out2);
 }
 
-void launch_atom_grad(const QsAtomDev* atoms, int natom, const float* const* luts, const int* kind,
-                      const ScatterArgs& s, const float* gV, float* grad, hipStream_t st) {
-  if (natom > 0) hipLaunchKernelGGL(k_atom_grad, dim3(natom), dim3(256), 0, st, atoms, natom, luts, kind, s, gV, grad);
+void launch_atom_grad(const QsAtomDev* atoms, int natom, const float* const* luts, const float* const* oluts,
+                      const int* kind, const ScatterArgs& s, const float* gV, float* grad, hipStream_t st) {
+  if (natom > 0)
+    hipLaunchKernelGGL(k_atom_grad, dim3(natom), dim3(256), 0, st, atoms, natom, luts, oluts, kind, s, gV, grad);
 }
 
 void launch_cplx_fill_zero(float* p, size_t nfloats, hipStream_t st) {

@@ -167,8 +167,17 @@ struct qs_ctx {
   } bl;
   DevBuf blpx, blpy, blstate, blstate2;
   // potential builder
-  std::vector<double> scat;  // 93 x 123 Rez table
-  DevBuf atoms, kinds, lutptrs, spl, luttmp, lutscr;
+  std::vector<double> scat;       // 93 x 123 Rez table
+  std::vector<double> scat_offs;  // 93 x 123 ion-offset table (scatParOffs)
+  DevBuf atoms, kinds, lutptrs, olutptrs, spl, luttmp, lutscr;
+  std::vector<DevBuf> oluts;      // ion-offset LUT per kind (empty buffer: no charged atom of the kind)
+  // what the last qs_build_potential built (qs_atom_gradients reuses its LUTs and kinds)
+  struct BuiltPot {
+    bool valid = false;
+    std::vector<int> Z;
+    std::vector<float> q;
+    qs_pot_params pp{};
+  } built;
   DevBuf reach, tcount, tstart, tlist;  // gather form: per-atom reach, atoms binned by pixel tile
   std::vector<DevBuf> luts;
   RowXform lx, lz;  // LUT transform lengths
@@ -496,6 +505,8 @@ void qs_destroy(qs_ctx* c) {
                     &c->blpx, &c->blpy, &c->blstate, &c->blstate2, &c->adj_ipos};
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : c->luts) b.release();
+  for (DevBuf& b : c->oluts) b.release();
+  c->olutptrs.release();
   for (DevBuf& b : c->ckpt) b.release();
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
@@ -636,6 +647,7 @@ int qs_set_potential(qs_ctx* c, const float* V, int nslice) {
   const size_t n = (size_t)nslice * p.potNx * p.potNy;
   HIPCHK(c, c->V.ensure(sizeof(float) * n));
   HIPCHK(c, hipMemcpyAsync(c->V.p, V, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+  c->built.valid = false;  // the potential no longer comes from the last qs_build_potential
   const int rc = build_trans(c, nslice);
   c->v_valid = rc == QS_OK;
   return rc;
@@ -668,16 +680,42 @@ int qs_set_scattering_table(qs_ctx* c, const double* table) {
   return QS_OK;
 }
 
+int qs_set_ion_scattering_table(qs_ctx* c, const double* table) {
+  if (!c || !table) return fail(c, QS_ERR_ARG, "qs_set_ion_scattering_table: bad arguments");
+  c->scat_offs.assign(table, table + 93 * 123);
+  return QS_OK;
+}
+
+// Taper of a scattering-factor table's s grid at smax (stemlib.cpp:1390-1413, :1598-1620):
+// the last three nodes become 1.0, 1.1, 1.2 smax and nodes beyond smax step down with zero
+// rows.
+static void taper_table(std::vector<double>& tab, double smax) {
+  double* sgrid = tab.data();
+  sgrid[122] = 1.2 * smax;
+  sgrid[121] = 1.1 * smax;
+  sgrid[120] = smax;
+  if (sgrid[119] > sgrid[120]) {
+    for (int ix = 0; ix < 123 - 10; ix++) {
+      if (sgrid[119 - ix] < sgrid[120] - 0.001 * (ix + 1)) break;
+      sgrid[119 - ix] = sgrid[120] - 0.001 * (ix + 1);
+      for (int r = 1; r < 93; r++) tab[(size_t)r * 123 + 119 - ix] = 0;
+    }
+  }
+}
+
 int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_params* pp, float* V_out) {
   if (!c || !pp || natom < 0 || (natom > 0 && !atoms)) return fail(c, QS_ERR_ARG, "qs_build_potential: bad arguments");
   invalidate(c);  // kernel arguments baked into a captured graph may change
   if (c->scat.size() != 93 * 123) return fail(c, QS_ERR_STATE, "qs_build_potential: scattering table not set");
   const qs_params& p = c->p;
+  c->built.valid = false;
+  bool charged = false;
   for (int i = 0; i < natom; i++) {
     if (atoms[i].Znum < 0 || atoms[i].Znum > 92) return fail(c, QS_ERR_ARG, "qs_build_potential: Z out of range");
-    if (atoms[i].Znum > 0 && atoms[i].q != 0.f)
-      return fail(c, QS_ERR_UNSUPPORTED, "qs_build_potential: charged atoms (ion offsets) are not supported");
+    if (atoms[i].Znum > 0 && atoms[i].q != 0.f) charged = true;
   }
+  if (charged && c->scat_offs.size() != 93 * 123)
+    return fail(c, QS_ERR_STATE, "qs_build_potential: charged atoms need qs_set_ion_scattering_table");
   // LUT geometry (stemlib.cpp:1366-1388), float products as the reference
   const float R = pp->atomRadius, dx = p.resolutionX, dz = p.sliceThickness;
   LutArgsHost g;
@@ -690,19 +728,12 @@ int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_
   g.dkz = nzp / (double)(float)(g.nz * dz);
   const double smax = 0.5 * g.nx * g.dkx / 2.0;
   if (g.nx > 2048 || g.nz > 2048) return fail(c, QS_ERR_UNSUPPORTED, "qs_build_potential: LUT grid above 2048");
-  // taper of the shared s grid (stemlib.cpp:1390-1413)
+  // taper of the shared s grid (stemlib.cpp:1390-1413), and of the ion table's (:1598-1620)
   std::vector<double> tab = c->scat;
-  double* sgrid = tab.data();
-  sgrid[122] = 1.2 * smax;
-  sgrid[121] = 1.1 * smax;
-  sgrid[120] = smax;
-  if (sgrid[119] > sgrid[120]) {
-    for (int ix = 0; ix < 123 - 10; ix++) {
-      if (sgrid[119 - ix] < sgrid[120] - 0.001 * (ix + 1)) break;
-      sgrid[119 - ix] = sgrid[120] - 0.001 * (ix + 1);
-      for (int r = 1; r < 93; r++) tab[(size_t)r * 123 + 119 - ix] = 0;
-    }
-  }
+  taper_table(tab, smax);
+  const double* sgrid = tab.data();
+  std::vector<double> otab = c->scat_offs;
+  if (charged) taper_table(otab, smax);
   // distinct LUT kinds: (Z, B) keys, or Z only with the B of the lowest atom in z (compat Q6)
   std::vector<std::pair<int, float>> keys;
   std::vector<int> kind(natom, 0);
@@ -719,6 +750,9 @@ int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_
     if (k < 0) { keys.push_back({atoms[i].Znum, B}); k = (int)keys.size() - 1; }
     kind[i] = k;
   }
+  std::vector<char> need_off(keys.size(), 0);  // kinds with a charged atom get an offset LUT
+  for (int i = 0; i < natom; i++)
+    if (atoms[i].Znum > 0 && atoms[i].q != 0.f) need_off[kind[i]] = 1;
   int rc = setup_rowxform(c, c->lx, g.nx);
   if (rc == QS_OK) rc = setup_rowxform(c, c->lz, g.nz);
   if (rc != QS_OK) return rc;
@@ -726,30 +760,58 @@ int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_
   la.nx = g.nx; la.nz = g.nz; la.nzPerSlice = nzp;
   la.dkx = g.dkx; la.dkz = g.dkz; la.smax2 = smax * smax;
   la.zPos = -2.0 * M_PI * ((double)dz / nzp * ((nzp - 1) / 2));
+  la.dky = g.dkx;
+  // normalisation switch: slice sum of an atom = f_e(0) instead of 47.8658 f_e(0) / (4 L_z)
+  la.scale = pp->lut_norm_fix ? 4.0 * g.nz * ((double)dz / nzp) / 47.8658 : 1.0;
   const int Nr = g.nx / 2, Nz = g.nz / 2;
   c->luts.resize(keys.size());
-  HIPCHK(c, c->spl.ensure(sizeof(double) * 5 * 123));
+  for (DevBuf& b : c->oluts) b.release();
+  c->oluts.resize(keys.size());
+  HIPCHK(c, c->spl.ensure(sizeof(double) * 7 * 123));
   HIPCHK(c, c->luttmp.ensure(sizeof(float2) * (size_t)g.nx * g.nz));
   HIPCHK(c, c->lutscr.ensure(sizeof(float2) * (size_t)g.nx * g.nz));
-  std::vector<const float*> lutp(keys.size());
-  std::vector<double> spl(5 * 123);
-  for (size_t k = 0; k < keys.size(); k++) {
-    std::copy(sgrid, sgrid + 123, spl.begin());
-    std::copy(tab.begin() + (size_t)keys[k].first * 123, tab.begin() + (size_t)keys[k].first * 123 + 123,
-              spl.begin() + 123);
+  std::vector<const float*> lutp(keys.size()), olutp(keys.size(), nullptr);
+  std::vector<double> spl(7 * 123);
+  // one LUT: spline fitted to (sg, row) for the (kx, kz) term, q_y sum over the nodes
+  // (qsg, qrow) with that spline's coefficients and step dky (stemlib.cpp:1419-1532)
+  auto make_lut = [&](const double* sg, const double* row, const double* qsg, const double* qrow, double dky,
+                      double B, DevBuf& out) -> int {
+    std::copy(sg, sg + 123, spl.begin());
+    std::copy(row, row + 123, spl.begin() + 123);
     akima(&spl[0], &spl[123], &spl[246], &spl[369], &spl[492], 123);
+    std::copy(qsg, qsg + 123, spl.begin() + 615);
+    std::copy(qrow, qrow + 123, spl.begin() + 738);
     HIPCHK(c, hipMemcpyAsync(c->spl.p, spl.data(), sizeof(double) * spl.size(), hipMemcpyHostToDevice, c->stream));
-    HIPCHK(c, c->luts[k].ensure(sizeof(float) * (size_t)Nr * Nz));
+    HIPCHK(c, out.ensure(sizeof(float) * (size_t)Nr * Nz));
     float2* tmp = c->luttmp.as<float2>();
     float2* scr = c->lutscr.as<float2>();
-    qs::launch_lut_fk(c->spl.as<double>(), la, (double)keys[k].second, tmp, c->stream);
+    qs::LutArgs lk = la;
+    lk.dky = dky;
+    qs::launch_lut_fk(c->spl.as<double>(), lk, B, tmp, c->stream);
     run_rowxform(c, c->lx, tmp, g.nz, +1, 1.0f);
     qs::launch_transpose(tmp, scr, 1, g.nz, g.nx, c->stream);
     run_rowxform(c, c->lz, scr, g.nx, +1, 1.0f);
     qs::launch_transpose(scr, tmp, 1, g.nx, g.nz, c->stream);
-    qs::launch_lut_extract(tmp, la, c->luts[k].as<float>(), c->stream);
+    qs::launch_lut_extract(tmp, lk, out.as<float>(), c->stream);
+    HIPCHK(c, hipStreamSynchronize(c->stream));  // spl is reused by the next LUT
+    return QS_OK;
+  };
+  // getAtomPotentialOffset3D's q_y step uses resolutionY (stemlib.cpp:1594)
+  const int lny = 2 * 2 * (int)ceil(R / p.resolutionY);
+  const double dky_off = 0.5 * 2 / (double)(float)(lny * p.resolutionY);
+  for (size_t k = 0; k < keys.size(); k++) {
+    const double* row = tab.data() + (size_t)keys[k].first * 123;
+    rc = make_lut(sgrid, row, sgrid, row, g.dkx, (double)keys[k].second, c->luts[k]);
+    if (rc != QS_OK) return rc;
     lutp[k] = c->luts[k].as<float>();
-    HIPCHK(c, hipStreamSynchronize(c->stream));  // spl is reused by the next kind
+    if (need_off[k]) {
+      const double* orow = otab.data() + (size_t)keys[k].first * 123;
+      const double* qsg = pp->compat_q5 ? sgrid : otab.data();
+      const double* qrow = pp->compat_q5 ? row : orow;
+      rc = make_lut(otab.data(), orow, qsg, qrow, dky_off, (double)keys[k].second, c->oluts[k]);
+      if (rc != QS_OK) return rc;
+      olutp[k] = c->oluts[k].as<float>();
+    }
   }
   // atoms, kinds, LUT pointers
   const size_t nV = (size_t)p.slices * p.potNx * p.potNy;
@@ -762,6 +824,11 @@ int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_
     HIPCHK(c, hipMemcpyAsync(c->atoms.p, atoms, sizeof(qs_atom) * natom, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->kinds.p, kind.data(), sizeof(int) * natom, hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->lutptrs.p, lutp.data(), sizeof(float*) * keys.size(), hipMemcpyHostToDevice, c->stream));
+    if (charged) {
+      HIPCHK(c, c->olutptrs.ensure(sizeof(float*) * keys.size()));
+      HIPCHK(c, hipMemcpyAsync(c->olutptrs.p, olutp.data(), sizeof(float*) * keys.size(), hipMemcpyHostToDevice,
+                               c->stream));
+    }
     qs::ScatterArgs sa{};
     sa.potNx = p.potNx; sa.potNy = p.potNy; sa.slices = p.slices;
     sa.dx = p.resolutionX; sa.dy = p.resolutionY; sa.sliceThickness = dz;
@@ -798,14 +865,24 @@ int qs_build_potential(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_
     HIPCHK(c, hipMemcpyAsync(c->tstart.p, start.data(), sizeof(int) * (ntile + 1), hipMemcpyHostToDevice, c->stream));
     HIPCHK(c, hipMemcpyAsync(c->tcount.p, start.data(), sizeof(int) * ntile, hipMemcpyHostToDevice, c->stream));
     qs::launch_bin_atoms(reach, natom, sa, zblocks, c->tcount.as<int>(), c->tlist.as<int>(), true, c->stream);
-    qs::launch_gather(reach, c->tstart.as<int>(), c->tlist.as<int>(), c->lutptrs.as<const float*>(), sa,
-                      c->V.as<float>(), c->stream);
+    qs::launch_gather(reach, c->tstart.as<int>(), c->tlist.as<int>(), c->lutptrs.as<const float*>(),
+                      charged ? c->olutptrs.as<const float*>() : nullptr, sa, c->V.as<float>(), c->stream);
     HIPCHK(c, hipStreamSynchronize(c->stream));  // the host arrays above are copied from
   }
   HIPCHK(c, hipGetLastError());
   if (V_out) HIPCHK(c, hipMemcpyAsync(V_out, c->V.p, sizeof(float) * nV, hipMemcpyDeviceToHost, c->stream));
   rc = build_trans(c, p.slices);
   c->v_valid = rc == QS_OK;
+  if (rc == QS_OK) {
+    c->built.Z.resize(natom);
+    c->built.q.resize(natom);
+    for (int i = 0; i < natom; i++) {
+      c->built.Z[i] = atoms[i].Znum;
+      c->built.q[i] = atoms[i].Znum > 0 ? atoms[i].q : 0.f;
+    }
+    c->built.pp = *pp;
+    c->built.valid = true;
+  }
   return rc;
 }
 
@@ -819,6 +896,7 @@ int qs_set_trans(qs_ctx* c, const float* trans, int nslice) {
   HIPCHK(c, hipMemcpyAsync(t, trans, sizeof(float2) * n, hipMemcpyHostToDevice, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   c->v_valid = false;
+  c->built.valid = false;
   return QS_OK;
 }
 
@@ -990,6 +1068,7 @@ int qs_set_configs(qs_ctx* c, int nconfig) {
   c->cur_config = 0;
   c->trans_slices = 0;  // every configuration is (re)loaded, configuration 0 first
   c->v_valid = false;
+  c->built.valid = false;
   return QS_OK;
 }
 
@@ -1801,12 +1880,31 @@ int qs_adjoint(qs_ctx* c, const int* ipos, int nprobe, const float* target, int 
 int qs_atom_gradients(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_params* pp, float* grad_xyz) {
   if (!c || !pp || !grad_xyz || natom < 0 || (natom > 0 && !atoms)) return fail(c, QS_ERR_ARG, "qs_atom_gradients: bad arguments");
   if (c->gV.bytes == 0) return fail(c, QS_ERR_STATE, "qs_atom_gradients: run qs_adjoint with grad_V first");
+  // the kinds and LUTs of the last qs_build_potential call are reused: the same atoms
+  // (count, Z, charge) and potential parameters are required, checked here on the host
+  // (the kernel indexes the kinds by atom and the LUTs by the geometry)
+  if (!c->built.valid)
+    return fail(c, QS_ERR_STATE, "qs_atom_gradients: the potential does not come from qs_build_potential");
+  if (natom != (int)c->built.Z.size())
+    return fail(c, QS_ERR_ARG, "qs_atom_gradients: atom count differs from the last qs_build_potential");
+  for (int i = 0; i < natom; i++)
+    if (atoms[i].Znum != c->built.Z[i] || (atoms[i].Znum > 0 && atoms[i].q != c->built.q[i]))
+      return fail(c, QS_ERR_ARG, "qs_atom_gradients: atom Z or charge differs from the last qs_build_potential");
+  {
+    const qs_pot_params& b = c->built.pp;
+    if (pp->atomRadius != b.atomRadius || pp->potOffsetX != b.potOffsetX || pp->potOffsetY != b.potOffsetY ||
+        pp->czOffset != b.czOffset || pp->centerSlices != b.centerSlices || pp->periodicXY != b.periodicXY ||
+        pp->tds != b.tds || pp->compat_q6 != b.compat_q6 || pp->lut_dr_fix != b.lut_dr_fix ||
+        pp->compat_q5 != b.compat_q5 || pp->lut_norm_fix != b.lut_norm_fix)
+      return fail(c, QS_ERR_ARG, "qs_atom_gradients: potential parameters differ from the last qs_build_potential");
+  }
   if (c->luts.empty() || natom == 0) {
     memset(grad_xyz, 0, sizeof(float) * 3 * natom);
     return QS_OK;
   }
   const qs_params& p = c->p;
-  // the kinds and LUTs of the last qs_build_potential call are reused: same atoms required
+  bool charged = false;
+  for (int i = 0; i < natom; i++) charged = charged || (atoms[i].Znum > 0 && atoms[i].q != 0.f);
   qs::ScatterArgs sa{};
   const float R = pp->atomRadius, dz = p.sliceThickness;
   int nzp = (int)floor(2 * dz / p.resolutionX);
@@ -1824,8 +1922,9 @@ int qs_atom_gradients(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_p
   HIPCHK(c, hipMemcpyAsync(c->atoms.p, atoms, sizeof(qs_atom) * natom, hipMemcpyHostToDevice, c->stream));
   DevBuf g;
   HIPCHK(c, g.ensure(sizeof(float) * 3 * natom));
-  qs::launch_atom_grad(c->atoms.as<qs::QsAtomDev>(), natom, c->lutptrs.as<const float*>(), c->kinds.as<int>(), sa,
-                       c->gV.as<float>(), g.as<float>(), c->stream);
+  qs::launch_atom_grad(c->atoms.as<qs::QsAtomDev>(), natom, c->lutptrs.as<const float*>(),
+                       charged ? c->olutptrs.as<const float*>() : nullptr, c->kinds.as<int>(), sa, c->gV.as<float>(),
+                       g.as<float>(), c->stream);
   HIPCHK(c, hipGetLastError());
   HIPCHK(c, hipMemcpyAsync(grad_xyz, g.p, sizeof(float) * 3 * natom, hipMemcpyDeviceToHost, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));

@@ -81,6 +81,8 @@ struct LutArgs {
   int nx, nz, nzPerSlice;
   double dkx, dkz, smax2;  // smax2 already squared
   double zPos;             // -2 pi dz/nzPerSlice * izOffset
+  double dky;              // q_y step of the q_y sum (dkx for the neutral LUT)
+  double scale;            // factor on the reference's 47.8658 dkx dkz / nz (1 = reference)
 };
 
 struct QsAtomDev {
@@ -112,13 +114,15 @@ struct AtomReach {
   int iAtomZ, iaz0, iaz1;      // slices iAtomZ + [iaz0, iaz1]; iAtomZ = INT_MIN: reaches nothing
   int iOffsZ0;                 // LUT z offset at iaz0
   int kind;                    // LUT index
+  float q;                     // ion charge: q times the kind's offset LUT is added
 };
 void launch_atom_reach(const QsAtomDev* atoms, int natom, const int* kind, const ScatterArgs& s, AtomReach* reach,
                        hipStream_t st);
 void launch_bin_atoms(const AtomReach* reach, int natom, const ScatterArgs& s, int zblocks, int* count_or_cursor,
                       int* list, bool fill, hipStream_t st);
+// oluts[kind]: ion-offset LUT of the kind, or null (no charged atom of that kind)
 void launch_gather(const AtomReach* reach, const int* tstart, const int* list, const float* const* luts,
-                   const ScatterArgs& s, float* V, hipStream_t st);
+                   const float* const* oluts, const ScatterArgs& s, float* V, hipStream_t st);
 
 bool supported_size(int n);
 int col_tile(int nx);
@@ -146,8 +150,8 @@ void launch_target_outside_band(const float* target, int nprobe, int nx, int ny,
                                 double* out, hipStream_t st);
 void launch_probe_grad(const ProbeArgs& a, const float2* G, double cscale, double wavlen_p, double* out2,
                        hipStream_t st);
-void launch_atom_grad(const QsAtomDev* atoms, int natom, const float* const* luts, const int* kind,
-                      const ScatterArgs& s, const float* gV, float* grad, hipStream_t st);
+void launch_atom_grad(const QsAtomDev* atoms, int natom, const float* const* luts, const float* const* oluts,
+                      const int* kind, const ScatterArgs& s, const float* gV, float* grad, hipStream_t st);
 void launch_cplx_fill_zero(float* p, size_t nfloats, hipStream_t st);
 
 }  // namespace qs

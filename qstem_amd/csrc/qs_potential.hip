@@ -38,7 +38,11 @@ __device__ double seval_dev(const double* __restrict__ x, const double* __restri
   return y[i] + (b[i] + (c[i] + d[i] * z) * z) * z;
 }
 
-// f(kx,kz) with the q_y sum (stemlib.cpp:1447-1480); spl = [x, y, b, c, d] each kNsf doubles.
+// f(kx,kz) with the q_y sum (stemlib.cpp:1447-1480, :1652-1680); spl = [x, y, b, c, d,
+// xq, yq] each kNsf doubles: the spline (x, y, b, c, d) evaluates the (kx, kz) term and
+// its coefficients with the nodes (xq, yq) the q_y sum.  (xq, yq) = (x, y) for the
+// neutral LUT; for the ion-offset LUT the reference passes the neutral table's nodes
+// with the offset table's coefficients (quirk Q5, stemlib.cpp:1672).
 __global__ void k_lut_fk(const double* __restrict__ spl, LutArgs g, double B, float2* __restrict__ out) {
   const int n = g.nx * g.nz;
   const double* sx = spl;
@@ -46,6 +50,8 @@ __global__ void k_lut_fk(const double* __restrict__ spl, LutArgs g, double B, fl
   const double* sb = spl + 2 * kNsf;
   const double* sc = spl + 3 * kNsf;
   const double* sd = spl + 4 * kNsf;
+  const double* qx = spl + 5 * kNsf;
+  const double* qy = spl + 6 * kNsf;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
     const int iz = i / g.nx, ix = i % g.nx;
     const double kz = g.dkz * (iz < g.nz / 2 ? iz : iz - g.nz);
@@ -55,9 +61,9 @@ __global__ void k_lut_fk(const double* __restrict__ spl, LutArgs g, double B, fl
     if (s2 < g.smax2) {
       double f = seval_dev(sx, sy, sb, sc, sd, sqrt(s2)) * exp(-s2 * B * 0.25);
       for (int iy = 1; iy < g.nx; iy++) {
-        double s3 = g.dkx * iy;
+        double s3 = g.dky * iy;
         s3 = s3 * s3 + s2;
-        if (s3 < g.smax2) f += 2.0 * seval_dev(sx, sy, sb, sc, sd, sqrt(s3)) * exp(-s3 * B * 0.25);
+        if (s3 < g.smax2) f += 2.0 * seval_dev(qx, qy, sb, sc, sd, sqrt(s3)) * exp(-s3 * B * 0.25);
         else break;
       }
       f *= g.dkx;
@@ -74,7 +80,7 @@ __global__ void k_lut_fk(const double* __restrict__ spl, LutArgs g, double B, fl
 __global__ void k_lut_extract(const float2* __restrict__ tmp, LutArgs g, float* __restrict__ lut) {
   const int Nr = g.nx / 2, Nz = g.nz / 2;
   const int izOff = (g.nzPerSlice - 1) / 2;
-  const double scale = 47.8658 * g.dkx * g.dkz / g.nz;
+  const double scale = 47.8658 * g.dkx * g.dkz / g.nz * g.scale;
   for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < Nr * Nz; i += gridDim.x * blockDim.x) {
     const int iz = i / Nr, ix = i % Nr;
     double zs = 0;
@@ -152,6 +158,7 @@ __global__ __launch_bounds__(256) void k_atom_reach(const QsAtomDev* __restrict_
   AtomReach r;
   if (!atom_setup(atoms[ia], s, r)) r.iAtomZ = INT_MIN;
   r.kind = kind[ia];
+  r.q = atoms[ia].q;
   reach[ia] = r;
 }
 
@@ -204,7 +211,8 @@ template <bool PERIODIC>
 __global__ __launch_bounds__(kPotTile * kPotTile) void k_gather(const AtomReach* __restrict__ reach,
                                                                const int* __restrict__ tstart,
                                                                const int* __restrict__ list,
-                                                               const float* const* __restrict__ luts, ScatterArgs s,
+                                                               const float* const* __restrict__ luts,
+                                                               const float* const* __restrict__ oluts, ScatterArgs s,
                                                                int tiles_y, float* __restrict__ V) {
   const int tile = blockIdx.x;  // tile (tx, ty) = tx * tiles_y + ty
   const int ix = (tile / tiles_y) * kPotTile + threadIdx.x / kPotTile;
@@ -226,6 +234,8 @@ __global__ __launch_bounds__(kPotTile * kPotTile) void k_gather(const AtomReach*
     for (int k = 0; k < n; k++) {
       const AtomReach r = sh[k];  // the same atom in every lane
       const gfloat* lut = (const gfloat*)luts[r.kind];
+      // ion charge: q times the offset LUT's sample at the same (r, z) (stemlib.cpp:966-1006)
+      const gfloat* olut = (r.q != 0.f && oluts) ? (const gfloat*)oluts[r.kind] : nullptr;
       // samples of box pixel (ax, ay) (unwrapped coordinates) into this pixel's sums
       auto add = [&](int ax, int ay) __attribute__((always_inline)) {
         double x2 = (float)(ax * s.dx - r.atomX);
@@ -241,7 +251,10 @@ __global__ __launch_bounds__(kPotTile * kPotTile) void k_gather(const AtomReach*
         for (int zz = 0; zz < kPotZc; zz++) {
           const int iaz = z0 + zz - r.iAtomZ;
           if (iaz < r.iaz0 || iaz > r.iaz1) continue;
-          acc[zz] += (float)lut_sample(lut, s, ir, ddr, r.iOffsZ0 + (iaz - r.iaz0) * step);
+          const int iOffsZ = r.iOffsZ0 + (iaz - r.iaz0) * step;
+          double pv = lut_sample(lut, s, ir, ddr, iOffsZ);
+          if (olut) pv += (double)r.q * lut_sample(olut, s, ir, ddr, iOffsZ);
+          acc[zz] += (float)pv;
         }
       };
       if constexpr (!PERIODIC) {  // the pixel itself, when inside the box
@@ -287,13 +300,15 @@ void launch_bin_atoms(const AtomReach* reach, int natom, const ScatterArgs& s, i
 }
 
 void launch_gather(const AtomReach* reach, const int* tstart, const int* list, const float* const* luts,
-                   const ScatterArgs& s, float* V, hipStream_t st) {
+                   const float* const* oluts, const ScatterArgs& s, float* V, hipStream_t st) {
   const int tiles_x = (s.potNx + kPotTile - 1) / kPotTile, tiles_y = (s.potNy + kPotTile - 1) / kPotTile;
   const dim3 grid((unsigned)(tiles_x * tiles_y), (unsigned)((s.slices + kPotZc - 1) / kPotZc));
   if (s.periodic)
-    hipLaunchKernelGGL(k_gather<true>, grid, dim3(kPotTile * kPotTile), 0, st, reach, tstart, list, luts, s, tiles_y, V);
+    hipLaunchKernelGGL(k_gather<true>, grid, dim3(kPotTile * kPotTile), 0, st, reach, tstart, list, luts, oluts, s,
+                       tiles_y, V);
   else
-    hipLaunchKernelGGL(k_gather<false>, grid, dim3(kPotTile * kPotTile), 0, st, reach, tstart, list, luts, s, tiles_y, V);
+    hipLaunchKernelGGL(k_gather<false>, grid, dim3(kPotTile * kPotTile), 0, st, reach, tstart, list, luts, oluts, s,
+                       tiles_y, V);
 }
 
 }  // namespace qs

@@ -98,6 +98,12 @@ def rez_table() -> np.ndarray:
     return np.fromfile(path, dtype="<f8").reshape(93, 123)
 
 
+def rez_ion_table() -> np.ndarray:
+    """Ion-offset scattering factors df_e/dq shipped as data (row 0 = s grid, row Z)."""
+    path = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data", "rez_scatparoffs_f64.bin")
+    return np.fromfile(path, dtype="<f8").reshape(93, 123)
+
+
 def _fptr(a: Optional[np.ndarray]):
     if a is None:
         return None
@@ -212,19 +218,40 @@ class Multislice:
                     "qs_set_scattering_table")
         self._have_table = True
 
-    def build_potential(self, atoms, atomRadius=5.0, potOffsetX=0.0, potOffsetY=0.0, czOffset=0.0,
-                        centerSlices=0, periodicXY=0, tds=0, compat_q6=0, lut_dr_fix=0, return_V=False):
+    def set_ion_scattering_table(self, table: Optional[np.ndarray] = None):
+        """Ion-offset table (93 x 123 doubles); defaults to qstem_amd/data/rez_scatparoffs_f64.bin."""
+        if table is None:
+            table = rez_ion_table()
+        table = np.ascontiguousarray(table, dtype=np.float64)
+        assert table.shape == (93, 123), table.shape
+        self._check(self._lib.qs_set_ion_scattering_table(self._h,
+                                                          table.ctypes.data_as(ctypes.POINTER(ctypes.c_double))),
+                    "qs_set_ion_scattering_table")
+        self._have_ion_table = True
+
+    @staticmethod
+    def _pot_params(atomRadius=5.0, potOffsetX=0.0, potOffsetY=0.0, czOffset=0.0, centerSlices=0, periodicXY=0,
+                    tds=0, compat_q6=0, lut_dr_fix=0, compat_q5=1, lut_norm_fix=0):
+        return _lib.QsPotParams(atomRadius, potOffsetX, potOffsetY, czOffset, int(centerSlices), int(periodicXY),
+                                int(tds), int(compat_q6), int(lut_dr_fix), int(compat_q5), int(lut_norm_fix))
+
+    def build_potential(self, atoms, return_V=False, **pot):
         """make3DSlices + initSTEMSlices on the GPU.  atoms: (n, 7) array of
-        x, y, z, dw, occ, q, Z (Cartesian Angstrom)."""
+        x, y, z, dw, occ, q, Z (Cartesian Angstrom); charged atoms add q times their
+        element's ion-offset LUT.  pot: the qs_pot_params fields (atomRadius=5.0,
+        potOffsetX, potOffsetY, czOffset, centerSlices, periodicXY, tds, compat_q6,
+        lut_dr_fix, compat_q5=1, lut_norm_fix)."""
         if not getattr(self, "_have_table", False):
             self.set_scattering_table()
+        if not getattr(self, "_have_ion_table", False) and np.any(np.asarray(atoms, dtype=np.float64).reshape(-1, 7)[:, 5]):
+            self.set_ion_scattering_table()
         arr = _atom_array(atoms)
-        pp = _lib.QsPotParams(atomRadius, potOffsetX, potOffsetY, czOffset, int(centerSlices), int(periodicXY),
-                              int(tds), int(compat_q6), int(lut_dr_fix))
+        pp = self._pot_params(**pot)
         V = np.empty((self.cfg.slices, self.cfg.potNx, self.cfg.potNy), np.float32) if return_V else None
         self._check(self._lib.qs_build_potential(self._h, arr, len(atoms), ctypes.byref(pp), _fptr(V)),
                     "qs_build_potential")
         self._nslice = self.cfg.slices
+        self._build_pot = dict(pot)
         return V
 
     # -- probe ---------------------------------------------------------------------
@@ -282,13 +309,14 @@ class Multislice:
             out["grad_df0"], out["grad_Cs"] = float(gp[0]), float(gp[1])
         return out
 
-    def atom_gradients(self, atoms, atomRadius=5.0, potOffsetX=0.0, potOffsetY=0.0, czOffset=0.0, centerSlices=0,
-                       periodicXY=0, tds=0, compat_q6=0, lut_dr_fix=0) -> np.ndarray:
-        """dL/d(x, y, z) per atom from the last adjoint (same atoms as build_potential)."""
+    def atom_gradients(self, atoms, **pot) -> np.ndarray:
+        """dL/d(x, y, z) per atom from the last adjoint (same atoms as build_potential;
+        the potential parameters default to that call's)."""
         atoms = np.asarray(atoms, dtype=np.float64).reshape(-1, 7)
         arr = _atom_array(atoms)
-        pp = _lib.QsPotParams(atomRadius, potOffsetX, potOffsetY, czOffset, int(centerSlices), int(periodicXY),
-                              int(tds), int(compat_q6), int(lut_dr_fix))
+        kw = dict(getattr(self, "_build_pot", {}))
+        kw.update(pot)
+        pp = self._pot_params(**kw)
         g = np.zeros((len(atoms), 3), np.float32)
         self._check(self._lib.qs_atom_gradients(self._h, arr, len(atoms), ctypes.byref(pp), _fptr(g)),
                     "qs_atom_gradients")

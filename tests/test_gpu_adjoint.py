@@ -88,6 +88,32 @@ def test_checkpoint_interval_does_not_change_gradients():
         assert abs(r["grad_df0"] - res[0]["grad_df0"]) <= 1e-6 * abs(res[0]["grad_df0"])
 
 
+def test_atom_gradients_check_the_build():
+    """qs_atom_gradients reuses the LUTs and kinds of the last qs_build_potential: a
+    different radius, atom count or charge, or a potential set otherwise since, is
+    rejected on the host before any launch."""
+    from qstem_amd import Multislice, QstemError
+    cfg = _cfg(1, nx=64, pad=16, slices=3)
+    cfg.sliceThickness = 2.0
+    atoms = np.array([[2.0, 2.0, 1.0, 0.5, 1.0, 0.0, 22], [2.8, 1.7, 3.1, 0.5, 1.0, 0.0, 8]])
+    with Multislice(cfg) as ms:
+        V = ms.build_potential(atoms, atomRadius=2.5, return_V=True)
+        ms.make_probe()
+        ms.adjoint(np.array([[0, 0]], np.int32), np.zeros((1, 64, 64), np.float32))
+        assert ms.atom_gradients(atoms).shape == (2, 3)
+        with pytest.raises(QstemError, match="parameters"):
+            ms.atom_gradients(atoms, atomRadius=5.0)
+        with pytest.raises(QstemError, match="count"):
+            ms.atom_gradients(atoms[:1])
+        q = atoms.copy()
+        q[0, 5] = 1.0
+        with pytest.raises(QstemError, match="charge"):
+            ms.atom_gradients(q)
+        ms.set_potential(V)
+        with pytest.raises(QstemError, match="qs_build_potential"):
+            ms.atom_gradients(atoms)
+
+
 def test_atom_position_gradients_fd(oracle):
     from qstem_amd import Multislice
     cfg = _cfg(1, nx=128, pad=24, slices=4)
