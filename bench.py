@@ -6,14 +6,22 @@ a = 3.905 A, 12x12 cells lateral, 40 cells thick -> 80 slices of 1.9525 A;
 defocus = -5 nm; 64x64 scan positions over one unit cell; 256 probes per
 launch; detectors BF 0-10, ABF 11-22, ADF 70-200 mrad.
 
+The projected potential is built on the GPU from the atoms of the 12x12x40
+supercell (neutral Sr, Ti, O with the Debye-Waller factors of the reference's
+SrTiO3.cfg; qs_build_potential), and its build time is reported in setup_s.
+
 A "step" is one launch group: 256 probes advanced through all 80 slices
 (20480 probe-slices).  --steps 16 covers the whole 64x64 scan.
 
 value = probe-slices/s over the timed steps (inputs resident in HBM).
 Also reported: the per-probe GPU loop (same kernels, one probe per launch
-sequence), the batched/per-probe ratio, the adjoint/forward time ratio, the
-detector error against the FP64 oracle on a sample of probes, the roofline
-of the slice-step kernel pair, and the oracle timed on the host cores.
+sequence), the batched/per-probe ratio, the adjoint/forward time ratio of the
+4D-STEM adjoint workload, the detector error against the FP64 oracle on a
+sample of probes, the roofline of the slice-step kernel pair, the plain CPU
+FP32 multislice (oracle/qs_cpu32.c, OpenMP over probes) timed on the host cores
+on a sample of this workload and on the CPU configuration (128^2, 20 slices,
+16x16 scan), and, as extra keys, the uneven frozen-phonon batch and the 4096^2
+TEM workload with its checkpointed adjoint.
 """
 import argparse
 import json
@@ -52,24 +60,6 @@ def perovskite_config():
     ipos = scan_offsets(x0, stopx, 64, x0, stopx, 64, dx, dx, pot, pot, nx, nx)
     potoff = float(x0 - 0.5 * nx * dx)
     return cfg, ipos, potoff
-
-
-def synthetic_potential(cfg, potoff):
-    """Projected potential of the perovskite window: Gaussian atom columns with
-    Z-scaled weights on the two distinct slice types, repeated through the
-    thickness (a stand-in with the right shape and scale; the multislice cost
-    does not depend on the values)."""
-    a = 3.905
-    n = cfg.potNx
-    x = potoff + np.arange(n) * cfg.resolutionX
-    planes = {0: [(0.0, 0.0, 38), (0.5, 0.5, 8)], 1: [(0.5, 0.5, 22), (0.5, 0.0, 8), (0.0, 0.5, 8)]}
-    V = np.zeros((2, n, n), np.float32)
-    for s, atoms in planes.items():
-        for fx, fy, Z in atoms:
-            ddx = ((x[:, None] - fx * a + a / 2) % a) - a / 2
-            ddy = ((x[None, :] - fy * a + a / 2) % a) - a / 2
-            V[s] += (3.0 * Z ** 0.7) * np.exp(-(ddx ** 2 + ddy ** 2) / (2 * 0.25 ** 2))
-    return np.concatenate([V] * (cfg.slices // 2), axis=0)
 
 
 def uneven_workload(nconfig=16, steps=3, streams=0):
@@ -285,6 +275,48 @@ def adjoint_workload(steps=3):
                     "segment recompute, reverse sweeps"}
 
 
+def cpu_config():
+    """The CPU configuration: perovskite 4x4 cells lateral, 10 cells thick, 2 slices per
+    cell (20 slices), 128^2 wave at dx = 0.040 A (the ADF ring inside the 2/3 band
+    limit), 200 kV, 20 mrad, Cs = 0, df = 0, 16x16 scan over one unit cell, ADF
+    70-200 mrad.  Returns (cfg, ipos, potential offset)."""
+    from qstem_amd import Detector, StemConfig, scan_offsets
+    a, dx, nx = 3.905, 0.040, 128
+    f32 = np.float32
+    x0, x1 = f32(2 * a), f32(3 * a)
+    rng_px = int(f32(x1 - x0) / f32(dx))
+    pot = nx + rng_px
+    cfg = StemConfig(nx=nx, ny=nx, resolutionX=dx, resolutionY=dx, v0=200.0, slices=20, sliceThickness=a / 2,
+                     potNx=pot, potNy=pot, by=4 * a, bandlimittrans=1, Cs=0.0, df0=0.0, alpha=20.0,
+                     detectors=[Detector(70.0, 200.0, "ADF")])
+    stopx = f32(x0 + f32(dx) * f32(rng_px))
+    ipos = scan_offsets(x0, stopx, 16, x0, stopx, 16, dx, dx, pot, pot, nx, nx)
+    return cfg, ipos, float(x0 - 0.5 * nx * dx)
+
+
+def cpu_info():
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return model, len(os.sched_getaffinity(0))
+
+
+def cpu32_rate(cfg, trans, psi0, ipos, bins, nbins, threads):
+    """Plain CPU FP32 multislice (one probe per OpenMP thread) over ipos: probe-slices/s."""
+    from oracle import oracle
+    oracle.cpu32_multislice(cfg, trans, psi0, ipos[:threads], bins=bins, nbins=nbins, nthreads=threads)  # warm-up
+    t0 = time.perf_counter()
+    _, det = oracle.cpu32_multislice(cfg, trans, psi0, ipos, bins=bins, nbins=nbins, nthreads=threads)
+    wall = time.perf_counter() - t0
+    return len(ipos) * trans.shape[0] / wall, wall, det
+
+
 def _oracle_worker(args):
     cfg_d, trans_path, probe_path, ipos, bins, nbins = args
     sys.path.insert(0, ROOT)
@@ -299,8 +331,9 @@ def _oracle_worker(args):
     return det, time.perf_counter() - t0
 
 
-def cpu_baseline(cfg, trans, psi0, ipos, bins, nbins, nsample, workdir):
-    """FP64 oracle on a bounded sample: one probe through all slices per worker process."""
+def oracle_sample(cfg, trans, psi0, ipos, bins, nbins, nsample, workdir):
+    """FP64 oracle on a bounded sample (the parity check of the timed path): one probe
+    through all slices per worker process."""
     import dataclasses
     from multiprocessing import get_context
     os.makedirs(workdir, exist_ok=True)
@@ -326,8 +359,10 @@ def main():
     ap.add_argument("--batch", type=int, default=256, help="probes per qs_run_stem call (one step)")
     ap.add_argument("--group", type=int, default=0, help="probes advanced together inside the library (0 = auto)")
     ap.add_argument("--loop-probes", type=int, default=8, help="probes for the per-probe loop baseline")
-    ap.add_argument("--cpu-sample", type=int, default=8, help="probes for the oracle CPU baseline (0 = skip)")
+    ap.add_argument("--cpu-sample", type=int, default=32, help="probes for the FP32 CPU baseline (0 = skip)")
+    ap.add_argument("--oracle-sample", type=int, default=4, help="probes checked against the FP64 oracle (0 = skip)")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-extra", action="store_true", help="skip the uneven-batch and TEM records")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches only")
     ap.add_argument("--tem-slices", type=int, default=200, help="slices of the tem workload")
     ap.add_argument("--workload", default="batched", choices=["batched", "uneven", "tem", "copy", "copyrate"],
@@ -386,15 +421,20 @@ def main():
         return
 
     from qstem_amd import Multislice, record_bins
+    from qstem_amd.specimen import build_supercell, perovskite_cell
+    import torch
 
     cfg, ipos_all, potoff = perovskite_config()
     P = a.batch
     nslice = cfg.slices
     bins, nbins = record_bins(nslice)
-    V = synthetic_potential(cfg, potoff)
+    atoms, _ = build_supercell(perovskite_cell(), 12, 12, 40)
     ms = Multislice(cfg)
+    ms.build_potential(atoms[:64], potOffsetX=potoff, potOffsetY=potoff)  # warm-up (LUT kernels, allocations)
+    torch.cuda.synchronize()
     t0 = time.perf_counter()
-    ms.set_potential(V)
+    ms.build_potential(atoms, potOffsetX=potoff, potOffsetY=potoff)
+    torch.cuda.synchronize()
     t_pot = time.perf_counter() - t0
     t0 = time.perf_counter()
     psi0 = ms.make_probe()
@@ -415,8 +455,6 @@ def main():
     def step(i):
         sl = ipos_all[(i % nbatches) * P:((i % nbatches) + 1) * P]
         return ms.run_stem(sl, bins=bins, nbins=nbins)
-
-    import torch
 
     def timed(graph, profile):
         """K steps between two device synchronisations; eager launches with per-launch
@@ -477,14 +515,9 @@ def main():
     # algorithmic bytes: wave in + out, the union transmission window read once per launch, detector sums
     bytes_per_probe_slice = 16.0 * N2 + 8.0 * W / G + 4.0 * ndet
     achieved = bytes_per_probe_slice * G / (t_step_ms * 1e-3) / 1e9
-    # measured HBM-side traffic of the launch pair (rocprofv3 FETCH_SIZE / WRITE_SIZE, calibrated
-    # against a plain copy), per probe-slice at 128 probes per launch, scaled to this launch size
+    # HBM traffic needs the profiler's counters, which this process cannot read about
+    # itself: null here, measured with rocprofv3 --pmc in a separate run (profiles/)
     traffic = None
-    tp = os.path.join(ROOT, "profiles", "traffic_sweeps_512.json")
-    if os.path.exists(tp):
-        with open(tp) as f:
-            per = json.load(f).get("bytes_per_probe_slice")
-        traffic = round(per * G) if per and G else None
 
     result = {
         "metric": METRIC,
@@ -498,7 +531,8 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "c64 (fp32 complex; fp64 chi)",
-        "data": "synthetic: perovskite-window projected potential (Gaussian atom columns), seeded positions",
+        "data": "synthetic: perovskite supercell atoms (neutral Sr/Ti/O, DW of SrTiO3.cfg), potential built on "
+                "the GPU",
         "config": {"workload": "batched plain STEM, perovskite 12x12x40 cells, 80 slices, 512x512 wave, "
                                "dx=0.030 A, 300 kV, 25 mrad, Cs=1 um, df=-5 nm, 64x64 scan, BF/ABF/ADF",
                    "probes_per_call": P, "probes_per_kernel_launch": tuned[2] or G, "streams": streams,
@@ -509,7 +543,8 @@ def main():
         "per_probe_loop": {"value": round(loop_rate, 1), "unit": "probe-slices/s", "probes": a.loop_probes},
         "batched_over_per_probe": round(value / loop_rate, 2),
         "kernel_ms": {"rows_sweep": round(ms_rows / max(n_rows, 1), 4), "cols_sweep": round(ms_cols / max(n_cols, 1), 4)},
-        "setup_s": {"transmission": round(t_pot, 3), "probe": round(t_probe, 4)},
+        "setup_s": {"potential_and_transmission": round(t_pot, 3), "atoms": int(len(atoms)),
+                    "probe": round(t_probe, 4)},
         "tuning": tuning,
         "roofline": {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
@@ -521,14 +556,14 @@ def main():
     if not a.no_adjoint:
         result["adjoint"] = adjoint_workload()
 
-    if a.cpu_sample > 0:
-        ns = min(a.cpu_sample, len(os.sched_getaffinity(0)), 16)
-        trans = ms.get_trans()
+    trans = ms.get_trans()
+    if a.oracle_sample > 0:
+        ns = min(a.oracle_sample, len(os.sched_getaffinity(0)), 16)
         import shutil
         import tempfile
         tmpdir = tempfile.mkdtemp(prefix="qs_bench_")
         try:
-            det_o, wall, per = cpu_baseline(cfg, trans, psi0, ipos_all[:ns], bins, nbins, ns, tmpdir)
+            det_o, _, _ = oracle_sample(cfg, trans, psi0, ipos_all[:ns], bins, nbins, ns, tmpdir)
         finally:
             shutil.rmtree(tmpdir, ignore_errors=True)
         det_g = ms.run_stem(ipos_all[:ns], bins=bins, nbins=nbins)["det"]
@@ -538,11 +573,45 @@ def main():
                 ref = det_o[b, d]
                 if np.any(ref != 0):
                     errs.append(float(np.linalg.norm(det_g[b, d] - ref) / np.linalg.norm(ref)))
-        result["cpu_baseline"] = {"value": round(ns * nslice / wall, 2), "unit": "probe-slices/s", "cores": ns,
-                                  "kind": "port",
-                                  "sample": "%d probes x %d slices of this workload, FP64 oracle (own radix-2 FFT), "
-                                            "one probe per process" % (ns, nslice)}
         result["max_rel_err_vs_oracle"] = max(errs) if errs else None
+        result["oracle_sample_probes"] = ns
+
+    if a.cpu_sample > 0:
+        model, ncpu = cpu_info()
+        threads = min(ncpu, int(os.environ.get("OMP_NUM_THREADS", ncpu)))
+        nsamp = max(a.cpu_sample, threads)
+        rate, wall, det_c = cpu32_rate(cfg, trans, psi0, ipos_all[:nsamp], bins, nbins, threads)
+        det_g = ms.run_stem(ipos_all[:nsamp], bins=bins, nbins=nbins)["det"]
+        cerr = max(float(np.linalg.norm(det_g[b, d] - det_c[b, d]) / np.linalg.norm(det_c[b, d]))
+                   for b in range(nbins) for d in range(ndet) if np.any(det_c[b, d]))
+        # the CPU configuration (128^2, 20 slices, 16x16 scan), whole scan
+        c1, ip1, off1 = cpu_config()
+        m1 = Multislice(c1)
+        m1.build_potential(build_supercell(perovskite_cell(), 4, 4, 10)[0], potOffsetX=off1, potOffsetY=off1)
+        psi1 = m1.make_probe()
+        b1, nb1 = record_bins(c1.slices)
+        rate1, wall1, _ = cpu32_rate(c1, m1.get_trans(), psi1, ip1, b1, nb1, threads)
+        m1.run_stem(ip1, bins=b1, nbins=nb1)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        m1.run_stem(ip1, bins=b1, nbins=nb1)
+        torch.cuda.synchronize()
+        gpu1 = len(ip1) * c1.slices / (time.perf_counter() - t0)
+        m1.close()
+        result["cpu_baseline"] = {
+            "value": round(rate, 1), "unit": "probe-slices/s", "cores": threads, "kind": "port",
+            "sample": "%d probes x %d slices of this workload (%.1f s wall), plain FP32 multislice "
+                      "(oracle/qs_cpu32.c: own Stockham FFT, one probe per OpenMP thread as doSTEM)" % (nsamp, nslice, wall),
+            "cpu_model": model, "host_cpus": ncpu, "max_rel_err_gpu_vs_cpu": cerr,
+            "cpu_config": {"workload": "128^2, 20 slices, 16x16 scan, perovskite 4x4x10 cells, 200 kV, ADF",
+                           "value": round(rate1, 1), "wall_s": round(wall1, 3),
+                           "gpu_same_config": round(gpu1, 1), "unit": "probe-slices/s"}}
+
+    if not a.no_extra:
+        ms.close()
+        del trans
+        result["uneven"] = uneven_workload(steps=2)
+        result["tem"] = tem_workload(nslice=a.tem_slices)
     print(json.dumps(result), flush=True)
 
 

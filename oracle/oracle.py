@@ -49,6 +49,41 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+CPU32_PATH = os.path.join(HERE, "_build", "libqs_cpu32.so")
+_cpu32 = None
+
+
+def load_cpu32():
+    """The FP32 CPU baseline (qs_cpu32.c): benchmark and test infrastructure."""
+    global _cpu32
+    if _cpu32 is None:
+        if not os.path.exists(CPU32_PATH):
+            build()
+        _cpu32 = ctypes.CDLL(CPU32_PATH)
+        _cpu32.qc_max_threads.restype = _I
+    return _cpu32
+
+
+def cpu32_multislice(cfg, trans, psi0, ipos, bins=None, nbins=0, want_exit=False, nthreads=0):
+    """FP32 CPU multislice of the probes ipos through trans (complex [nslice][potNx][potNy]):
+    returns (exit waves [n][nx][ny] complex64 or None, det [nbins][ndet][n] float32 or None)."""
+    trans = np.ascontiguousarray(trans, dtype=np.complex64)
+    psi0 = np.ascontiguousarray(psi0, dtype=np.complex64)
+    ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
+    n = ipos.shape[0]
+    ndet = len(cfg.detectors)
+    det = np.zeros((nbins, ndet, n), np.float32) if (bins is not None and ndet) else None
+    b = np.ascontiguousarray(bins, dtype=np.int32) if bins is not None else None
+    ex = np.empty((n, cfg.nx, cfg.ny), np.complex64) if want_exit else None
+    p = params(cfg)
+    rc = load_cpu32().qc_multislice(ctypes.byref(p), _p(trans.view(np.float32), _F), trans.shape[0],
+                                    _p(psi0.view(np.float32), _F), _p(ipos, _I), n, _p(b, _I), nbins, _p(det, _F),
+                                    _p(ex.view(np.float32) if ex is not None else None, _F), int(nthreads))
+    if rc != 0:
+        raise ValueError("qc_multislice: unsupported size")
+    return ex, det
+
+
 def load():
     global _lib
     if _lib is None:
