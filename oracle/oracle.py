@@ -305,6 +305,38 @@ def make_slices_d(cfg, g, luts, kinds, xyz, Z, potOffsetX, potOffsetY, czOffset=
     return V
 
 
+def adjoint(cfg, V, ipos, psi0, target):
+    """FP64 adjoint (qo_adjoint): (loss, dL/dV [nslice][potNx][potNy], g_psi0 [nx][ny]) for
+    L = sum (|DFT(psi_exit)/N^2|^2 - target)^2 over the probes ipos."""
+    V = np.ascontiguousarray(V, dtype=np.float64)
+    ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
+    psi0 = np.ascontiguousarray(psi0, dtype=np.complex128)
+    target = np.ascontiguousarray(target, dtype=np.float64)
+    loss = np.zeros(1)
+    gV = np.zeros(V.shape)
+    gpsi = np.zeros((cfg.nx, cfg.ny), np.complex128)
+    p = params(cfg)
+    load().qo_adjoint(ctypes.byref(p), _p(V), V.shape[0], _p(ipos, _I), ipos.shape[0], _p(psi0), _p(target),
+                      _p(loss), _p(gV), _p(gpsi))
+    return float(loss[0]), gV, gpsi
+
+
+def probe_param_gradients(cfg, gpsi0, h=1e-5):
+    """dL/d(df0), dL/d(Cs) from the adjoint's g_psi0: 2 Re sum conj(g) dpsi0/dtheta, with
+    dpsi0/dtheta by FP64 central differences of the probe (a smooth function of both)."""
+    out = []
+    for name in ("df0", "Cs"):
+        base = float(getattr(cfg, name))
+        step = h * max(1.0, abs(base))
+        args = {"df0": float(cfg.df0), "Cs": float(cfg.Cs)}
+        args[name] = base + step
+        pp = probe_d(cfg, **args)
+        args[name] = base - step
+        pm = probe_d(cfg, **args)
+        out.append(float(2 * np.real(np.vdot(gpsi0, (pp - pm) / (2 * step)))))
+    return out
+
+
 def diffraction_loss(cfg, trans, ipos, psi0, target):
     """L = sum_p sum_k (|FFT(psi_exit)/(nx ny)|^2 - target)^2 with the oracle's FP64 multislice."""
     ex, _, _ = multislice(cfg, trans, ipos, psi0)

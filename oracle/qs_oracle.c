@@ -291,16 +291,31 @@ void qo_make_trans(const qo_params *p, const float *V, int nslice, double comple
 }
 
 /* Same with a double-precision potential (finite-difference checks). */
+/* Interaction constant of initSTEMSlices: gamma*lambda with gamma = 1 + v0/511 in float
+ * (stemlib.cpp:2271-2280). */
+static double trans_sigma(const qo_params *p) {
+  float mm0 = 1.0F + p->v0 / 511.0F;
+  return mm0 * qo_wavelength(p->v0);
+}
+
+static void bandlimit_d(const qo_params *p, double complex *trans, int nslice);
+
 void qo_make_trans_d(const qo_params *p, const double *V, int nslice, double complex *trans) {
-  const int nx = p->potNx, ny = p->potNy;
-  const size_t n2 = (size_t)nx * ny;
-  float mm0 = 1.0F + p->v0 / 511.0F; /* :2271, float arithmetic */
-  double scale = mm0 * qo_wavelength(p->v0);
+  const size_t n2 = (size_t)p->potNx * p->potNy;
+  const double scale = trans_sigma(p);
   for (size_t i = 0; i < (size_t)nslice * n2; i++) {
     double vz = V[i] * scale;
     trans[i] = cos(vz) + I * sin(vz);
   }
-  if (!p->bandlimittrans) return;
+  if (p->bandlimittrans) bandlimit_d(p, trans, nslice);
+}
+
+/* The band limit of initSTEMSlices (stemlib.cpp:2290-2336): FFT, 2/3 disc with
+ * 1/(potNx potNy), IFFT, in place on [nslice][potNx][potNy]; a real diagonal mask
+ * between F and its inverse, so the operator is self-adjoint. */
+static void bandlimit_d(const qo_params *p, double complex *trans, int nslice) {
+  const int nx = p->potNx, ny = p->potNy;
+  const size_t n2 = (size_t)nx * ny;
   /* k-tables (:2221-2238) in float; Q1 uses muls->by for negative ky */
   float *kx2 = (float *)malloc(sizeof(float) * nx), *ky2 = (float *)malloc(sizeof(float) * ny);
   for (int i = 0; i < nx; i++) {
@@ -455,6 +470,102 @@ void qo_multislice(const qo_params *p, const double complex *trans, int nslice,
   }
   free(w); free(px); free(py); free(kx2); free(ky2);
   (void)nbins;
+}
+
+/* FP64 adjoint of the multislice for the loss of the library's qs_adjoint,
+ *   L = sum_p sum_k (I_p(k) - T_p(k))^2,  I_p = |Phi_p|^2,  Phi_p = FFT(psi_exit,p) / N^2,
+ * the diffraction intensity after the last slice (N^2 = nx*ny), with every slice of V
+ * and the incident waves psi0 [nx][ny] shared by the nprobe window offsets ipos.  Not a
+ * reference routine (QSTEM has no adjoint): reverse mode of qo_multislice and
+ * qo_make_trans_d.  Conventions: for complex z, g_z = dL/d(conj z), so that
+ * dL = 2 Re sum conj(g_z) dz; through y = F x (unnormalised DFT) g_x = F^H g_y = IDFT(g_y);
+ * through y = x c, g_x = conj(c) g_y and g_c += conj(x) g_y; t = BL(u) with the
+ * self-adjoint band limit, u = exp(i sigma V): dL/dV = 2 sigma Im(g_u conj(u)).
+ * Outputs: loss; gV [nslice][potNx][potNy] (nullable); gpsi0 [nx][ny] = g of the incident
+ * wave summed over probes (nullable).  Waves of all slices are kept per probe (FP64). */
+void qo_adjoint(const qo_params *p, const double *V, int nslice, const int *ipos, int nprobe,
+                const double complex *psi0, const double *target, double *loss, double *gV,
+                double complex *gpsi0) {
+  const int nx = p->nx, ny = p->ny, pnx = p->potNx, pny = p->potNy;
+  const size_t n2 = (size_t)nx * ny, slice = (size_t)pnx * pny;
+  double complex *t = (double complex *)malloc(sizeof(double complex) * slice * nslice);
+  qo_make_trans_d(p, V, nslice, t);
+  double complex *gt = (double complex *)calloc(slice * nslice, sizeof(double complex));
+  float *kx2 = (float *)malloc(sizeof(float) * nx), *ky2 = (float *)malloc(sizeof(float) * ny);
+  float k2max;
+  qo_k_tables(p, kx2, ky2, &k2max);
+  const double wl = qo_wavelength(p->v0), dz = p->sliceThickness;
+  const double ax = (double)p->resolutionX * nx, by = (double)p->resolutionY * ny;
+  double complex *P = (double complex *)malloc(sizeof(double complex) * n2); /* propagator x mask */
+  for (int ix = 0; ix < nx; ix++) {
+    double kx = ((ix > nx / 2) ? (double)(ix - nx) : (double)ix) / ax;
+    for (int iy = 0; iy < ny; iy++) {
+      double ky = ((iy > ny / 2) ? (double)(iy - ny) : (double)iy) / by;
+      double tx = M_PI * dz * wl * kx * kx, ty = M_PI * dz * wl * ky * ky;
+      int in = kx2[ix] < k2max && (kx2[ix] + ky2[iy]) < k2max;
+      P[(size_t)ix * ny + iy] = in ? (cos(tx) - I * sin(tx)) * (cos(ty) - I * sin(ty)) : 0;
+    }
+  }
+  double complex *psi = (double complex *)malloc(sizeof(double complex) * n2 * (nslice + 1));
+  double complex *g = (double complex *)malloc(sizeof(double complex) * n2);
+  if (gpsi0) memset(gpsi0, 0, sizeof(double complex) * n2);
+  const double N2 = (double)n2;
+  double L = 0;
+  for (int ip = 0; ip < nprobe; ip++) {
+    const int px = ipos[2 * ip], py = ipos[2 * ip + 1];
+    memcpy(psi, psi0, sizeof(double complex) * n2);
+    /* forward: psi[s+1] = IDFT(P DFT(psi[s] t_s)) / N^2 */
+    for (int s = 0; s < nslice; s++) {
+      const double complex *ts = t + (size_t)s * slice;
+      double complex *a = psi + (size_t)(s + 1) * n2;
+      for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++)
+          a[(size_t)ix * ny + iy] = psi[(size_t)s * n2 + (size_t)ix * ny + iy] * ts[(size_t)(ix + px) * pny + iy + py];
+      qo_fft2d(a, nx, ny, -1);
+      for (size_t k = 0; k < n2; k++) a[k] *= P[k];
+      qo_fft2d(a, nx, ny, +1);
+      for (size_t k = 0; k < n2; k++) a[k] /= N2;
+    }
+    /* loss and its seed: Phi = DFT(psi_S)/N^2, g_Phi = 2 (I - T) Phi, g_psiS = IDFT(g_Phi)/N^2 */
+    memcpy(g, psi + (size_t)nslice * n2, sizeof(double complex) * n2);
+    qo_fft2d(g, nx, ny, -1);
+    const double *T = target + (size_t)ip * n2;
+    for (size_t k = 0; k < n2; k++) {
+      double complex phi = g[k] / N2;
+      double I_ = creal(phi) * creal(phi) + cimag(phi) * cimag(phi);
+      L += (I_ - T[k]) * (I_ - T[k]);
+      g[k] = 2.0 * (I_ - T[k]) * phi / N2;
+    }
+    qo_fft2d(g, nx, ny, +1);
+    /* reverse: through psi[s+1] = IDFT(P DFT(a_s)) / N^2 and a_s = psi[s] t_s */
+    for (int s = nslice - 1; s >= 0; s--) {
+      qo_fft2d(g, nx, ny, -1); /* (IDFT)^H = DFT */
+      for (size_t k = 0; k < n2; k++) g[k] = conj(P[k]) * g[k] / N2;
+      qo_fft2d(g, nx, ny, +1); /* DFT^H = IDFT: g is now g_a */
+      const double complex *ts = t + (size_t)s * slice;
+      double complex *gts = gt + (size_t)s * slice;
+      const double complex *ps = psi + (size_t)s * n2;
+      for (int ix = 0; ix < nx; ix++)
+        for (int iy = 0; iy < ny; iy++) {
+          const size_t k = (size_t)ix * ny + iy, w = (size_t)(ix + px) * pny + iy + py;
+          gts[w] += conj(ps[k]) * g[k];
+          g[k] = conj(ts[w]) * g[k];
+        }
+    }
+    if (gpsi0)
+      for (size_t k = 0; k < n2; k++) gpsi0[k] += g[k];
+  }
+  *loss = L;
+  if (gV) {
+    /* g_u = BL(g_t) (self-adjoint), dL/dV = 2 sigma Im(g_u conj(u)), u = exp(i sigma V) */
+    if (p->bandlimittrans) bandlimit_d(p, gt, nslice);
+    const double sg = trans_sigma(p);
+    for (size_t i = 0; i < slice * nslice; i++) {
+      const double complex u = cos(V[i] * sg) + I * sin(V[i] * sg);
+      gV[i] = 2.0 * sg * cimag(gt[i] * conj(u));
+    }
+  }
+  free(t); free(gt); free(kx2); free(ky2); free(P); free(psi); free(g);
 }
 
 /* ------------------------------------------------------------------------ */

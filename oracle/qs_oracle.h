@@ -86,6 +86,13 @@ void qo_multislice(const qo_params *p, const double complex *trans, int nslice,
                    const int *ipos, int nprobe, double complex *psi_io,
                    const int *bin_of_slice, int nbins, double *det, double *diff);
 
+/* FP64 adjoint of qo_multislice for L = sum (|DFT(psi_exit)/N^2|^2 - target)^2 (no reference
+ * counterpart): loss, dL/dV [nslice][potNx][potNy], and g_psi0 = dL/d(conj psi0) summed over
+ * probes [nx][ny] (nullable outputs). */
+void qo_adjoint(const qo_params *p, const double *V, int nslice, const int *ipos, int nprobe,
+                const double complex *psi0, const double *target, double *loss, double *gV,
+                double complex *gpsi0);
+
 /* ---- potential (stemlib.cpp:383-1541) ---- */
 typedef struct qo_lut_geom { int nx, nz, nzPerSlice, Nr, Nz_lut; double dkx, dkz, smax2; } qo_lut_geom;
 void qo_lut_geometry(float atomRadius, float resX, float sliceThickness, qo_lut_geom *g);

@@ -1706,7 +1706,7 @@ static int adj_forward(qs_ctx* c, const char* who, const int* ipos, int nprobe, 
   qs::launch_fill_zero(A.gx, st_elems, c->stream);
   A.f.trans = trans_base(c) + (size_t)(S - 1) * slice_elems;
   A.xprev = c->state.as<float2>();
-  qs::launch_adj_cols(A, true, c->stream);
+  qs::launch_adj_cols(A, true, false, c->stream);
   // loss = sum over band pixels (partials) + sum of target^2 outside the band
   qs::launch_target_outside_band(c->target.as<float>(), nprobe, p.nx, p.ny, c->band_a, c->band_b,
                                  c->dsum.as<double>() + 2, c->stream);
@@ -1789,7 +1789,7 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
       A.f.trans = trans_base(c) + (size_t)s * slice_elems;
       A.gT = (b ? c->gT2 : c->gT).as<float>() + (gslices > 1 ? 2 * (size_t)s * slice_elems : 0);
       A.xprev = nullptr;
-      if (s < S - 1) qs::launch_adj_cols(A, false, c->stream);
+      if (s < S - 1) qs::launch_adj_cols(A, false, overlap, c->stream);
       if (s > 0) A.xprev = (s - 1 >= s0) ? c->ckpt[nseg + (s - 1 - s0)].as<float2>() : c->ckpt[j].as<float2>();
       if (pending[b]) HIPCHK(c, hipStreamWaitEvent(c->stream, c->evbl[b], 0));  // slice s+2's band limit read it
       if (grad_each && !whole) qs::launch_cplx_fill_zero(A.gT, 2 * slice_elems, c->stream);

@@ -144,7 +144,9 @@ void launch_probe_window(const ProbeArgs& a, float2* psi, double* partial, hipSt
 void launch_fill_zero(float2* p, size_t n, hipStream_t st);
 hipError_t set_col_lds_limits();
 void launch_adj_rows(const AdjArgs& a, bool first_slice, hipStream_t st);
-void launch_adj_cols(const AdjArgs& a, bool last_slice, hipStream_t st);
+// direct: allow the register-staged variant where the size has one (chosen when the
+// band limit of dL/dt runs beside the reverse sweeps on the second stream)
+void launch_adj_cols(const AdjArgs& a, bool last_slice, bool direct, hipStream_t st);
 void launch_grad_v(const float2* gu, const float* V, float* gV, size_t n, double sigma, hipStream_t st);
 void launch_target_outside_band(const float* target, int nprobe, int nx, int ny, int band_a, int band_b,
                                 double* out, hipStream_t st);

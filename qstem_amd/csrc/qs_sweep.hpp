@@ -649,7 +649,10 @@ void blue_t(float2* d, long nrows, int n, const float2* chirp, const float2* vha
 //
 // Column sweep (slice s, reverse):  g_Phi = colFFT(g_X)  [last slice: 4 (I - T) Phi,
 // Phi = colFFT(X_{S-1})/nx, I = |Phi|^2]  ->  g_Psi = conj(P) g_Phi  ->  g_R = colIFFT(g_Psi).
-template <int NX, bool LAST>
+// DIRECT: at sizes where col_direct holds, the non-last sweep takes its staging loads
+// straight into the transform registers (one LDS column slot); measured faster only with
+// the band limit of dL/dt running beside it on the second stream, slower alone.
+template <int NX, bool LAST, bool DIRECT>
 __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_adj_cols(AdjArgs A) {
   const SweepArgs& a = A.f;
   constexpr int E = Plan<NX>::E;
@@ -666,7 +669,7 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_adj_cols(AdjArgs A) {
   const int pg = a.probe0 + p;
   const int c0 = tile_id * TC;
   const size_t off = (size_t)pg * a.nx * a.kcp;
-  if constexpr (col_direct<NX>() && !LAST) {
+  if constexpr (col_direct<NX>() && !LAST && DIRECT) {
     // as the forward column sweep at this size: the staging loads are the transform's
     // registers, one LDS column slot for the exchanges
     const int t = threadIdx.x % T;
@@ -848,20 +851,25 @@ void launch_adj_rows_t(const AdjArgs& a, bool first, hipStream_t st) {
 }
 
 template <int N>
-void launch_adj_cols_t(const AdjArgs& a, bool last, hipStream_t st) {
-  const size_t lds = col_direct<N>() && !last ? sizeof(float2) * ColCfg<N>::STRIDE : col_lds_bytes_t<N>();
+void launch_adj_cols_t(const AdjArgs& a, bool last, bool direct, hipStream_t st) {
+  const bool dir = col_direct<N>() && !last && direct;
+  const size_t lds = dir ? sizeof(float2) * ColCfg<N>::STRIDE : col_lds_bytes_t<N>();
   dim3 grid((unsigned)(a.f.nprobe * a.f.ntiles));
-  if (last) hipLaunchKernelGGL((k_adj_cols<N, true>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
-  else hipLaunchKernelGGL((k_adj_cols<N, false>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
+  if (last) hipLaunchKernelGGL((k_adj_cols<N, true, false>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
+  else if (dir) hipLaunchKernelGGL((k_adj_cols<N, false, true>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
+  else hipLaunchKernelGGL((k_adj_cols<N, false, false>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
 }
 
 template <int N>
 hipError_t set_adj_lds_t() {
   const int b = col_lds_bytes_t<N>();
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, true>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, true, false>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, b);
   if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, false>),
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, false, false>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, b);
+  if (e == hipSuccess && col_direct<N>())
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, false, true>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, b);
   return e;
 }

@@ -82,7 +82,8 @@ def test_checkpoint_interval_does_not_change_gradients():
             # 0: the library's choice, every slice kept when the states fit in memory
             assert ms.last_ckpt_interval == (min(k, cfg.slices) if k > 0 else 1), k
     for r in res[1:]:
-        assert r["loss"] == res[0]["loss"]
+        # the last column sweep sums the loss with LDS float atomics: order-dependent rounding
+        assert abs(r["loss"] - res[0]["loss"]) <= 1e-6 * abs(res[0]["loss"])
         g0 = res[0]["grad_V"]
         assert np.linalg.norm(r["grad_V"] - g0) <= 1e-6 * np.linalg.norm(g0)  # atomics: order-dependent rounding
         assert abs(r["grad_df0"] - res[0]["grad_df0"]) <= 1e-6 * abs(res[0]["grad_df0"])
@@ -267,16 +268,19 @@ def test_adjoint_plane_wave_large(oracle, nx):
     assert abs(ad - fd) <= 5e-3 * abs(fd), (ad, fd)
 
 
-def test_band_limit_overlap_keeps_gradients():
+@pytest.mark.parametrize("nx,slices", [(256, 5), (4096, 3)])
+def test_band_limit_overlap_keeps_gradients(nx, slices):
     """With two streams the band limit of dL/dt of slice s runs beside the reverse
     sweeps of slice s-1 (two dL/dt buffers in turn); one stream runs them in order.
     Loss and dL/dV agree (dL/dt adds in an order set by atomics only where windows
-    overlap; here one window covers the slice, so they are equal to rounding)."""
+    overlap; here one window covers the slice, so they are equal to rounding).  At
+    4096^2 the two modes also take the two variants of the reverse column sweep
+    (registers staged directly beside the overlap, through LDS without it)."""
     from qstem_amd import Multislice, StemConfig
-    nx = 256
-    cfg = StemConfig(nx=nx, ny=nx, resolutionX=0.05, resolutionY=0.05, v0=300.0, slices=5, sliceThickness=1.9525,
-                     potNx=nx, potNy=nx, bandlimittrans=1, Cs=0.0, df0=0.0, alpha=20.0, electronScale=1.0)
-    V = smooth_potential(5, nx, nx, seed=63, vmax=40.0, sigma_px=4.0)
+    cfg = StemConfig(nx=nx, ny=nx, resolutionX=0.05, resolutionY=0.05, v0=300.0, slices=slices,
+                     sliceThickness=1.9525, potNx=nx, potNy=nx, bandlimittrans=1, Cs=0.0, df0=0.0, alpha=20.0,
+                     electronScale=1.0)
+    V = smooth_potential(slices, nx, nx, seed=63, vmax=40.0, sigma_px=4.0)
     ipos = np.zeros((1, 2), np.int32)
     target = np.zeros((1, nx, nx), np.float32)
     outs = {}

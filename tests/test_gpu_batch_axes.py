@@ -146,3 +146,68 @@ def test_two_streams_equal_one_stream():
     for g in range(2):
         assert rel_l2(outs[2][g]["det"], outs[1][g]["det"]) < 1e-6, g
         assert np.allclose(outs[2][g]["intensity"], outs[1][g]["intensity"], rtol=1e-6), g
+
+
+def test_uneven_frozen_phonon_carbon_gold(oracle):
+    """The uneven frozen-phonon configuration, reduced: a seeded amorphous-carbon slab
+    (2.0 g/cm^3, 1.4 A minimum separation) 25.6 x 25.6 x 8 A with an embedded gold FCC
+    sphere of 5 A radius; 3 Gaussian phonon configurations (sigma 0.08 A) built on the
+    GPU (qs_build_potential, tds); 6 slices of 1 A on a 1024^2 grid at 0.025 A, 200 kV;
+    windows of 1024^2, 512^2 and 256^2 advanced together by one qs_run_groups call.
+    Every item's detector images (by qs_run_groups) and exit wave (by its context's
+    qs_run_stem) against the FP64 oracle (LUTs, potential, transmission, probe and
+    multislice recomputed from the same atoms) at 1e-4 relative L2."""
+    from qstem_amd import Detector, Multislice, StemConfig, record_bins, run_groups
+    from qstem_amd.specimen import carbon_gold_specimen, gaussian_phonons
+    dx, potN, S, R = 0.025, 1024, 6, 2.5
+    atoms = carbon_gold_specimen(box=(25.6, 25.6, 8.0), sphere_radius=5.0, seed=3)
+    assert np.sum(atoms[:, 6] == 79) > 20 and np.sum(atoms[:, 6] == 6) > 200
+    confs = [gaussian_phonons(atoms, 0.08, seed=k + 1) for k in range(3)]
+    dets = [Detector(0.0, 10.0, "BF"), Detector(40.0, 100.0, "ADF")]
+
+    def cfg(n):
+        return StemConfig(nx=n, ny=n, resolutionX=dx, resolutionY=dx, v0=200.0, slices=S, sliceThickness=1.0,
+                          potNx=potN, potNy=potN, bandlimittrans=1, Cs=0.0, df0=0.0, alpha=20.0, detectors=dets)
+
+    items = {1024: ([[0, 0]], [1]), 512: ([[100, 300], [512, 0]], [0, 2]), 256: ([[700, 20], [300, 768]], [2, 1])}
+    bins, nbins = record_bins(S, outputInterval=2)
+    ctxs = {n: Multislice(cfg(n)) for n in items}
+    try:
+        big = ctxs[1024]
+        big.set_configs(3)
+        Vg0 = None
+        for k in range(3):
+            big.select_config(k)
+            V = big.build_potential(confs[k], atomRadius=R, tds=1, return_V=(k == 0))
+            Vg0 = V if k == 0 else Vg0
+        for n in (512, 256):
+            ctxs[n].share_trans(big)
+        groups = []
+        for n, (ip, cf) in items.items():
+            ctxs[n].make_probe()
+            groups.append((ctxs[n], np.array(ip, np.int32), np.array(cf, np.int32)))
+        print("uneven carbon/gold: run_groups over %s" % list(items), flush=True)
+        outs = run_groups(groups, bins=bins, nbins=nbins)
+        exits = [ms.run_stem(ip, bins=bins, nbins=nbins, config=cf, want_exit=True)["exit"] for ms, ip, cf in groups]
+    finally:
+        for ms in ctxs.values():
+            ms.close()
+    # the oracle: LUTs of C and Au (B = 0 with frozen phonons), potential and transmission per configuration
+    g = oracle.lut_geometry(R, dx, 1.0)
+    luts = oracle.atom_luts(g, [6, 79], [0.0, 0.0], 1.0)
+    trans = []
+    for k in range(3):
+        kinds = [0 if int(z) == 6 else 1 for z in confs[k][:, 6]]
+        Vo = oracle.make_slices(cfg(1024), g, luts, kinds, [tuple(a) for a in confs[k]], 0.0, 0.0, atomRadius=R)
+        if k == 0:
+            assert rel_l2(Vg0, Vo) < 2e-5
+        trans.append(oracle.make_trans(cfg(1024), Vo))
+    for (n, (ip, cf)), out, ex in zip(items.items(), outs, exits):
+        psi0 = oracle.probe(cfg(n))
+        for i in range(len(ip)):
+            exo, deto, _ = oracle.multislice(cfg(n), trans[cf[i]], np.array(ip[i:i + 1], np.int32), psi0, bins=bins,
+                                             nbins=nbins)
+            e_ex = rel_l2(ex[i], exo[0])
+            e_det = max(rel_l2(out["det"][b, d, i], deto[b, d, 0]) for b in range(nbins) for d in range(len(dets)))
+            print("  %d^2 item %d (config %d): exit %.2e, detectors %.2e" % (n, i, cf[i], e_ex, e_det))
+            assert e_ex < 1e-4 and e_det < 1e-4, (n, i)
