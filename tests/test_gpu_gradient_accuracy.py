@@ -104,3 +104,37 @@ def test_atom_gradient_accuracy(oracle):
     err = rel_l2(g_gpu, g_ref)
     print("atom gradients (%d atoms): relative L2 error vs FP64 chain %.2e" % (na, err))
     assert err < 1e-4
+
+
+@pytest.mark.parametrize("n,slices,nprobe,pad", [(512, 4, 3, 32), (1024, 3, 1, 0), (2048, 2, 1, 0)])
+def test_gradient_accuracy_large_sizes(oracle, n, slices, nprobe, pad):
+    """dL/dV, dL/d(df0, Cs) of the kept column and row sweep shapes at 512^2 (16-column
+    tiles, 512-thread blocks), 1024^2 (8-column tiles, {16,8,8} over 64 lanes) and
+    2048^2 (block-wide {8,8,8,4}) against the FP64 adjoint: an indexing or
+    synchronisation defect of the reverse sweeps would show here even where the forward
+    parity tests pass.  Probe incident waves over a smooth strong phase object (the
+    plane-wave weak-phase case is FP32-conditioned, as the FD tests note).  At 512^2
+    three probes' windows overlap (dL/dt by atomics); at 1024^2 and 2048^2 one window
+    covers the whole slice (dL/dt stored by its single writer)."""
+    from qstem_amd import Multislice, StemConfig
+    dx = 0.025 if n >= 1024 else 0.03
+    cfg = StemConfig(nx=n, ny=n, resolutionX=dx, resolutionY=dx, v0=200.0, slices=slices, sliceThickness=2.0,
+                     potNx=n + pad, potNy=n + pad, bandlimittrans=1, Cs=1.0e4, df0=-40.0, alpha=25.0,
+                     electronScale=1.0)
+    rng = np.random.default_rng(n)
+    V = _random_potential(slices, n + pad, dx, seed=n + 1, max_phase=1.5)
+    ipos = rng.integers(0, pad + 1, (nprobe, 2)).astype(np.int32)
+    ex, _, _ = oracle.multislice(cfg, oracle.make_trans_d(cfg, 0.9 * V.astype(np.float64)), ipos,
+                                 oracle.probe_d(cfg, -40.0, 1.0e4))
+    target = (np.abs(np.fft.fft2(ex, axes=(1, 2)) / (n * n)) ** 2).astype(np.float32)
+    with Multislice(cfg) as ms:
+        ms.set_potential(V)
+        ms.make_probe()
+        g = ms.adjoint(ipos, target)
+    psi0 = oracle.probe_d(cfg, float(np.float32(cfg.df0)), float(np.float32(cfg.Cs)))
+    loss, gV, gpsi = oracle.adjoint(cfg, V.astype(np.float64), ipos, psi0, target.astype(np.float64))
+    g_df, g_cs = oracle.probe_param_gradients(cfg, gpsi)
+    e = {"loss": abs(g["loss"] / loss - 1), "grad_V": rel_l2(g["grad_V"], gV),
+         "grad_df0": abs(g["grad_df0"] / g_df - 1), "grad_Cs": abs(g["grad_Cs"] / g_cs - 1)}
+    print("%d^2: relative errors vs FP64 adjoint: " % n + ", ".join("%s %.2e" % kv for kv in e.items()))
+    assert max(e.values()) < 1e-4
