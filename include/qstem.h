@@ -46,7 +46,9 @@ extern "C" {
 /* The MULS fields the STEM path reads (libs/data_containers.h:103-308).
  * Float where MULS stores float_tt, so the same rounded values are used. */
 typedef struct qs_params {
-  int nx, ny;                 /* wave grid; powers of two, 64..4096 */
+  int nx, ny;                 /* wave grid: powers of two 64..4096 take the fused sweeps; any
+                               * other size in 16..2048 (e.g. 400) an unfused path (qs_run_stem
+                               * only: no adjoint, uneven batch, autotuner or no-allocation run) */
   float resolutionX, resolutionY;
   float v0;                   /* kV */
   int slices;                 /* slices per slab */

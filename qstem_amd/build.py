@@ -58,7 +58,7 @@ def build_lib(force=True, jobs=8):
         src = os.path.join(CSRC, "qs_sizes.hip")
         obj = os.path.join(OBJ, "qs_sizes_%d.o" % n)
         jobs_list.append((obj, [HIPCC] + HIPFLAGS + ["-DQS_N=%d" % n, "-c", src, "-o", obj], [src] + headers))
-    for name in ("qs_setup.hip", "qs_capi.cpp", "qs_potential.hip", "qs_adjoint.hip"):
+    for name in ("qs_setup.hip", "qs_capi.cpp", "qs_potential.hip", "qs_adjoint.hip", "qs_generic.hip"):
         src = os.path.join(CSRC, name)
         if not os.path.exists(src):
             continue

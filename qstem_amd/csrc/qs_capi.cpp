@@ -158,6 +158,11 @@ struct qs_ctx {
   DevBuf state, ipos, partial, partial2, det, diff, exitw, total;
   DevBuf pkx2, pky2, probe_partial;
   RowXform rx, ry;  // potNx and potNy row transforms
+  // wave grids that are not powers of two (or outside 64..4096): the unfused path of
+  // qs_generic.hip with row transforms of lengths nx and ny
+  bool generic = false;
+  RowXform wx, wy;
+  DevBuf gscratch;
   // band limit of the transmission functions as band-compact sweeps (potNx and
   // potNy powers of two): row FFT into the ky band, column FFT / disk mask with
   // 1/(potNx potNy) / column IFFT, row IFFT; else row transforms and transposes
@@ -317,8 +322,9 @@ int qs_create(const qs_params* p, qs_ctx** out) {
   *out = nullptr;
   g_create_error.clear();
   if (!p) { g_create_error = "params is NULL"; return QS_ERR_ARG; }
-  if (!qs::supported_size(p->nx) || !qs::supported_size(p->ny)) {
-    g_create_error = "nx and ny must be powers of two in [64, 4096]";
+  const bool fused = qs::supported_size(p->nx) && qs::supported_size(p->ny);
+  if (!fused && (p->nx < 16 || p->ny < 16 || p->nx > 2048 || p->ny > 2048)) {
+    g_create_error = "nx and ny must be powers of two in [64, 4096] or any size in [16, 2048]";
     return QS_ERR_UNSUPPORTED;
   }
   if (p->potNx < p->nx || p->potNy < p->ny) { g_create_error = "potNx/potNy smaller than nx/ny"; return QS_ERR_ARG; }
@@ -390,7 +396,8 @@ int qs_create(const qs_params* p, qs_ctx** out) {
   c->band_a = a;
   c->band_b = b;
   c->kc = a + 1 + b;
-  c->tc = qs::col_tile(nx);
+  c->generic = !fused;
+  c->tc = fused ? qs::col_tile(nx) : 1;
   c->ntiles = (c->kc + c->tc - 1) / c->tc;
   // row pitch of the band-compact state: whole 128-byte lines, so that the two
   // halves of a line are adjacent column tiles (one block advances both)
@@ -422,6 +429,11 @@ int qs_create(const qs_params* p, qs_ctx** out) {
       upload(c->dkx2, c->kx2, c->stream) != hipSuccess || upload(c->dky2, c->ky2, c->stream) != hipSuccess) {
     c->err = "device allocation of lookup tables failed";
     return bad(QS_ERR_HIP);
+  }
+  if (c->generic) {  // row transforms of the wave's lengths (Stockham or Bluestein)
+    int rc = setup_rowxform(c, c->wx, nx);
+    if (rc == QS_OK) rc = setup_rowxform(c, c->wy, ny);
+    if (rc != QS_OK) return bad(rc);
   }
   // transmission band-limit tables (stemlib.cpp:2219-2262), float arithmetic, Q1 optional
   if (p->bandlimittrans) {
@@ -502,7 +514,9 @@ void qs_destroy(qs_ctx* c) {
                     &c->lx.tw, &c->lx.chirp_f, &c->lx.vhat_f, &c->lx.chirp_b, &c->lx.vhat_b,
                     &c->lz.tw, &c->lz.chirp_f, &c->lz.vhat_f, &c->lz.chirp_b, &c->lz.vhat_b,
                     &c->gx, &c->target, &c->lossp, &c->gT, &c->gT2, &c->gprobe, &c->gV, &c->dsum,
-                    &c->blpx, &c->blpy, &c->blstate, &c->blstate2, &c->adj_ipos};
+                    &c->blpx, &c->blpy, &c->blstate, &c->blstate2, &c->adj_ipos,
+                    &c->wx.tw, &c->wx.chirp_f, &c->wx.vhat_f, &c->wx.chirp_b, &c->wx.vhat_b,
+                    &c->wy.tw, &c->wy.chirp_f, &c->wy.vhat_f, &c->wy.chirp_b, &c->wy.vhat_b, &c->gscratch};
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : c->luts) b.release();
   for (DevBuf& b : c->oluts) b.release();
@@ -963,9 +977,11 @@ int qs_make_probe(qs_ctx* c, float* out) {
   float2* sc = c->scratch.as<float2>();
   qs::launch_probe_kspace(a, pr, c->stream);
   // 2D inverse FFT (unnormalised): rows over ky, transpose, rows over kx, transpose back
-  qs::launch_rows_fft(pr, nx, ny, +1, c->twy.as<float2>(), 1.0f, c->stream);
+  if (c->generic) run_rowxform(c, c->wy, pr, nx, +1, 1.0f);
+  else qs::launch_rows_fft(pr, nx, ny, +1, c->twy.as<float2>(), 1.0f, c->stream);
   qs::launch_transpose(pr, sc, 1, nx, ny, c->stream);
-  qs::launch_rows_fft(sc, ny, nx, +1, c->twx.as<float2>(), 1.0f, c->stream);
+  if (c->generic) run_rowxform(c, c->wx, sc, ny, +1, 1.0f);
+  else qs::launch_rows_fft(sc, ny, nx, +1, c->twx.as<float2>(), 1.0f, c->stream);
   qs::launch_transpose(sc, pr, 1, ny, nx, c->stream);
   qs::launch_probe_window(a, pr, c->probe_partial.as<double>(), c->stream);
   HIPCHK(c, hipGetLastError());
@@ -1353,9 +1369,115 @@ static int finish_run(RunPlan& r, hipStream_t st) {
   return QS_OK;
 }
 
+// qs_run_stem for wave grids that are not powers of two: per slice and probe group,
+// transmit, forward 2D FFT (rows, transpose, rows), propagator and detector sums,
+// inverse 2D FFT, on one stream over a full-size state (qs_generic.hip).
+static int run_generic(qs_ctx* c, const int* ipos, const int* config, int nprobe, int start, const int* bins,
+                       int nslice, int nbins, float* det, float* diff, float* exit_wave, float* int_intensity) {
+  const char* who = "qs_run_stem";
+  int rc = check_run(c, who, ipos, config, nprobe, start, bins, nslice, nbins, det);
+  if (rc != QS_OK) return rc;
+  const qs_params& p = c->p;
+  const int nx = p.nx, ny = p.ny;
+  const size_t n2 = (size_t)nx * ny, slice = (size_t)p.potNx * p.potNy;
+  // probe group: its transposed copy at most 256 MB
+  const int B = c->batch > 0 ? std::min(c->batch, nprobe)
+                             : (int)std::max<size_t>(1, std::min<size_t>(nprobe, (size_t(256) << 20) / (8 * n2)));
+  const int nchunk = qs::gen_chunks(ny);
+  const bool want_det = det && p.ndet > 0;
+  hipStream_t st = c->stream;
+  HIPCHK(c, c->state.ensure(sizeof(float2) * n2 * nprobe));
+  HIPCHK(c, c->gscratch.ensure(sizeof(float2) * n2 * B));
+  HIPCHK(c, c->ipos.ensure(sizeof(int) * 2 * nprobe));
+  HIPCHK(c, hipMemcpyAsync(c->ipos.p, ipos, sizeof(int) * 2 * nprobe, hipMemcpyHostToDevice, st));
+  if (config) {
+    HIPCHK(c, c->cfgidx.ensure(sizeof(int) * nprobe));
+    HIPCHK(c, hipMemcpyAsync(c->cfgidx.p, config, sizeof(int) * nprobe, hipMemcpyHostToDevice, st));
+  }
+  HIPCHK(c, c->partial.ensure(sizeof(float) * (size_t)B * nchunk * (qs::kMaxDet + 1)));
+  const size_t nd = want_det ? (size_t)nbins * p.ndet * nprobe : 0;
+  if (want_det) {
+    HIPCHK(c, c->det.ensure(sizeof(float) * nd));
+    HIPCHK(c, hipMemcpyAsync(c->det.p, det, sizeof(float) * nd, hipMemcpyHostToDevice, st));
+  }
+  if (diff) HIPCHK(c, c->diff.ensure(sizeof(float) * n2 * nprobe));
+  if (int_intensity) HIPCHK(c, c->total.ensure(sizeof(float) * nprobe));
+  HIPCHK(c, hipEventRecord(c->ev0, st));
+  qs::GenPropArgs pa{};
+  pa.nx = nx;
+  pa.ny = ny;
+  pa.px = c->px.as<float2>();
+  pa.py = c->py.as<float2>();
+  pa.kx2 = c->dkx2.as<float>();
+  pa.ky2 = c->dky2.as<float>();
+  pa.k2max = c->k2max;
+  pa.ndet = p.ndet;
+  for (int d = 0; d < p.ndet; d++) {
+    pa.det_sx[d] = (int)p.det_shiftx[d];
+    pa.det_sy[d] = (int)p.det_shifty[d];
+    pa.det_k2in[d] = c->det_k2in[d];
+    pa.det_k2out[d] = c->det_k2out[d];
+  }
+  pa.partial = c->partial.as<float>();
+  pa.diff_scale = (float)pow((double)n2, 1.5);
+  float2* sc = c->gscratch.as<float2>();
+  for (int p0 = 0; p0 < nprobe; p0 += B) {
+    const int nb = std::min(B, nprobe - p0);
+    float2* stt = c->state.as<float2>() + (size_t)p0 * n2;
+    for (int s = 0; s < nslice; s++) {
+      qs::GenArgs g{};
+      g.nx = nx;
+      g.ny = ny;
+      g.nprobe = nb;
+      g.probe0 = p0;
+      g.potNy = p.potNy;
+      g.probe = c->probe.as<float2>();
+      g.trans = trans_base(c) + (size_t)s * slice;
+      g.ipos = c->ipos.as<int2>();
+      g.cfg = config ? c->cfgidx.as<int>() : nullptr;
+      g.cfg_stride = (long long)trans_slices_of(c) * (long long)slice;
+      g.state = stt;
+      qs::launch_gen_transmit(g, s == 0 && start == 0, st);
+      run_rowxform(c, c->wy, stt, (long)nb * nx, -1, 1.0f);
+      qs::launch_transpose(stt, sc, nb, nx, ny, st);
+      run_rowxform(c, c->wx, sc, (long)nb * ny, -1, 1.0f);
+      const int bin = want_det && bins ? bins[s] : -1;
+      const bool last = s == nslice - 1, tot = last && int_intensity;
+      pa.data = sc;
+      pa.probe0 = p0;
+      pa.rec = (bin >= 0 || tot) ? 1 : 0;
+      pa.diff = (last && diff) ? c->diff.as<float>() : nullptr;
+      qs::launch_gen_propagate(pa, nb, st);
+      if (pa.rec)
+        qs::launch_reduce_partials(pa.partial, nb, nchunk, bin >= 0 ? p.ndet : 0, p.electronScale,
+                                   bin >= 0 ? c->det.as<float>() + (size_t)bin * p.ndet * nprobe : nullptr, nprobe,
+                                   tot ? c->total.as<float>() : nullptr, p0, st);
+      run_rowxform(c, c->wx, sc, (long)nb * ny, +1, 1.0f);
+      qs::launch_transpose(sc, stt, nb, ny, nx, st);
+      run_rowxform(c, c->wy, stt, (long)nb * nx, +1, 1.0f);
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, st));
+  if (want_det) HIPCHK(c, hipMemcpyAsync(det, c->det.p, sizeof(float) * nd, hipMemcpyDeviceToHost, st));
+  if (diff) HIPCHK(c, hipMemcpyAsync(diff, c->diff.p, sizeof(float) * n2 * nprobe, hipMemcpyDeviceToHost, st));
+  if (exit_wave)
+    HIPCHK(c, hipMemcpyAsync(exit_wave, c->state.p, sizeof(float2) * n2 * nprobe, hipMemcpyDeviceToHost, st));
+  if (int_intensity)
+    HIPCHK(c, hipMemcpyAsync(int_intensity, c->total.p, sizeof(float) * nprobe, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  c->state_valid = true;
+  c->state_probes = nprobe;
+  return QS_OK;
+}
+
 int qs_run_stem_cfg(qs_ctx* c, const int* ipos, const int* config, int nprobe, int start, const int* bin_of_slice,
                     int nslice, int nbins, float* det, float* diff, float* exit_wave, float* int_intensity) {
   if (!c) return QS_ERR_ARG;
+  if (c->generic)
+    return run_generic(c, ipos, config, nprobe, start, bin_of_slice, nslice, nbins, det, diff, exit_wave,
+                       int_intensity);
   RunPlan r;
   int rc = plan_run(c, "qs_run_stem", ipos, config, nprobe, start, bin_of_slice, nslice, nbins, det, diff, exit_wave,
                     int_intensity, r);
@@ -1460,6 +1582,8 @@ int qs_run_stem_noalloc(qs_ctx* c, const int* ipos, int nprobe, const int* bin_o
                         float* det, float* int_intensity) {
   if (!c) return QS_ERR_ARG;
   const char* who = "qs_run_stem_noalloc";
+  if (c->generic)
+    return fail(c, QS_ERR_UNSUPPORTED, std::string(who) + ": wave grids that are not powers of two use qs_run_stem");
   int rc = check_run(c, who, ipos, nullptr, nprobe, 0, bin_of_slice, nslice, nbins, det);
   if (rc != QS_OK) return rc;
   const qs_params& p = c->p;
@@ -1528,6 +1652,8 @@ int qs_run_groups(const qs_group* groups, int ngroup, const int* bin_of_slice, i
   for (int g = 0; g < ngroup; g++) {
     qs_ctx* c = groups[g].ctx;
     if (!c) return fail(c0, QS_ERR_ARG, "qs_run_groups: null context");
+    if (c->generic)
+      return fail(c0, QS_ERR_UNSUPPORTED, "qs_run_groups: wave grids that are not powers of two use qs_run_stem");
     if (trans_base(c) != t0 || trans_slices_of(c) != trans_slices_of(c0))
       return fail(c0, QS_ERR_ARG, "qs_run_groups: contexts must share one transmission function (qs_share_trans)");
     for (int h = 0; h < g; h++)
@@ -1658,6 +1784,8 @@ static int adj_forward(qs_ctx* c, const char* who, const int* ipos, int nprobe, 
   if (!c) return QS_ERR_ARG;
   c->adj_saved = false;
   if (!ipos || nprobe < 1 || !target) return fail(c, QS_ERR_ARG, std::string(who) + ": bad arguments");
+  if (c->generic)
+    return fail(c, QS_ERR_UNSUPPORTED, std::string(who) + ": the adjoint needs power-of-two wave grids");
   const qs_params& p = c->p;
   const int S = trans_slices_of(c);
   if (S < 1) return fail(c, QS_ERR_STATE, std::string(who) + ": no transmission functions");
@@ -1959,6 +2087,7 @@ std::map<TuneKey, TuneVal> g_tune_cache;
 int qs_autotune(qs_ctx* c, const int* ipos, int nprobe, int* batch, int* rows_per_wave, int* tiles_per_block,
                 float* rate) {
   if (!c || !ipos || nprobe < 1) return fail(c, QS_ERR_ARG, "qs_autotune: bad arguments");
+  if (c->generic) return fail(c, QS_ERR_UNSUPPORTED, "qs_autotune: tunes the power-of-two sweeps only");
   if (trans_slices_of(c) < 1 || !c->have_probe)
     return fail(c, QS_ERR_STATE, "qs_autotune: set the potential and the probe first");
   int dev = 0;

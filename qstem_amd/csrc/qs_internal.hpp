@@ -124,6 +124,36 @@ void launch_bin_atoms(const AtomReach* reach, int natom, const ScatterArgs& s, i
 void launch_gather(const AtomReach* reach, const int* tstart, const int* list, const float* const* luts,
                    const float* const* oluts, const ScatterArgs& s, float* V, hipStream_t st);
 
+// Multislice of non-power-of-two wave grids (qs_generic.hip): full-size state
+// [probe][nx][ny], separate transmit / row-transform / transpose / propagate passes.
+struct GenArgs {
+  int nx, ny, nprobe, probe0, potNy;
+  const float2* probe;  // [nx][ny] (first slice)
+  const float2* trans;  // transmission slice [potNx][potNy] of configuration 0
+  const int2* ipos;     // window offset per global probe
+  const int* cfg;       // nullable: configuration per global probe
+  long long cfg_stride;
+  float2* state;        // [nprobe][nx][ny]
+};
+struct GenPropArgs {
+  int nx, ny, probe0;
+  float2* data;                // [nprobe][ny][nx] spectrum (transposed)
+  const float2* px;            // propagator x factor (includes 1/(nx ny))
+  const float2* py;
+  const float* kx2;
+  const float* ky2;
+  float k2max;
+  int rec, ndet;
+  int det_sx[kMaxDet], det_sy[kMaxDet];
+  float det_k2in[kMaxDet], det_k2out[kMaxDet];
+  float* partial;              // [nprobe][gen_chunks(ny)][kMaxDet+1] when rec
+  float* diff;                 // nullable: [nprobe_total][nx][ny]
+  float diff_scale;
+};
+void launch_gen_transmit(const GenArgs& a, bool first, hipStream_t st);
+int gen_chunks(int ny);
+void launch_gen_propagate(const GenPropArgs& a, int nprobe, hipStream_t st);
+
 bool supported_size(int n);
 int col_tile(int nx);
 

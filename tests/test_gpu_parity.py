@@ -180,3 +180,58 @@ def test_autotune_keeps_results():
         out = ms.run_stem(ipos, want_exit=True)
     assert np.array_equal(out["exit"], ref["exit"])
     assert rel_l2(out["det"], ref["det"]) < 1e-6
+
+
+@pytest.mark.parametrize("shape", [(400, 400), (96, 160), (200, 128)])
+def test_generic_grid_sizes_match_oracle(oracle, shape):
+    """Wave grids that are not powers of two (the reference's stem_STO_4x4.qsc uses
+    400 x 400) take the unfused path: probe, exit waves, every detector image (with a
+    shifted detector), diffraction patterns and intensities against the oracle, and
+    a run continued over a second call equal to one call over all slices."""
+    nx, ny = shape
+    cfg = small_config(nx=nx, ny=ny, pad=24, slices=4, dx=0.03)
+    V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=12)
+    V[2:] = V[:2]  # a continued call restarts at slice 0: slices 2, 3 repeat 0, 1
+    ipos = probe_positions(cfg, 3, seed=6)
+    bins, nbins = oracle.record_bins(cfg.slices, outputInterval=2)
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        t32 = ms.get_trans()
+        psi0 = ms.make_probe()
+        ms.set_batch(2)  # two probe groups
+        out = ms.run_stem(ipos, bins=bins, nbins=nbins, want_diff=True, want_exit=True, want_intensity=True)
+        ms.run_stem(ipos, nslice=2, bins=bins[:2], nbins=nbins, want_det=False)
+        cont = ms.run_stem(ipos, nslice=2, start=1, bins=bins[2:], nbins=nbins, want_det=False, want_exit=True)
+    assert rel_l2(psi0, oracle.probe(cfg)) < 2e-5
+    assert rel_l2(t32, oracle.make_trans(cfg, V)) < 2e-5
+    ex, det, diff = oracle.multislice(cfg, t32.astype(np.complex128), ipos, psi0.astype(np.complex128),
+                                      bins=bins, nbins=nbins, want_diff=True)
+    assert rel_l2(out["exit"], ex) < TOL
+    assert rel_l2(out["diff"], diff) < TOL
+    checked = 0
+    for b in range(nbins):
+        for d in range(det.shape[1]):
+            if np.any(det[b, d] != 0):
+                assert rel_l2(out["det"][b, d], det[b, d]) < TOL, (b, d)
+                checked += 1
+    assert checked >= 2 * nbins
+    norm = np.sum(np.abs(ex) ** 2, axis=(1, 2)) / (nx * ny)
+    assert np.max(np.abs(out["intensity"] - norm) / norm) < TOL
+    assert rel_l2(cont["exit"], out["exit"]) < 1e-6
+
+
+def test_generic_grid_refusals():
+    """The adjoint, uneven batches, the autotuner and the no-allocation run need the
+    power-of-two sweeps: they refuse other grids with an error, before any launch."""
+    from qstem_amd import QstemError, run_groups
+    cfg = small_config(nx=96, ny=96, pad=8, slices=2, dx=0.05)
+    with _ctx(cfg) as ms:
+        ms.set_potential(smooth_potential(2, cfg.potNx, cfg.potNy, seed=3))
+        ms.make_probe()
+        ipos = np.zeros((1, 2), np.int32)
+        for call in (lambda: ms.adjoint(ipos, np.zeros((1, 96, 96), np.float32)),
+                     lambda: ms.run_stem_noalloc(ipos),
+                     lambda: ms.autotune(ipos),
+                     lambda: run_groups([(ms, ipos, None)])):
+            with pytest.raises(QstemError, match="power"):
+                call()
