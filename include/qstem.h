@@ -14,11 +14,14 @@
  *   qs_set_trans       direct write of muls.trans                  libs/data_containers.h:112-116
  *   qs_get_trans       read back of muls.trans                     stem3/stemlib.cpp:1280-1306 (save potential)
  *   qs_make_probe      probe(MULS*, WavePtr, double dx, double dy) stem3/stemlib.cpp:1952-2191
+ *   qs_make_probe_at   probe() at any window position (doCBED)     stem3/stem3.cpp:2097
  *   qs_run_stem        doSTEM scan loop body: runMulsSTEM +        stem3/stem3.cpp:2820-2946,
  *                      transmit/propagate_slow/collectIntensity/   stem3/stemlib.cpp:2375-2534, 2576-2687,
  *                      fft_normalize for a batch of positions     2760-2906
  *   qs_build_potential make3DSlices + getAtomPotential3D            stem3/stemlib.cpp:383-1541
+ *   qs_run_slices      runMulsSTEM over a sub-range of the slab    stem3/stemlib.cpp:2403-2482
  *   qs_record_bins     tCount/interval bookkeeping of collectIntensity stem3/stemlib.cpp:2588-2607
+ *   qs_record_bins_repeat  the same with "sequence: r p" repeats    stem3/stemlib.cpp:2403, 2429
  *   qs_last_error      (none: the reference prints and exits)
  *
  * Threading: one context = one stream; calls on one context must not overlap.
@@ -141,15 +144,26 @@ int qs_build_potential(qs_ctx *ctx, const qs_atom *atoms, int natom, const qs_po
  * doSTEM changes dE_E between frozen-phonon runs (stem3.cpp:2750). */
 int qs_set_optics(qs_ctx *ctx, const qs_params *p);
 
-/* Forms the probe at the window centre (the only position the driver uses,
- * stem3.cpp:2838); keeps it on the device; copies it to out (nx*ny complex64) when non-NULL. */
+/* Forms the probe at the window centre, (nx/2)*resolutionX and (ny/2)*resolutionY in
+ * float as doSTEM passes them (stem3.cpp:2838); keeps it on the device; copies it to
+ * out (nx*ny complex64) when non-NULL. */
 int qs_make_probe(qs_ctx *ctx, float *out);
+/* qs_make_probe with the probe centred at (x, y) Angstrom from the window's corner:
+ * doCBED places it at scan_x_start - potOffsetX plus the source-size offset
+ * (stem3.cpp:2093-2097). */
+int qs_make_probe_at(qs_ctx *ctx, double x, double y, float *out);
 
 /* Bin bookkeeping of collectIntensity: for a slab of `slices` slices starting at
  * absolute slice totalSliceCount, bin_of_slice[s] receives the output-thickness
  * bin that slice s leaves as its final value (or -1).  Returns the number of bins
  * (tCount+1), or -1 on bad arguments. */
 int qs_record_bins(int slices, int cellDiv, int outputInterval, int totalSliceCount, int *bin_of_slice);
+/* qs_record_bins for a slab run mulsRepeat1 times ("sequence: r p", runMulsSTEM's
+ * mRepeat loop): bin_of_step has slices*mulsRepeat1 entries, step k = repeat k/slices,
+ * slice k%slices, recorded under the reference's slice index totalSliceCount +
+ * (k%slices)*(1 + k/slices) (stemlib.cpp:2429).  Returns tCount+1, or -1. */
+int qs_record_bins_repeat(int slices, int cellDiv, int outputInterval, int totalSliceCount, int mulsRepeat1,
+                          int *bin_of_step);
 
 /* Incident wave [nx][ny] complex64 (host) in place of the STEM probe, e.g. the
  * plane wave psi = 1 of TEM/CBED-style runs (the reference's doTEM initialises
@@ -204,6 +218,13 @@ int qs_select_config(qs_ctx *ctx, int config);
  * config NULL means configuration 0 for every item. */
 int qs_run_stem_cfg(qs_ctx *ctx, const int *ipos, const int *config, int nprobe, int start, const int *bin_of_slice,
                     int nslice, int nbins, float *det, float *diff, float *exit_wave, float *int_intensity);
+/* qs_run_stem_cfg over transmission slices first_slice .. first_slice+nslice-1
+ * (bin_of_slice[s] belongs to slice first_slice+s): a slab run in pieces, e.g. up to
+ * each output thickness of TEM and CBED runs, which write the wave or diffraction
+ * pattern there (interimWave, collectIntensity, stemlib.cpp:2476-2482, 2663-2676). */
+int qs_run_slices(qs_ctx *ctx, const int *ipos, const int *config, int nprobe, int start, int first_slice,
+                  const int *bin_of_slice, int nslice, int nbins, float *det, float *diff, float *exit_wave,
+                  float *int_intensity);
 
 /* Use src's transmission functions (all configurations) instead of dst's own:
  * contexts of different window sizes over one potential grid (same potNx/potNy,

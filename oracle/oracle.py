@@ -140,10 +140,14 @@ def wavelength(kev):
     return load().qo_wavelength(kev)
 
 
-def probe(cfg):
+def probe(cfg, at=None):
+    """probe() at the window centre, or centred at `at` = (x, y) Angstrom (qo_probe_at)."""
     out = np.empty((cfg.nx, cfg.ny), dtype=np.complex128)
     p = params(cfg)
-    load().qo_probe(ctypes.byref(p), _p(out))
+    if at is None:
+        load().qo_probe(ctypes.byref(p), _p(out))
+    else:
+        load().qo_probe_at(ctypes.byref(p), ctypes.c_double(at[0]), ctypes.c_double(at[1]), _p(out))
     return out
 
 

@@ -168,13 +168,25 @@ double qo_seval(const double *x, const double *y, const double *b, const double 
  * normalised real-space probe [nx][ny] with sum |psi|^2 = nx*ny. */
 void qo_probe(const qo_params *p, double complex *psi) { qo_probe_d(p, p->df0, p->Cs, psi); }
 
+static void probe_impl(const qo_params *p, double df0, double Cs, double px, double py, double complex *psi);
+
 /* probe() with defocus and Cs given in double precision (finite-difference checks). */
 void qo_probe_d(const qo_params *p, double df0, double Cs, double complex *psi) {
+  /* the driver's argument nx/2*resX, float-valued (stem3.cpp:2838) */
+  const float cx = (float)(p->nx / 2) * p->resolutionX, cy = (float)(p->ny / 2) * p->resolutionY;
+  probe_impl(p, df0, Cs, (double)cx, (double)cy, psi);
+}
+
+/* probe() centred at (px, py) Angstrom from the window corner (doCBED, stem3.cpp:2097). */
+void qo_probe_at(const qo_params *p, double px, double py, double complex *psi) {
+  probe_impl(p, p->df0, p->Cs, px, py, psi);
+}
+
+static void probe_impl(const qo_params *p, double df0, double Cs, double px, double py, double complex *psi) {
   const int nx = p->nx, ny = p->ny;
   const double ax = nx * (double)p->resolutionX, by = ny * (double)p->resolutionY;
-  /* stemlib.cpp:1975-1976: dx = ax - (nx/2)*resX with the argument float-valued */
-  const float cx = (float)(nx / 2) * p->resolutionX, cy = (float)(ny / 2) * p->resolutionY;
-  const double dx = ax - (double)cx, dy = by - (double)cy;
+  /* stemlib.cpp:1974-1975: dx = ax - dx */
+  const double dx = ax - px, dy = by - py;
   const double v0 = p->v0;
   const double wavlen = 12.26 / sqrt(v0 * 1.e3 + v0 * v0 * 0.9788); /* :2014 */
   const double rx = 1.0 / ax, ry = 1.0 / by, rx2 = rx * rx, ry2 = ry * ry;

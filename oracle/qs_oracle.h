@@ -68,6 +68,7 @@ double qo_seval(const double *x, const double *y, const double *b, const double 
 /* ---- hot path pieces ---- */
 void qo_probe(const qo_params *p, double complex *psi);
 void qo_probe_d(const qo_params *p, double df0, double Cs, double complex *psi);
+void qo_probe_at(const qo_params *p, double px, double py, double complex *psi);
 void qo_k_tables(const qo_params *p, float *kx2, float *ky2, float *k2max);
 void qo_det_k2(const qo_params *p, float *k2in, float *k2out);
 void qo_make_trans(const qo_params *p, const float *V, int nslice, double complex *trans);

@@ -81,15 +81,18 @@ SIGNATURES = {
     "qs_set_trans": (_I, [_P, _FP, _I]),
     "qs_get_trans": (_I, [_P, _FP]),
     "qs_make_probe": (_I, [_P, _FP]),
+    "qs_make_probe_at": (_I, [_P, ctypes.c_double, ctypes.c_double, _FP]),
     "qs_set_optics": (_I, [_P, ctypes.POINTER(QsParams)]),
     "qs_set_scattering_table": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "qs_set_ion_scattering_table": (_I, [_P, ctypes.POINTER(ctypes.c_double)]),
     "qs_build_potential": (_I, [_P, ctypes.POINTER(QsAtom), _I, ctypes.POINTER(QsPotParams), _FP]),
     "qs_record_bins": (_I, [_I, _I, _I, _I, _IP]),
+    "qs_record_bins_repeat": (_I, [_I, _I, _I, _I, _I, _IP]),
     "qs_set_incident_wave": (_I, [_P, _FP]),
     "qs_device_memory": (_I, [ctypes.POINTER(ctypes.c_longlong), ctypes.POINTER(ctypes.c_longlong), _I]),
     "qs_run_stem": (_I, [_P, _IP, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
     "qs_run_stem_cfg": (_I, [_P, _IP, _IP, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
+    "qs_run_slices": (_I, [_P, _IP, _IP, _I, _I, _I, _IP, _I, _I, _FP, _FP, _FP, _FP]),
     "qs_run_stem_noalloc": (_I, [_P, _IP, _I, _IP, _I, _I, _FP, _FP]),
     "qs_set_configs": (_I, [_P, _I]),
     "qs_select_config": (_I, [_P, _I]),

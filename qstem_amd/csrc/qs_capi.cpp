@@ -924,8 +924,9 @@ int qs_get_trans(qs_ctx* c, float* out) {
   return QS_OK;
 }
 
-int qs_make_probe(qs_ctx* c, float* out) {
+int qs_make_probe_at(qs_ctx* c, double x, double y, float* out) {
   if (!c) return QS_ERR_ARG;
+  if (!std::isfinite(x) || !std::isfinite(y)) return fail(c, QS_ERR_ARG, "qs_make_probe_at: position not finite");
   invalidate(c);  // kernel arguments baked into a captured graph may change
   const qs_params& p = c->p;
   const int nx = p.nx, ny = p.ny;
@@ -959,9 +960,8 @@ int qs_make_probe(qs_ctx* c, float* out) {
   a.on4 = p.a44 > 0 || p.a42 > 0 || p.Cs != 0;
   a.on5 = p.a55 > 0 || p.a53 > 0 || p.a51 > 0;
   a.on6 = p.a66 > 0 || p.a64 > 0 || p.C5 != 0 || (!(p.compat & QS_COMPAT_Q2) && p.a62 > 0);
-  const float cxf = (float)(nx / 2) * p.resolutionX, cyf = (float)(ny / 2) * p.resolutionY;
-  a.sx = a.ax - (double)cxf;
-  a.sy = a.by - (double)cyf;
+  a.sx = a.ax - x;  // probe(): dx = ax - dx (stemlib.cpp:1974-1975)
+  a.sy = a.by - y;
   a.ismoth = p.ismoth;
   a.gaussFlag = p.gaussFlag;
   a.gaussScale = p.gaussScale;
@@ -999,6 +999,13 @@ int qs_make_probe(qs_ctx* c, float* out) {
   return QS_OK;
 }
 
+int qs_make_probe(qs_ctx* c, float* out) {
+  if (!c) return QS_ERR_ARG;
+  // the driver's window centre, nx/2*resolutionX in float (stem3.cpp:2838)
+  const float cx = (float)(c->p.nx / 2) * c->p.resolutionX, cy = (float)(c->p.ny / 2) * c->p.resolutionY;
+  return qs_make_probe_at(c, (double)cx, (double)cy, out);
+}
+
 int qs_set_optics(qs_ctx* c, const qs_params* p) {
   if (!c || !p) return fail(c, QS_ERR_ARG, "qs_set_optics: bad arguments");
   invalidate(c);  // kernel arguments baked into a captured graph may change
@@ -1034,24 +1041,31 @@ int qs_device_memory(long long* current_bytes, long long* peak_bytes, int reset_
   return QS_OK;
 }
 
-int qs_record_bins(int slices, int cellDiv, int outputInterval, int totalSliceCount, int* bin_of_slice) {
-  if (slices < 1 || cellDiv < 1 || outputInterval < 0 || !bin_of_slice) return -1;
+int qs_record_bins_repeat(int slices, int cellDiv, int outputInterval, int totalSliceCount, int mulsRepeat1,
+                          int* bin_of_step) {
+  if (slices < 1 || cellDiv < 1 || outputInterval < 0 || mulsRepeat1 < 1 || !bin_of_step) return -1;
   const int total = slices * cellDiv;
   const int tCount = outputInterval > 0 ? total / outputInterval : 0;
   const int nb = tCount + 1;
   std::vector<int> last(nb, -1);
-  for (int s = 0; s < slices; s++) {
-    const int slice = totalSliceCount + s;
+  for (int k = 0; k < slices * mulsRepeat1; k++) {
+    // runMulsSTEM records step (repeat r, slice s) under slice totalSliceCount + s (1 + r)
+    const int r = k / slices, s = k % slices;
+    const long long slice = (long long)totalSliceCount + (long long)s * (1 + r);
     int t;
     if (outputInterval == 0) t = 0;
-    else if (slice < total - 1) t = slice / outputInterval;
+    else if (slice < total - 1) t = (int)(slice / outputInterval);
     else t = tCount;
-    bin_of_slice[s] = -1;
-    if (t >= 0 && t < nb) last[t] = s;
+    bin_of_step[k] = -1;
+    if (t >= 0 && t < nb) last[t] = k;
   }
   for (int t = 0; t < nb; t++)
-    if (last[t] >= 0) bin_of_slice[last[t]] = t;
+    if (last[t] >= 0) bin_of_step[last[t]] = t;
   return nb;
+}
+
+int qs_record_bins(int slices, int cellDiv, int outputInterval, int totalSliceCount, int* bin_of_slice) {
+  return qs_record_bins_repeat(slices, cellDiv, outputInterval, totalSliceCount, 1, bin_of_slice);
 }
 
 int qs_set_batch(qs_ctx* c, int batch) {
@@ -1158,6 +1172,7 @@ int qs_kernel_times(const qs_ctx* c, double* ms_rows, double* ms_cols, long* n_r
 struct RunPlan {
   qs_ctx* c = nullptr;
   int nprobe = 0, start = 0, nslice = 0, nbins = 0, B = 0;
+  int slice0 = 0;  // first transmission slice of the call
   const int* bins = nullptr;
   bool want_det = false, has_cfg = false;
   float *det = nullptr, *diff = nullptr, *exit_wave = nullptr, *intensity = nullptr;
@@ -1171,11 +1186,13 @@ struct RunPlan {
 
 // Argument checks shared by the run entry points.
 static int check_run(qs_ctx* c, const char* who, const int* ipos, const int* config, int nprobe, int start,
-                     const int* bin_of_slice, int nslice, int nbins, const float* det) {
+                     const int* bin_of_slice, int nslice, int nbins, const float* det, int slice0 = 0) {
   const qs_params& p = c->p;
   const std::string w(who);
   if (!ipos || nprobe < 1 || nslice < 1 || (start != 0 && start != 1)) return fail(c, QS_ERR_ARG, w + ": bad arguments");
-  if (nslice > trans_slices_of(c)) return fail(c, QS_ERR_STATE, w + ": fewer transmission slices than nslice");
+  if (slice0 < 0) return fail(c, QS_ERR_ARG, w + ": negative first slice");
+  if ((long long)slice0 + nslice > trans_slices_of(c))
+    return fail(c, QS_ERR_STATE, w + ": fewer transmission slices than first slice + nslice");
   if (start == 0 && !c->have_probe) return fail(c, QS_ERR_STATE, w + ": qs_make_probe not called");
   if (start == 1 && (!c->state_valid || c->state_probes != nprobe))
     return fail(c, QS_ERR_STATE, w + ": no wave state to continue for this probe count");
@@ -1248,15 +1265,16 @@ static void fill_sweep_args(qs_ctx* c, RunPlan& r, const int* d_ipos, const int*
 
 static int plan_run(qs_ctx* c, const char* who, const int* ipos, const int* config, int nprobe, int start,
                     const int* bin_of_slice, int nslice, int nbins, float* det, float* diff, float* exit_wave,
-                    float* int_intensity, RunPlan& r) {
+                    float* int_intensity, RunPlan& r, int slice0 = 0) {
   const qs_params& p = c->p;
-  const int rc = check_run(c, who, ipos, config, nprobe, start, bin_of_slice, nslice, nbins, det);
+  const int rc = check_run(c, who, ipos, config, nprobe, start, bin_of_slice, nslice, nbins, det, slice0);
   if (rc != QS_OK) return rc;
   const int nx = p.nx, ny = p.ny;
   r = RunPlan{};
   r.c = c;
   r.nprobe = nprobe;
   r.start = start;
+  r.slice0 = slice0;
   r.nslice = nslice;
   r.nbins = nbins;
   r.bins = bin_of_slice;
@@ -1328,7 +1346,7 @@ static hipError_t enqueue_slice(RunPlan& r, int p0, int s, hipStream_t st, float
   const int nb = std::min(r.B, r.nprobe - p0);
   a.nprobe = nb;
   a.probe0 = p0;
-  a.trans = trans_base(c) + (size_t)s * r.slice_elems;
+  a.trans = trans_base(c) + (size_t)(r.slice0 + s) * r.slice_elems;
   const bool last = (s == r.nslice - 1);
   hipError_t e = lm ? lm->mark(0, true, st) : hipSuccess;
   if (e != hipSuccess) return e;
@@ -1372,10 +1390,11 @@ static int finish_run(RunPlan& r, hipStream_t st) {
 // qs_run_stem for wave grids that are not powers of two: per slice and probe group,
 // transmit, forward 2D FFT (rows, transpose, rows), propagator and detector sums,
 // inverse 2D FFT, on one stream over a full-size state (qs_generic.hip).
-static int run_generic(qs_ctx* c, const int* ipos, const int* config, int nprobe, int start, const int* bins,
-                       int nslice, int nbins, float* det, float* diff, float* exit_wave, float* int_intensity) {
+static int run_generic(qs_ctx* c, const int* ipos, const int* config, int nprobe, int start, int slice0,
+                       const int* bins, int nslice, int nbins, float* det, float* diff, float* exit_wave,
+                       float* int_intensity) {
   const char* who = "qs_run_stem";
-  int rc = check_run(c, who, ipos, config, nprobe, start, bins, nslice, nbins, det);
+  int rc = check_run(c, who, ipos, config, nprobe, start, bins, nslice, nbins, det, slice0);
   if (rc != QS_OK) return rc;
   const qs_params& p = c->p;
   const int nx = p.nx, ny = p.ny;
@@ -1432,7 +1451,7 @@ static int run_generic(qs_ctx* c, const int* ipos, const int* config, int nprobe
       g.probe0 = p0;
       g.potNy = p.potNy;
       g.probe = c->probe.as<float2>();
-      g.trans = trans_base(c) + (size_t)s * slice;
+      g.trans = trans_base(c) + (size_t)(slice0 + s) * slice;
       g.ipos = c->ipos.as<int2>();
       g.cfg = config ? c->cfgidx.as<int>() : nullptr;
       g.cfg_stride = (long long)trans_slices_of(c) * (long long)slice;
@@ -1472,15 +1491,16 @@ static int run_generic(qs_ctx* c, const int* ipos, const int* config, int nprobe
   return QS_OK;
 }
 
-int qs_run_stem_cfg(qs_ctx* c, const int* ipos, const int* config, int nprobe, int start, const int* bin_of_slice,
-                    int nslice, int nbins, float* det, float* diff, float* exit_wave, float* int_intensity) {
+int qs_run_slices(qs_ctx* c, const int* ipos, const int* config, int nprobe, int start, int first_slice,
+                  const int* bin_of_slice, int nslice, int nbins, float* det, float* diff, float* exit_wave,
+                  float* int_intensity) {
   if (!c) return QS_ERR_ARG;
   if (c->generic)
-    return run_generic(c, ipos, config, nprobe, start, bin_of_slice, nslice, nbins, det, diff, exit_wave,
-                       int_intensity);
+    return run_generic(c, ipos, config, nprobe, start, first_slice, bin_of_slice, nslice, nbins, det, diff,
+                       exit_wave, int_intensity);
   RunPlan r;
   int rc = plan_run(c, "qs_run_stem", ipos, config, nprobe, start, bin_of_slice, nslice, nbins, det, diff, exit_wave,
-                    int_intensity, r);
+                    int_intensity, r, first_slice);
   if (rc != QS_OK) return rc;
   LaunchMarks lm{c};
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
@@ -1512,7 +1532,7 @@ int qs_run_stem_cfg(qs_ctx* c, const int* ipos, const int* config, int nprobe, i
   };
   if (c->use_graph && !c->profiling) {
     // the graph bakes in every kernel argument: key on the call shape and the buffers
-    std::vector<long long> key = {nprobe, r.B, nslice, start, nbins, r.want_det, diff != nullptr,
+    std::vector<long long> key = {nprobe, r.B, nslice, start, r.slice0, nbins, r.want_det, diff != nullptr,
                                   exit_wave != nullptr, int_intensity != nullptr, r.has_cfg,
                                   (long long)trans_base(c), (long long)c->state.p, (long long)c->det.p,
                                   (long long)c->diff.p, (long long)c->exitw.p, (long long)c->total.p,
@@ -1569,6 +1589,12 @@ int qs_run_stem_cfg(qs_ctx* c, const int* ipos, const int* config, int nprobe, i
     else { c->ms_cols += ms; c->n_cols++; }
   }
   return QS_OK;
+}
+
+int qs_run_stem_cfg(qs_ctx* c, const int* ipos, const int* config, int nprobe, int start, const int* bin_of_slice,
+                    int nslice, int nbins, float* det, float* diff, float* exit_wave, float* int_intensity) {
+  return qs_run_slices(c, ipos, config, nprobe, start, 0, bin_of_slice, nslice, nbins, det, diff, exit_wave,
+                       int_intensity);
 }
 
 int qs_run_stem(qs_ctx* c, const int* ipos, int nprobe, int start, const int* bin_of_slice, int nslice, int nbins,

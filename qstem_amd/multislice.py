@@ -16,7 +16,7 @@ reference's printf+exit(0).
 import ctypes
 import os
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+from typing import List, Optional, Sequence, Tuple
 
 import numpy as np
 
@@ -131,12 +131,16 @@ def _atom_array(atoms):
     return arr
 
 
-def record_bins(slices: int, cellDiv: int = 1, outputInterval: int = 0, totalSliceCount: int = 0):
-    """Bin written by each slice as its final value (collectIntensity, stemlib.cpp:2588-2607)."""
+def record_bins(slices: int, cellDiv: int = 1, outputInterval: int = 0, totalSliceCount: int = 0,
+                mulsRepeat1: int = 1):
+    """Bin written by each slice step as its final value (collectIntensity,
+    stemlib.cpp:2588-2607); with mulsRepeat1 > 1 ("sequence: r p") the slab's
+    slices*mulsRepeat1 steps, recorded under runMulsSTEM's slice index
+    totalSliceCount + s*(1 + r) (stemlib.cpp:2429)."""
     lib = _lib.load()
-    out = np.empty(slices, dtype=np.int32)
-    nb = lib.qs_record_bins(slices, cellDiv, outputInterval if outputInterval > 0 else slices,
-                            totalSliceCount, _iptr(out))
+    out = np.empty(slices * mulsRepeat1, dtype=np.int32)
+    nb = lib.qs_record_bins_repeat(slices, cellDiv, outputInterval if outputInterval > 0 else slices,
+                                   totalSliceCount, mulsRepeat1, _iptr(out))
     if nb < 0:
         raise QstemError("bad bin arguments")
     return out, nb
@@ -255,9 +259,15 @@ class Multislice:
         return V
 
     # -- probe ---------------------------------------------------------------------
-    def make_probe(self) -> np.ndarray:
+    def make_probe(self, at: Optional[Tuple[float, float]] = None) -> np.ndarray:
+        """The probe at the window centre, or centred at `at` = (x, y) Angstrom from
+        the window's corner (qs_make_probe_at)."""
         out = np.empty((self.cfg.nx, self.cfg.ny), dtype=np.complex64)
-        self._check(self._lib.qs_make_probe(self._h, _fptr(out.view(np.float32))), "qs_make_probe")
+        if at is None:
+            self._check(self._lib.qs_make_probe(self._h, _fptr(out.view(np.float32))), "qs_make_probe")
+        else:
+            self._check(self._lib.qs_make_probe_at(self._h, float(at[0]), float(at[1]), _fptr(out.view(np.float32))),
+                        "qs_make_probe_at")
         return out
 
     def set_incident_wave(self, psi: np.ndarray):
@@ -396,16 +406,17 @@ class Multislice:
     def run_stem(self, ipos: np.ndarray, nslice: Optional[int] = None, start: int = 0,
                  bins: Optional[Sequence[int]] = None, nbins: int = 0, det: Optional[np.ndarray] = None,
                  want_det: bool = True, want_diff: bool = False, want_exit: bool = False,
-                 want_intensity: bool = False, config: Optional[np.ndarray] = None) -> dict:
-        """Advance len(ipos) probes through nslice slices (item i on phonon
-        configuration config[i] when given).  Returns a dict with
-        'det' [nbins][ndet][nprobe], 'diff', 'exit', 'intensity' as requested."""
+                 want_intensity: bool = False, config: Optional[np.ndarray] = None, first_slice: int = 0) -> dict:
+        """Advance len(ipos) probes through transmission slices first_slice ..
+        first_slice + nslice - 1 (item i on phonon configuration config[i] when
+        given).  Returns a dict with 'det' [nbins][ndet][nprobe], 'diff', 'exit',
+        'intensity' as requested."""
         ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
         n = ipos.shape[0]
         if config is not None:
             config = np.ascontiguousarray(config, dtype=np.int32).reshape(-1)
             assert config.shape[0] == n
-        nslice = self._nslice if nslice is None else nslice
+        nslice = self._nslice - first_slice if nslice is None else nslice
         if bins is None:
             b, nb = record_bins(nslice)
             bins, nbins = b, nb
@@ -422,9 +433,9 @@ class Multislice:
         diff = np.empty((n, self.cfg.nx, self.cfg.ny), dtype=np.float32) if want_diff else None
         exitw = np.empty((n, self.cfg.nx, self.cfg.ny), dtype=np.complex64) if want_exit else None
         inten = np.empty(n, dtype=np.float32) if want_intensity else None
-        rc = self._lib.qs_run_stem_cfg(self._h, _iptr(ipos), _iptr(config), n, int(start), _iptr(bins), int(nslice),
-                                       int(nbins), _fptr(det), _fptr(diff),
-                                       _fptr(exitw.view(np.float32) if exitw is not None else None), _fptr(inten))
+        rc = self._lib.qs_run_slices(self._h, _iptr(ipos), _iptr(config), n, int(start), int(first_slice), _iptr(bins),
+                                     int(nslice), int(nbins), _fptr(det), _fptr(diff),
+                                     _fptr(exitw.view(np.float32) if exitw is not None else None), _fptr(inten))
         self._check(rc, "qs_run_stem")
         if diff is not None:
             out["diff"] = diff

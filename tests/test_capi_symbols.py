@@ -52,3 +52,18 @@ def test_create_without_device_fails_cleanly():
     from cases import small_config
     with pytest.raises(QstemError):
         Multislice(small_config())
+
+
+def test_record_bins_repeat_matches_oracle(oracle):
+    """qs_record_bins_repeat (host logic, no device) against the oracle's qo_record_bins
+    over slab counts, offsets, intervals and repeat counts, including slab offsets past
+    the last bin and intervals that do not divide the slice count."""
+    from qstem_amd import record_bins
+    for slices in (1, 5, 6, 40):
+        for cellDiv in (1, 3):
+            for interval in (1, 2, 7, slices):
+                for reps in (1, 2, 3):
+                    for tsc in (0, slices, 2 * slices * cellDiv):
+                        got, nb = record_bins(slices, cellDiv, interval, tsc, mulsRepeat1=reps)
+                        want, nbo = oracle.record_bins(slices, cellDiv, interval, tsc, mulsRepeat1=reps)
+                        assert nb == nbo and list(got) == list(want), (slices, cellDiv, interval, reps, tsc)

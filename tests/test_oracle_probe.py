@@ -57,3 +57,19 @@ def test_chi_closed_form(oracle):
     ratio = Psi[inside] / (np.exp(-1j * chi[inside]) * shift[inside])
     assert np.allclose(ratio, ratio[0], rtol=1e-9, atol=1e-9)
     assert np.all(np.abs(Psi[~inside]) < 1e-9 * np.abs(Psi).max())
+
+
+def test_probe_at_position_is_a_shift(oracle):
+    """qo_probe_at: moving the centre by whole pixels (dx, dy) is a pure phase ramp in k,
+    so the real-space probe is the centred one rolled by (dx, dy) pixels; the centre
+    itself reproduces qo_probe.  A wrong sign of the shift term fails the roll."""
+    c = cfg(Cs=2e4, df0=-40.0)
+    psi0 = oracle.probe(c)
+    cx = float(np.float32(32) * np.float32(0.1))
+    assert np.array_equal(oracle.probe(c, at=(cx, cx)), psi0)
+    dx = float(np.float32(0.1))  # the pixel size as the config stores it
+    shifted = oracle.probe(c, at=(cx + 5 * dx, cx - 3 * dx))
+    assert np.abs(shifted - np.roll(psi0, (5, -3), axis=(0, 1))).max() < 1e-9
+    ix, iy = np.unravel_index(np.argmax(np.abs(shifted)), shifted.shape)
+    i0, j0 = np.unravel_index(np.argmax(np.abs(psi0)), psi0.shape)
+    assert (ix - i0, iy - j0) == (5, -3)
