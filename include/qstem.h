@@ -84,8 +84,10 @@ void qs_destroy(qs_ctx *ctx);
 /* Last error message of ctx, or of the calling thread's last failed qs_create when ctx is NULL. */
 const char *qs_last_error(const qs_ctx *ctx);
 
-/* Projected potential V[nslice][potNx][potNy] (host, float) -> transmission
- * functions exp(i*gamma*lambda*V), band limited when bandlimittrans is set. */
+/* Projected potential V[nslice][potNx][potNy] (float, host memory or device memory of
+ * the context's device) -> transmission functions exp(i*gamma*lambda*V), band limited
+ * when bandlimittrans is set.  A device V must be complete when the call is made (the
+ * call does not wait for other streams). */
 int qs_set_potential(qs_ctx *ctx, const float *V, int nslice);
 /* Transmission functions directly, complex64 [nslice][potNx][potNy] (interleaved re,im). */
 int qs_set_trans(qs_ctx *ctx, const float *trans, int nslice);
@@ -283,6 +285,9 @@ int qs_adjoint(qs_ctx *ctx, const int *ipos, int nprobe, const float *target, in
  * allowed).  qs_last_run_ms reports each half's device time.  No reference
  * counterpart. */
 int qs_forward_save(qs_ctx *ctx, const int *ipos, int nprobe, const float *target, int ckpt_every, float *loss);
+/* (target of qs_forward_save and qs_adjoint, and grad_V of qs_adjoint_saved and
+ * qs_adjoint, may be host memory or device memory of the context's device; both calls
+ * have finished with them when they return.) */
 /* Checkpoint interval of the last qs_adjoint / qs_forward_save (0 before the first).
  * No reference counterpart. */
 int qs_last_ckpt_interval(const qs_ctx *ctx);

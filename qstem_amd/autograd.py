@@ -3,9 +3,10 @@
     loss = MultisliceLoss.apply(V, df0, Cs, ms, ipos, target)
     loss.backward()   # V.grad, df0.grad, Cs.grad
 
-V is the projected potential [nslice][potNx][potNy] (any device; it is
-moved through host memory to the library, which owns its device buffers),
-df0 and Cs are 0-d tensors in Angstrom.  The forward pass runs the forward
+V is the projected potential [nslice][potNx][potNy]; a V in GPU memory is read
+by the library in place and dL/dV is written into a tensor on V's device (no
+copy through host memory); a CPU V goes through host copies.  df0 and Cs are
+0-d tensors in Angstrom.  The forward pass runs the forward
 half of the adjoint (qs_forward_save: the loss, checkpoints kept by the
 context); the backward pass runs the reverse half (qs_adjoint_saved) for the
 inputs that need gradients.  Between the two, the context may run other
@@ -26,9 +27,11 @@ class MultisliceLoss(torch.autograd.Function):
             cfg.df0 = float(df0)
             cfg.Cs = float(Cs)
             ms._check(ms._lib.qs_set_optics(ms._h, cfg.to_c()), "qs_set_optics")
-        ms.set_potential(V.detach().to("cpu", torch.float32).numpy())
+        ms.set_potential(V.detach() if V.is_cuda else V.detach().to(torch.float32).numpy())
         ms.make_probe()
-        tgt = target.detach().to("cpu", torch.float32).numpy() if torch.is_tensor(target) else target
+        tgt = target
+        if torch.is_tensor(target) and not target.is_cuda:
+            tgt = target.detach().to(torch.float32).numpy()
         loss = ms.forward_save(np.asarray(ipos), tgt)
         ctx.ms = ms
         ctx.v_device, ctx.v_dtype = V.device, V.dtype
@@ -37,11 +40,14 @@ class MultisliceLoss(torch.autograd.Function):
     @staticmethod
     def backward(ctx, grad_out):
         need_v, need_df, need_cs = ctx.needs_input_grad[:3]
-        out = ctx.ms.adjoint_saved(want_grad_V=need_v, want_grad_probe=need_df or need_cs)
+        on_device = ctx.v_device.type == "cuda"
+        out = ctx.ms.adjoint_saved(want_grad_V=need_v, want_grad_probe=need_df or need_cs,
+                                   device=ctx.v_device if on_device else None)
         go = grad_out.detach().to("cpu", torch.float64)
         gV = gdf = gcs = None
         if need_v:
-            gV = torch.from_numpy(out["grad_V"]).to(ctx.v_device, ctx.v_dtype) * go.to(ctx.v_device, ctx.v_dtype)
+            g = out["grad_V"] if on_device else torch.from_numpy(out["grad_V"])
+            gV = g.to(ctx.v_device, ctx.v_dtype) * grad_out.detach().to(ctx.v_device, ctx.v_dtype)
         if need_df:
             gdf = torch.tensor(out["grad_df0"], dtype=torch.float64) * go
         if need_cs:

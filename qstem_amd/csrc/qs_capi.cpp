@@ -163,6 +163,13 @@ struct qs_ctx {
   bool generic = false;
   RowXform wx, wy;
   DevBuf gscratch;
+  // generic sizes whose prime factors are 2, 3, 5 and 7: the fused mixed-radix path
+  // (k_gen_rows / k_gen_cols): radix plans, column tiles holding in-band ky columns,
+  // rows per block of the row kernel
+  bool mixed = false;
+  qs::GenPlan gpx{}, gpy{};
+  DevBuf gtiles;
+  int gntiles = 0, gtc = 0, grpb = 0;
   // band limit of the transmission functions as band-compact sweeps (potNx and
   // potNy powers of two): row FFT into the ky band, column FFT / disk mask with
   // 1/(potNx potNy) / column IFFT, row IFFT; else row transforms and transposes
@@ -434,6 +441,22 @@ int qs_create(const qs_params* p, qs_ctx** out) {
     int rc = setup_rowxform(c, c->wx, nx);
     if (rc == QS_OK) rc = setup_rowxform(c, c->wy, ny);
     if (rc != QS_OK) return bad(rc);
+    c->mixed = qs::gen_plan(nx, &c->gpx) && qs::gen_plan(ny, &c->gpy);
+    if (c->mixed) {
+      c->gtc = qs::gen_cols_per_tile(nx);
+      c->grpb = qs::gen_rows_per_block(ny);
+      std::vector<int> tiles;  // tiles of gtc columns with a ky column inside the band
+      for (int y0 = 0; y0 < ny; y0 += c->gtc) {
+        bool any = false;
+        for (int y = y0; y < std::min(ny, y0 + c->gtc); y++) any = any || c->ky2[y] < c->k2max;
+        if (any) tiles.push_back(y0);
+      }
+      c->gntiles = (int)tiles.size();
+      if (c->gntiles == 0 || upload(c->gtiles, tiles, c->stream) != hipSuccess) {
+        c->err = "device allocation of the column tile list failed";
+        return bad(QS_ERR_HIP);
+      }
+    }
   }
   // transmission band-limit tables (stemlib.cpp:2219-2262), float arithmetic, Q1 optional
   if (p->bandlimittrans) {
@@ -516,7 +539,7 @@ void qs_destroy(qs_ctx* c) {
                     &c->gx, &c->target, &c->lossp, &c->gT, &c->gT2, &c->gprobe, &c->gV, &c->dsum,
                     &c->blpx, &c->blpy, &c->blstate, &c->blstate2, &c->adj_ipos,
                     &c->wx.tw, &c->wx.chirp_f, &c->wx.vhat_f, &c->wx.chirp_b, &c->wx.vhat_b,
-                    &c->wy.tw, &c->wy.chirp_f, &c->wy.vhat_f, &c->wy.chirp_b, &c->wy.vhat_b, &c->gscratch};
+                    &c->wy.tw, &c->wy.chirp_f, &c->wy.vhat_f, &c->wy.chirp_b, &c->wy.vhat_b, &c->gscratch, &c->gtiles};
   for (DevBuf* b : bufs) b->release();
   for (DevBuf& b : c->luts) b.release();
   for (DevBuf& b : c->oluts) b.release();
@@ -660,7 +683,8 @@ int qs_set_potential(qs_ctx* c, const float* V, int nslice) {
   const qs_params& p = c->p;
   const size_t n = (size_t)nslice * p.potNx * p.potNy;
   HIPCHK(c, c->V.ensure(sizeof(float) * n));
-  HIPCHK(c, hipMemcpyAsync(c->V.p, V, sizeof(float) * n, hipMemcpyHostToDevice, c->stream));
+  // host or device memory (hipMemcpyDefault: a torch tensor's storage on this device)
+  HIPCHK(c, hipMemcpyAsync(c->V.p, V, sizeof(float) * n, hipMemcpyDefault, c->stream));
   c->built.valid = false;  // the potential no longer comes from the last qs_build_potential
   const int rc = build_trans(c, nslice);
   c->v_valid = rc == QS_OK;
@@ -1387,12 +1411,129 @@ static int finish_run(RunPlan& r, hipStream_t st) {
   return QS_OK;
 }
 
-// qs_run_stem for wave grids that are not powers of two: per slice and probe group,
-// transmit, forward 2D FFT (rows, transpose, rows), propagator and detector sums,
-// inverse 2D FFT, on one stream over a full-size state (qs_generic.hip).
+// qs_run_stem for wave grids whose sizes factor into 2, 3, 5 and 7 (not both powers
+// of two): per probe group and slice, k_gen_rows (y-IFFT of slice s-1, transmit,
+// y-FFT) and k_gen_cols (x-FFT, propagator, detector sums, x-IFFT) over the
+// y-spectrum state [probe][x][ky] (in-band columns only); the exit wave by a final
+// y-IFFT (qs_generic.hip).
+static int run_mixed(qs_ctx* c, const int* ipos, const int* config, int nprobe, int start, int slice0,
+                     const int* bins, int nslice, int nbins, float* det, float* diff, float* exit_wave,
+                     float* int_intensity) {
+  int rc = check_run(c, "qs_run_stem", ipos, config, nprobe, start, bins, nslice, nbins, det, slice0);
+  if (rc != QS_OK) return rc;
+  const qs_params& p = c->p;
+  const int nx = p.nx, ny = p.ny;
+  const size_t n2 = (size_t)nx * ny, slice = (size_t)p.potNx * p.potNy;
+  // probe group: its state at most 160 MB, resident in the Infinity Cache across the two kernels
+  const int B = c->batch > 0 ? std::min(c->batch, nprobe)
+                             : (int)std::max<size_t>(1, std::min<size_t>(nprobe, (size_t(160) << 20) / (8 * n2)));
+  const bool want_det = det && p.ndet > 0;
+  hipStream_t st = c->stream;
+  HIPCHK(c, c->state.ensure(sizeof(float2) * n2 * nprobe));
+  HIPCHK(c, c->ipos.ensure(sizeof(int) * 2 * nprobe));
+  HIPCHK(c, hipMemcpyAsync(c->ipos.p, ipos, sizeof(int) * 2 * nprobe, hipMemcpyHostToDevice, st));
+  if (config) {
+    HIPCHK(c, c->cfgidx.ensure(sizeof(int) * nprobe));
+    HIPCHK(c, hipMemcpyAsync(c->cfgidx.p, config, sizeof(int) * nprobe, hipMemcpyHostToDevice, st));
+  }
+  HIPCHK(c, c->partial.ensure(sizeof(float) * (size_t)B * c->gntiles * (qs::kMaxDet + 1)));
+  const size_t nd = want_det ? (size_t)nbins * p.ndet * nprobe : 0;
+  if (want_det) {
+    HIPCHK(c, c->det.ensure(sizeof(float) * nd));
+    HIPCHK(c, hipMemcpyAsync(c->det.p, det, sizeof(float) * nd, hipMemcpyHostToDevice, st));
+  }
+  if (diff) {  // columns outside the band are never visited: their pattern is zero
+    HIPCHK(c, c->diff.ensure(sizeof(float) * n2 * nprobe));
+    HIPCHK(c, hipMemsetAsync(c->diff.p, 0, sizeof(float) * n2 * nprobe, st));
+  }
+  if (exit_wave) HIPCHK(c, c->exitw.ensure(sizeof(float2) * n2 * nprobe));
+  if (int_intensity) HIPCHK(c, c->total.ensure(sizeof(float) * nprobe));
+  HIPCHK(c, hipEventRecord(c->ev0, st));
+  qs::GenRowArgs ra{};
+  ra.nx = nx;
+  ra.ny = ny;
+  ra.potNy = p.potNy;
+  ra.rpb = c->grpb;
+  ra.probe = c->probe.as<float2>();
+  ra.ipos = c->ipos.as<int2>();
+  ra.cfg = config ? c->cfgidx.as<int>() : nullptr;
+  ra.cfg_stride = (long long)trans_slices_of(c) * (long long)slice;
+  ra.state = c->state.as<float2>();
+  ra.exitw = c->exitw.as<float2>();
+  ra.ky2 = c->dky2.as<float>();
+  ra.k2max = c->k2max;
+  ra.plan = c->gpy;
+  ra.tw = c->twy.as<float2>();
+  qs::GenColArgs ca{};
+  ca.nx = nx;
+  ca.ny = ny;
+  ca.tc = c->gtc;
+  ca.state = c->state.as<float2>();
+  ca.tiles = c->gtiles.as<int>();
+  ca.px = c->px.as<float2>();
+  ca.py = c->py.as<float2>();
+  ca.kx2 = c->dkx2.as<float>();
+  ca.ky2 = c->dky2.as<float>();
+  ca.k2max = c->k2max;
+  ca.ndet = p.ndet;
+  for (int d = 0; d < p.ndet; d++) {  // shifts reduced to [0, nx) and [0, ny)
+    ca.det_sx[d] = ((int)p.det_shiftx[d] % nx + nx) % nx;
+    ca.det_sy[d] = ((int)p.det_shifty[d] % ny + ny) % ny;
+    ca.det_k2in[d] = c->det_k2in[d];
+    ca.det_k2out[d] = c->det_k2out[d];
+  }
+  ca.partial = c->partial.as<float>();
+  ca.diff_scale = (float)pow((double)n2, 1.5);
+  ca.plan = c->gpx;
+  ca.tw = c->twx.as<float2>();
+  for (int p0 = 0; p0 < nprobe; p0 += B) {
+    const int nb = std::min(B, nprobe - p0);
+    ra.nprobe = ca.nprobe = nb;
+    ra.probe0 = ca.probe0 = p0;
+    for (int s = 0; s < nslice; s++) {
+      ra.mode = (s == 0 && start == 0) ? 0 : 1;
+      ra.trans = trans_base(c) + (size_t)(slice0 + s) * slice;
+      qs::launch_gen_rows(ra, st);
+      const int bin = want_det && bins ? bins[s] : -1;
+      const bool last = s == nslice - 1, tot = last && int_intensity;
+      ca.rec = (bin >= 0 || tot) ? 1 : 0;
+      ca.diff = (last && diff) ? c->diff.as<float>() : nullptr;
+      qs::launch_gen_cols(ca, c->gntiles, st);
+      if (ca.rec)
+        qs::launch_reduce_partials(ca.partial, nb, c->gntiles, bin >= 0 ? p.ndet : 0, p.electronScale,
+                                   bin >= 0 ? c->det.as<float>() + (size_t)bin * p.ndet * nprobe : nullptr, nprobe,
+                                   tot ? c->total.as<float>() : nullptr, p0, st);
+    }
+    if (exit_wave) {  // the real-space exit waves: y-IFFT of the state, no transmit
+      ra.mode = 2;
+      qs::launch_gen_rows(ra, st);
+    }
+  }
+  HIPCHK(c, hipGetLastError());
+  HIPCHK(c, hipEventRecord(c->ev1, st));
+  if (want_det) HIPCHK(c, hipMemcpyAsync(det, c->det.p, sizeof(float) * nd, hipMemcpyDeviceToHost, st));
+  if (diff) HIPCHK(c, hipMemcpyAsync(diff, c->diff.p, sizeof(float) * n2 * nprobe, hipMemcpyDeviceToHost, st));
+  if (exit_wave)
+    HIPCHK(c, hipMemcpyAsync(exit_wave, c->exitw.p, sizeof(float2) * n2 * nprobe, hipMemcpyDeviceToHost, st));
+  if (int_intensity)
+    HIPCHK(c, hipMemcpyAsync(int_intensity, c->total.p, sizeof(float) * nprobe, hipMemcpyDeviceToHost, st));
+  HIPCHK(c, hipStreamSynchronize(st));
+  HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
+  c->state_valid = true;
+  c->state_probes = nprobe;
+  return QS_OK;
+}
+
+// qs_run_stem for other wave grids that are not powers of two (a prime factor above
+// 7): per slice and probe group, transmit, forward 2D FFT (rows, transpose, rows),
+// propagator and detector sums, inverse 2D FFT, on one stream over a full-size state
+// (qs_generic.hip).
 static int run_generic(qs_ctx* c, const int* ipos, const int* config, int nprobe, int start, int slice0,
                        const int* bins, int nslice, int nbins, float* det, float* diff, float* exit_wave,
                        float* int_intensity) {
+  if (c->mixed)
+    return run_mixed(c, ipos, config, nprobe, start, slice0, bins, nslice, nbins, det, diff, exit_wave,
+                     int_intensity);
   const char* who = "qs_run_stem";
   int rc = check_run(c, who, ipos, config, nprobe, start, bins, nslice, nbins, det, slice0);
   if (rc != QS_OK) return rc;
@@ -1837,7 +1978,7 @@ static int adj_forward(qs_ctx* c, const char* who, const int* ipos, int nprobe, 
   HIPCHK(c, hipMemcpyAsync(c->adj_ipos.p, ipos, sizeof(int) * 2 * nprobe, hipMemcpyHostToDevice, c->stream));
   const size_t nt = (size_t)nprobe * p.nx * p.ny;
   HIPCHK(c, c->target.ensure(sizeof(float) * nt));
-  HIPCHK(c, hipMemcpyAsync(c->target.p, target, sizeof(float) * nt, hipMemcpyHostToDevice, c->stream));
+  HIPCHK(c, hipMemcpyAsync(c->target.p, target, sizeof(float) * nt, hipMemcpyDefault, c->stream));
   HIPCHK(c, c->partial.ensure(sizeof(float) * (size_t)nprobe * c->ntiles * (qs::kMaxDet + 1)));
   HIPCHK(c, c->lossp.ensure(sizeof(float) * (size_t)nprobe * c->ntiles));
   HIPCHK(c, c->dsum.ensure(sizeof(double) * 4));
@@ -1994,7 +2135,7 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
   double ds[4];
   HIPCHK(c, hipMemcpyAsync(ds, c->dsum.p, sizeof(double) * 4, hipMemcpyDeviceToHost, c->stream));
   if (grad_V)
-    HIPCHK(c, hipMemcpyAsync(grad_V, c->gV.p, sizeof(float) * slice_elems * S, hipMemcpyDeviceToHost, c->stream));
+    HIPCHK(c, hipMemcpyAsync(grad_V, c->gV.p, sizeof(float) * slice_elems * S, hipMemcpyDefault, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
   HIPCHK(c, hipEventElapsedTime(&c->last_ms, c->ev0, c->ev1));
   if (grad_probe) {

@@ -11,6 +11,8 @@
 // the fused sweeps.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "qs_fft.hpp"
 #include "qs_internal.hpp"
 
@@ -103,6 +105,390 @@ int gen_chunks(int ny) { return (ny + kGenRows - 1) / kGenRows; }
 
 void launch_gen_propagate(const GenPropArgs& a, int nprobe, hipStream_t st) {
   hipLaunchKernelGGL(k_gen_propagate, dim3((unsigned)gen_chunks(a.ny), (unsigned)nprobe), dim3(256), 0, st, a);
+}
+
+
+// ============================================================================
+// Mixed-radix path for wave grids whose sizes factor into 2, 3, 5 and 7 (400 =
+// 4*4*5*5, 384, 480, 600, ...): the fused form of the power-of-two sweeps without
+// their compile-time plans.  The state between slices is the y-spectrum
+// [probe][x][ky] of the wave, stored only for the ky columns inside the 2/3 band
+// (the others are zero after propagation and are neither read nor written):
+//   k_gen_rows (per x row):       y-IFFT (finishes slice s-1) -> transmit ->
+//                                 y-FFT (starts slice s); exit mode: y-IFFT only
+//   k_gen_cols (per column tile): x-FFT -> propagator, band limit, detector sums,
+//                                 diffraction pattern -> x-IFFT
+// Transforms run in LDS as Stockham passes with a runtime radix list and a
+// twiddle table W_n^m = exp(-2 pi i m/n), m < n.
+
+namespace {
+
+template <int DIR>
+__device__ __forceinline__ float2 rot_i(float2 v) {  // DIR * i * v
+  return DIR < 0 ? make_float2(v.y, -v.x) : make_float2(-v.y, v.x);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft3(float2* a) {
+  const float h = 0.86602540378443865f;  // sin(2 pi/3)
+  const float2 b = cadd(a[1], a[2]), d = csub(a[1], a[2]);
+  const float2 t = make_float2(a[0].x - 0.5f * b.x, a[0].y - 0.5f * b.y);
+  const float2 u = rot_i<DIR>(cscale(d, h));
+  a[0] = cadd(a[0], b);
+  a[1] = cadd(t, u);
+  a[2] = csub(t, u);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft5(float2* a) {
+  const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;  // cos(2 pi/5), cos(4 pi/5)
+  const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;   // sin(2 pi/5), sin(4 pi/5)
+  const float2 b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
+  const float2 d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
+  const float2 a0 = a[0];
+  const float2 r1 = make_float2(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
+  const float2 r2 = make_float2(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
+  const float2 i1 = rot_i<DIR>(make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
+  const float2 i2 = rot_i<DIR>(make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
+  a[0] = cadd(a0, cadd(b1, b2));
+  a[1] = cadd(r1, i1);
+  a[4] = csub(r1, i1);
+  a[2] = cadd(r2, i2);
+  a[3] = csub(r2, i2);
+}
+
+template <int DIR>
+__device__ __forceinline__ void dft7(float2* a) {
+  const float c1 = 0.62348980185873353f, c2 = -0.22252093395631440f, c3 = -0.90096886790241913f;
+  const float s1 = 0.78183148246802981f, s2 = 0.97492791218182361f, s3 = 0.43388373911755812f;
+  const float2 b1 = cadd(a[1], a[6]), b2 = cadd(a[2], a[5]), b3 = cadd(a[3], a[4]);
+  const float2 d1 = csub(a[1], a[6]), d2 = csub(a[2], a[5]), d3 = csub(a[3], a[4]);
+  const float2 a0 = a[0];
+  // X_k = a0 + sum_j cos(2 pi jk/7) b_j +- DIR i sum_j sin(2 pi jk/7) d_j, k = 1, 2, 3
+  auto re = [&](float x1, float x2, float x3) {
+    return make_float2(a0.x + x1 * b1.x + x2 * b2.x + x3 * b3.x, a0.y + x1 * b1.y + x2 * b2.y + x3 * b3.y);
+  };
+  auto im = [&](float y1, float y2, float y3) {
+    return rot_i<DIR>(make_float2(y1 * d1.x + y2 * d2.x + y3 * d3.x, y1 * d1.y + y2 * d2.y + y3 * d3.y));
+  };
+  const float2 r1 = re(c1, c2, c3), r2 = re(c2, c3, c1), r3 = re(c3, c1, c2);
+  const float2 i1 = im(s1, s2, s3), i2 = im(s2, -s3, -s1), i3 = im(s3, -s1, s2);
+  a[0] = cadd(a0, cadd(b1, cadd(b2, b3)));
+  a[1] = cadd(r1, i1);
+  a[6] = csub(r1, i1);
+  a[2] = cadd(r2, i2);
+  a[5] = csub(r2, i2);
+  a[3] = cadd(r3, i3);
+  a[4] = csub(r3, i3);
+}
+
+template <int R, int DIR>
+__device__ __forceinline__ void dft_any(float2* a) {
+  if constexpr (R == 3) dft3<DIR>(a);
+  else if constexpr (R == 5) dft5<DIR>(a);
+  else if constexpr (R == 7) dft7<DIR>(a);
+  else dft<R, DIR>(a);
+}
+
+// Largest LDS footprint of one block in float2 (V vectors of gen_ld(n) each), and
+// the block size.
+constexpr int kGenLds = 8064, kGenBlock = 512;  // 63 KB of LDS, 1 KB left for static arrays
+
+// LDS position of element i of a vector: one float2 of padding every 16, so that
+// the strided writes of the first passes (element j*R + k) spread over the banks.
+__device__ __forceinline__ int lp(int i) { return i + (i >> 4); }
+// Vector stride in LDS for length n (odd, so that adjacent vectors start on other banks).
+__host__ __device__ constexpr int gen_ld(int n) { return n + (n >> 4) + 1; }
+
+// q = a / d, r = a % d for 0 <= a < 2^24, d >= 1 and a/d < 2^21, with rd = 1/d: the
+// float quotient is then within one of the integer one (relative error below 2^-22),
+// and one correction step makes it exact.
+__device__ __forceinline__ void fdivmod(int a, int d, float rd, int& q, int& r) {
+  q = (int)((float)a * rd);
+  r = a - q * d;
+  if (r >= d) {
+    q++;
+    r -= d;
+  } else if (r < 0) {
+    q--;
+    r += d;
+  }
+}
+
+// One Stockham pass of radix R over V vectors of length n in LDS (element i of
+// vector v at x[v*ld + lp(i)]), all threads of the block; ns = product of the
+// earlier radices.  All reads precede all writes (in place, two barriers).
+template <int R, int DIR>
+__device__ __forceinline__ void gen_pass(float2* x, int V, int n, int ld, int ns, const float2* __restrict__ tw,
+                                         int tid) {
+  constexpr int MAXB = (kGenLds / R + kGenBlock - 1) / kGenBlock;
+  const int nb = n / R, total = V * nb, stride = n / (ns * R);
+  const float rnb = 1.0f / (float)nb, rns = 1.0f / (float)ns;
+  float2 a[MAXB][R];
+  int dsto[MAXB];
+#pragma unroll
+  for (int i = 0; i < MAXB; i++) {
+    const int b = tid + i * kGenBlock;
+    if (b < total) {
+      int v, j, q, jm;
+      fdivmod(b, nb, rnb, v, j);
+      fdivmod(j, ns, rns, q, jm);
+      const float2* src = x + v * ld;
+#pragma unroll
+      for (int k = 0; k < R; k++) a[i][k] = src[lp(j + k * nb)];
+      if (ns > 1) {
+#pragma unroll
+        for (int k = 1; k < R; k++) {
+          const float2 w = tw[k * jm * stride];
+          a[i][k] = DIR < 0 ? cmul(a[i][k], w) : cmulc(a[i][k], w);
+        }
+      }
+      dft_any<R, DIR>(a[i]);
+      dsto[i] = (v << 16) | (q * ns * R + jm);  // vector (< 16) and output element (< 2^16)
+    }
+  }
+  __syncthreads();
+#pragma unroll
+  for (int i = 0; i < MAXB; i++) {
+    const int b = tid + i * kGenBlock;
+    if (b < total) {
+      const int e = dsto[i] & 0xffff;
+      float2* dst = x + (dsto[i] >> 16) * ld;
+#pragma unroll
+      for (int k = 0; k < R; k++) dst[lp(e + k * ns)] = a[i][k];
+    }
+  }
+  __syncthreads();
+}
+
+template <int DIR>
+__device__ void gen_fft(float2* x, int V, int ld, const GenPlan& pl, const float2* __restrict__ tw) {
+  int ns = 1;
+  for (int ip = 0; ip < pl.np; ip++) {
+    const int R = pl.r[ip];
+    // the thread index is made opaque per pass so that no pass's index arithmetic
+    // is hoisted out of the loop (it would hold registers through all passes)
+    int tid = threadIdx.x;
+    asm volatile("" : "+v"(tid));
+    if (R == 2) gen_pass<2, DIR>(x, V, pl.n, ld, ns, tw, tid);
+    else if (R == 3) gen_pass<3, DIR>(x, V, pl.n, ld, ns, tw, tid);
+    else if (R == 4) gen_pass<4, DIR>(x, V, pl.n, ld, ns, tw, tid);
+    else if (R == 5) gen_pass<5, DIR>(x, V, pl.n, ld, ns, tw, tid);
+    else if (R == 7) gen_pass<7, DIR>(x, V, pl.n, ld, ns, tw, tid);
+    else gen_pass<8, DIR>(x, V, pl.n, ld, ns, tw, tid);
+    ns *= R;
+  }
+}
+
+// Elements per thread when a block holds kGenLds values: all global loads of a
+// phase are issued before the first LDS store (one batch of kGenPer per thread).
+constexpr int kGenPer = (kGenLds + kGenBlock - 1) / kGenBlock;
+
+}  // namespace
+
+// Rows: a.rpb rows (probe, x) per block (a power of two); thread t works on row
+// t / TPR of the block and its columns y = t % TPR + k TPR, TPR = kGenBlock / rpb.
+// mode 0: the probe is the input (first slice of a fresh run); 1: the state's
+// y-spectrum (y-IFFT first); 2: exit: y-IFFT of the state into a.exitw (real space),
+// no transmit.  The transmission values are loaded with the input, before the y-IFFT.
+__global__ __launch_bounds__(kGenBlock) void k_gen_rows(GenRowArgs a) {
+  extern __shared__ float2 gsm[];
+  const int ny = a.ny, ld = gen_ld(ny);
+  const int nrows = a.nprobe * a.nx;
+  const int r0 = blockIdx.x * a.rpb;
+  const int V = min(a.rpb, nrows - r0);
+  const int lt = 31 - __clz(kGenBlock / a.rpb);  // log2 TPR
+  const int vr = threadIdx.x >> lt, yt = threadIdx.x & ((1 << lt) - 1);
+  const bool rowok = vr < V;
+  int p = 0, x = 0;
+  if (rowok) fdivmod(r0 + vr, a.nx, 1.0f / (float)a.nx, p, x);
+  const int pg = a.probe0 + p;
+  const size_t rowe = ((size_t)pg * a.nx + x) * ny;  // state / exit row
+  float2* lrow = gsm + vr * ld;
+  const float2* trow = a.trans;
+  if (rowok && a.mode != 2) {  // transmit (stemlib.cpp:2880-2887): the window's row
+    const int2 ip = a.ipos[pg];
+    trow = a.trans + (a.cfg ? (long long)a.cfg[pg] * a.cfg_stride : 0LL) + (size_t)(x + ip.x) * a.potNy + ip.y;
+  }
+  float2 sv[kGenPer], tr[kGenPer];
+#pragma unroll
+  for (int k = 0; k < kGenPer; k++) {
+    const int y = yt + (k << lt);
+    sv[k] = tr[k] = make_float2(0.f, 0.f);
+    if (rowok && y < ny) {
+      if (a.mode == 0) sv[k] = a.probe[(size_t)x * ny + y];
+      else if (a.ky2[y] < a.k2max) sv[k] = a.state[rowe + y];
+      if (a.mode != 2) tr[k] = trow[y];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kGenPer; k++) {
+    const int y = yt + (k << lt);
+    if (rowok && y < ny) lrow[lp(y)] = sv[k];
+  }
+  __syncthreads();
+  if (a.mode != 0) gen_fft<+1>(gsm, V, ld, a.plan, a.tw);
+  if (a.mode == 2) {
+#pragma unroll
+    for (int k = 0; k < kGenPer; k++) {
+      const int y = yt + (k << lt);
+      if (rowok && y < ny) a.exitw[rowe + y] = lrow[lp(y)];
+    }
+    return;
+  }
+#pragma unroll
+  for (int k = 0; k < kGenPer; k++) {
+    const int y = yt + (k << lt);
+    if (rowok && y < ny) lrow[lp(y)] = cmul(lrow[lp(y)], tr[k]);
+  }
+  __syncthreads();
+  gen_fft<-1>(gsm, V, ld, a.plan, a.tw);
+#pragma unroll
+  for (int k = 0; k < kGenPer; k++) {
+    const int y = yt + (k << lt);
+    if (rowok && y < ny && a.ky2[y] < a.k2max) a.state[rowe + y] = lrow[lp(y)];
+  }
+}
+
+// Columns: a tile of a.tc adjacent ky columns (a power of two; a.tiles[blockIdx.x],
+// only tiles with a column inside the band) of probe blockIdx.y, held column by
+// column in LDS; thread t works on column t % tc and rows x = t / tc + k kGenBlock/tc.
+// x-FFT, then propagate_slow's propagator and band limit (stemlib.cpp:2831-2849),
+// collectIntensity's detector sums (:2623-2659) into partial[p][tile][kMaxDet + 1]
+// (slot kMaxDet: the total) when rec, the fftshifted pattern into diff when set, then
+// the x-IFFT.  Detector shifts arrive reduced to [0, nx) and [0, ny).
+__global__ __launch_bounds__(kGenBlock) void k_gen_cols(GenColArgs a) {
+  extern __shared__ float2 gsm[];
+  const int nx = a.nx, ny = a.ny, ld = gen_ld(nx);
+  const int p = blockIdx.y, tile = blockIdx.x;
+  const int y0 = a.tiles[tile];
+  const int V = min(a.tc, ny - y0);
+  const int lc = 31 - __clz(a.tc);  // log2 tc
+  const int c = threadIdx.x & (a.tc - 1), xt = threadIdx.x >> lc, xs = kGenBlock >> lc;
+  const int ky = y0 + c;
+  const bool colok = c < V;
+  const bool inband = colok && a.ky2[ky] < a.k2max;
+  float2* st = a.state + (size_t)(a.probe0 + p) * nx * ny + (colok ? ky : 0);
+  float2* lcol = gsm + c * ld;
+  float2 sv[kGenPer];
+#pragma unroll
+  for (int k = 0; k < kGenPer; k++) {
+    const int x = xt + k * xs;
+    sv[k] = make_float2(0.f, 0.f);
+    if (inband && x < nx) sv[k] = st[(size_t)x * ny];
+  }
+#pragma unroll
+  for (int k = 0; k < kGenPer; k++) {
+    const int x = xt + k * xs;
+    if (colok && x < nx) lcol[lp(x)] = sv[k];
+  }
+  __syncthreads();
+  gen_fft<-1>(gsm, V, ld, a.plan, a.tw);
+  float dsum[kMaxDet + 1];
+#pragma unroll
+  for (int d = 0; d <= kMaxDet; d++) dsum[d] = 0.f;
+  const float2 pyk = colok ? a.py[ky] : make_float2(0.f, 0.f);
+  const float ky2 = colok ? a.ky2[ky] : 0.f;
+  for (int k = 0; k < kGenPer; k++) {
+    const int kx = xt + k * xs;
+    if (!colok || kx >= nx) continue;
+    float2 v = lcol[lp(kx)];
+    const float kx2 = a.kx2[kx];
+    const bool in = kx2 < a.k2max && (kx2 + ky2) < a.k2max;
+    v = in ? cmul(cmul(v, pyk), a.px[kx]) : make_float2(0.f, 0.f);
+    lcol[lp(kx)] = v;
+    const float e = cnorm(v);
+    if (a.rec) {
+      dsum[kMaxDet] += e;
+      for (int d = 0; d < a.ndet; d++) {
+        int sx = kx - a.det_sx[d], sy = ky - a.det_sy[d];
+        sx += sx < 0 ? nx : 0;
+        sy += sy < 0 ? ny : 0;
+        const float k2 = a.kx2[sx] + a.ky2[sy];
+        dsum[d] += (k2 >= a.det_k2in[d] && k2 <= a.det_k2out[d]) ? e : 0.f;
+      }
+    }
+    if (a.diff) {
+      const int hx = kx + nx / 2, hy = ky + ny / 2;
+      a.diff[((size_t)(a.probe0 + p) * nx + (hx >= nx ? hx - nx : hx)) * ny + (hy >= ny ? hy - ny : hy)] =
+          e * a.diff_scale;
+    }
+  }
+  if (a.rec) {
+    __shared__ float red[kMaxDet + 1][kGenBlock / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int d = 0; d <= kMaxDet; d++) {
+      float s = dsum[d];
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0) red[d][wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x <= kMaxDet) {
+      float s = 0.f;
+      for (int w = 0; w < kGenBlock / 64; w++) s += red[threadIdx.x][w];
+      a.partial[((size_t)p * gridDim.x + tile) * (kMaxDet + 1) + threadIdx.x] = s;
+    }
+  }
+  __syncthreads();
+  gen_fft<+1>(gsm, V, ld, a.plan, a.tw);
+#pragma unroll
+  for (int k = 0; k < kGenPer; k++) {
+    const int x = xt + k * xs;
+    if (inband && x < nx) st[(size_t)x * ny] = lcol[lp(x)];
+  }
+}
+
+bool gen_plan(int n, GenPlan* pl) {
+  int m = n, e2 = 0;
+  pl->n = n;
+  pl->np = 0;
+  while (m % 2 == 0) {
+    m /= 2;
+    e2++;
+  }
+  auto push = [&](int r) {
+    if (pl->np >= kGenMaxPasses) return false;
+    pl->r[pl->np++] = r;
+    return true;
+  };
+  // powers of two: radix 8 and 4 passes, one radix 2 when the exponent is odd and < 3
+  while (e2 >= 3 && (e2 % 2 == 1 || e2 >= 6)) {
+    if (!push(8)) return false;
+    e2 -= 3;
+  }
+  while (e2 >= 2) {
+    if (!push(4)) return false;
+    e2 -= 2;
+  }
+  if (e2 == 1 && !push(2)) return false;
+  for (int r : {3, 5, 7})
+    while (m % r == 0) {
+      if (!push(r)) return false;
+      m /= r;
+    }
+  return m == 1 && n >= 2 && (long)gen_ld(n) <= kGenLds;
+}
+
+// largest power of two <= min(16, kGenLds / gen_ld(n)) (at least 1)
+static int gen_pow2_fit(int n) {
+  const int m = std::max(1, std::min(16, kGenLds / gen_ld(n)));
+  int r = 1;
+  while (2 * r <= m) r *= 2;
+  return r;
+}
+int gen_rows_per_block(int ny) { return gen_pow2_fit(ny); }
+int gen_cols_per_tile(int nx) { return gen_pow2_fit(nx); }
+
+void launch_gen_rows(const GenRowArgs& a, hipStream_t st) {
+  const int nrows = a.nprobe * a.nx;
+  const unsigned grid = (unsigned)((nrows + a.rpb - 1) / a.rpb);
+  hipLaunchKernelGGL(k_gen_rows, dim3(grid), dim3(kGenBlock), sizeof(float2) * a.rpb * gen_ld(a.ny), st, a);
+}
+
+void launch_gen_cols(const GenColArgs& a, int ntiles, hipStream_t st) {
+  hipLaunchKernelGGL(k_gen_cols, dim3((unsigned)ntiles, (unsigned)a.nprobe), dim3(kGenBlock),
+                     sizeof(float2) * a.tc * gen_ld(a.nx), st, a);
 }
 
 }  // namespace qs

@@ -154,6 +154,51 @@ void launch_gen_transmit(const GenArgs& a, bool first, hipStream_t st);
 int gen_chunks(int ny);
 void launch_gen_propagate(const GenPropArgs& a, int nprobe, hipStream_t st);
 
+// Mixed-radix fused path (sizes whose prime factors are 2, 3, 5, 7): the state is the
+// y-spectrum [probe][x][ky], stored for the ky columns inside the 2/3 band only.
+constexpr int kGenMaxPasses = 16;
+struct GenPlan {
+  int n, np;
+  int r[kGenMaxPasses];  // radices of the Stockham passes (2, 3, 4, 5, 7, 8), product n
+};
+struct GenRowArgs {
+  int nx, ny, nprobe, probe0, potNy, rpb, mode;
+  const float2* probe;  // [nx][ny] (mode 0)
+  const float2* trans;  // transmission slice [potNx][potNy] of configuration 0
+  const int2* ipos;     // window offset per global probe
+  const int* cfg;       // nullable: configuration per global probe
+  long long cfg_stride;
+  float2* state;        // [nprobe_total][nx][ny] y-spectrum, in-band columns
+  float2* exitw;        // mode 2: [nprobe_total][nx][ny] real-space waves
+  const float* ky2;
+  float k2max;
+  GenPlan plan;         // length ny
+  const float2* tw;     // W_ny^m, m < ny
+};
+struct GenColArgs {
+  int nx, ny, nprobe, probe0, tc;
+  float2* state;
+  const int* tiles;     // first column of each tile
+  const float2* px;     // propagator x factor (includes 1/(nx ny))
+  const float2* py;
+  const float* kx2;
+  const float* ky2;
+  float k2max;
+  int rec, ndet;
+  int det_sx[kMaxDet], det_sy[kMaxDet];
+  float det_k2in[kMaxDet], det_k2out[kMaxDet];
+  float* partial;       // [nprobe][ntiles][kMaxDet+1] when rec
+  float* diff;          // nullable: [nprobe_total][nx][ny]
+  float diff_scale;
+  GenPlan plan;         // length nx
+  const float2* tw;     // W_nx^m, m < nx
+};
+bool gen_plan(int n, GenPlan* pl);  // false: n has a prime factor above 7
+int gen_rows_per_block(int ny);
+int gen_cols_per_tile(int nx);
+void launch_gen_rows(const GenRowArgs& a, hipStream_t st);
+void launch_gen_cols(const GenColArgs& a, int ntiles, hipStream_t st);
+
 bool supported_size(int n);
 int col_tile(int nx);
 

@@ -116,6 +116,21 @@ def _iptr(a: Optional[np.ndarray]):
     return a.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
 
 
+def _is_device_tensor(x) -> bool:
+    """A torch tensor in GPU memory (the library takes its storage pointer)."""
+    return type(x).__module__.startswith("torch") and getattr(x, "is_cuda", False)
+
+
+def _device_f32(t):
+    """Contiguous float32 view of a device tensor and its pointer, after the work
+    queued on the tensor's current stream has finished (the library's own stream
+    does not wait for torch's)."""
+    import torch
+    t = t.detach().to(torch.float32).contiguous()
+    torch.cuda.current_stream(t.device).synchronize()
+    return t, ctypes.cast(t.data_ptr(), ctypes.POINTER(ctypes.c_float))
+
+
 def _atom_array(atoms):
     """(n, 7) x, y, z, dw, occ, q, Z -> ctypes array of struct qs_atom (filled through numpy)."""
     atoms = np.asarray(atoms, dtype=np.float64).reshape(-1, 7)
@@ -195,10 +210,16 @@ class Multislice:
         self.close()
 
     # -- potential / transmission ------------------------------------------------
-    def set_potential(self, V: np.ndarray):
-        V = np.ascontiguousarray(V, dtype=np.float32)
-        assert V.ndim == 3 and V.shape[1:] == (self.cfg.potNx, self.cfg.potNy), V.shape
-        self._check(self._lib.qs_set_potential(self._h, _fptr(V), V.shape[0]), "qs_set_potential")
+    def set_potential(self, V):
+        """Projected potential [nslice][potNx][potNy]: a numpy array, or a torch tensor in
+        GPU memory, which the library reads in place (no copy through the host)."""
+        if _is_device_tensor(V):
+            V, ptr = _device_f32(V)
+        else:
+            V = np.ascontiguousarray(V, dtype=np.float32)
+            ptr = _fptr(V)
+        assert V.ndim == 3 and tuple(V.shape[1:]) == (self.cfg.potNx, self.cfg.potNy), V.shape
+        self._check(self._lib.qs_set_potential(self._h, ptr, V.shape[0]), "qs_set_potential")
         self._nslice = V.shape[0]
 
     def set_trans(self, t: np.ndarray):
@@ -296,22 +317,35 @@ class Multislice:
             out["grad_df0"], out["grad_Cs"] = float(gp[0]), float(gp[1])
         return out
 
-    def forward_save(self, ipos: np.ndarray, target: np.ndarray, ckpt_every: int = 0) -> float:
+    def forward_save(self, ipos: np.ndarray, target, ckpt_every: int = 0) -> float:
         """Loss of qs_adjoint from its forward pass alone (qs_forward_save); the context
-        keeps the checkpoints for a later adjoint_saved."""
+        keeps the checkpoints for a later adjoint_saved.  target: numpy array or torch
+        tensor in GPU memory."""
         ipos = np.ascontiguousarray(ipos, dtype=np.int32).reshape(-1, 2)
-        target = np.ascontiguousarray(target, dtype=np.float32)
-        assert target.shape == (ipos.shape[0], self.cfg.nx, self.cfg.ny), target.shape
+        if _is_device_tensor(target):
+            target, tptr = _device_f32(target)
+        else:
+            target = np.ascontiguousarray(target, dtype=np.float32)
+            tptr = _fptr(target)
+        assert tuple(target.shape) == (ipos.shape[0], self.cfg.nx, self.cfg.ny), target.shape
         loss = np.zeros(1, np.float32)
-        self._check(self._lib.qs_forward_save(self._h, _iptr(ipos), ipos.shape[0], _fptr(target), int(ckpt_every),
+        self._check(self._lib.qs_forward_save(self._h, _iptr(ipos), ipos.shape[0], tptr, int(ckpt_every),
                                               _fptr(loss)), "qs_forward_save")
         return float(loss[0])
 
-    def adjoint_saved(self, want_grad_V: bool = True, want_grad_probe: bool = True) -> dict:
-        """Gradients from the forward pass kept by the last forward_save (qs_adjoint_saved)."""
-        gV = np.empty((self._nslice, self.cfg.potNx, self.cfg.potNy), np.float32) if want_grad_V else None
+    def adjoint_saved(self, want_grad_V: bool = True, want_grad_probe: bool = True, device=None) -> dict:
+        """Gradients from the forward pass kept by the last forward_save (qs_adjoint_saved).
+        device (a torch CUDA device): grad_V is returned as a tensor there, written by the
+        library in place."""
         gp = np.zeros(2, np.float32) if want_grad_probe else None
-        self._check(self._lib.qs_adjoint_saved(self._h, _fptr(gV), _fptr(gp)), "qs_adjoint_saved")
+        if want_grad_V and device is not None:
+            import torch
+            gV = torch.empty((self._nslice, self.cfg.potNx, self.cfg.potNy), dtype=torch.float32, device=device)
+            gptr = ctypes.cast(gV.data_ptr(), ctypes.POINTER(ctypes.c_float))
+        else:
+            gV = np.empty((self._nslice, self.cfg.potNx, self.cfg.potNy), np.float32) if want_grad_V else None
+            gptr = _fptr(gV)
+        self._check(self._lib.qs_adjoint_saved(self._h, gptr, _fptr(gp)), "qs_adjoint_saved")
         out = {}
         if gV is not None:
             out["grad_V"] = gV

@@ -159,26 +159,31 @@ def test_atom_position_gradients_fd(oracle):
         assert g[ia, 2] == 0.0
 
 
-def test_torch_autograd_function(oracle):
-    """torch.autograd.Function wrapper: gradients equal the C-ABI adjoint's; a gradient step lowers the loss."""
+@pytest.mark.parametrize("device", ["cpu", "cuda"])
+def test_torch_autograd_function(oracle, device):
+    """torch.autograd.Function wrapper: gradients equal the C-ABI adjoint's; a gradient
+    step lowers the loss.  With V and the target in GPU memory the library reads them in
+    place and writes dL/dV into a tensor on that device."""
     import torch
     from qstem_amd import Multislice
     from qstem_amd.autograd import multislice_loss
     cfg = _cfg(1, slices=4)
     V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=3, vmax=20.0)
     ipos = probe_positions(cfg, 2, seed=1)
-    target = _target(oracle, cfg, V, ipos)
+    target_np = _target(oracle, cfg, V, ipos)
+    target = torch.tensor(target_np, device=device) if device == "cuda" else target_np
     with Multislice(cfg) as ms:
-        Vt = torch.tensor(V, requires_grad=True)
+        Vt = torch.tensor(V, requires_grad=True, device=device)
         df = torch.tensor(float(np.float32(cfg.df0)), dtype=torch.float64, requires_grad=True)
         cs = torch.tensor(float(np.float32(cfg.Cs)), dtype=torch.float64, requires_grad=True)
         loss = multislice_loss(Vt, df, cs, ms, ipos, target)
         loss.backward()
         ms.set_potential(V)
         ms.make_probe()
-        ref = ms.adjoint(ipos, target)
+        ref = ms.adjoint(ipos, target_np)
         assert abs(float(loss) - ref["loss"]) <= 1e-6 * ref["loss"]
-        assert np.allclose(Vt.grad.numpy(), ref["grad_V"], rtol=1e-5, atol=1e-6 * np.abs(ref["grad_V"]).max())
+        assert Vt.grad.device == Vt.device
+        assert np.allclose(Vt.grad.cpu().numpy(), ref["grad_V"], rtol=1e-5, atol=1e-6 * np.abs(ref["grad_V"]).max())
         assert abs(float(df.grad) - ref["grad_df0"]) <= 1e-5 * abs(ref["grad_df0"])
         with torch.no_grad():
             step = 0.1 * float(loss) / float((Vt.grad ** 2).sum())
@@ -186,9 +191,9 @@ def test_torch_autograd_function(oracle):
         loss2 = multislice_loss(V2, df.detach(), cs.detach(), ms, ipos, target)
         assert float(loss2) < float(loss)
         # only V needs a gradient: the reverse pass skips the probe gradients
-        V3 = torch.tensor(V, requires_grad=True)
+        V3 = torch.tensor(V, requires_grad=True, device=device)
         multislice_loss(V3, df.detach(), cs.detach(), ms, ipos, target).backward()
-        assert np.allclose(V3.grad.numpy(), ref["grad_V"], rtol=1e-5, atol=1e-6 * np.abs(ref["grad_V"]).max())
+        assert np.allclose(V3.grad.cpu().numpy(), ref["grad_V"], rtol=1e-5, atol=1e-6 * np.abs(ref["grad_V"]).max())
 
 
 def test_forward_save_then_adjoint_saved(oracle):

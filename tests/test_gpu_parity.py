@@ -29,6 +29,26 @@ def test_probe_matches_oracle(oracle, n):
     assert rel_l2(g, o) < 2e-5
 
 
+@pytest.mark.parametrize("n,opts", [(128, dict(gaussFlag=1, gaussScale=0.1)), (256, dict(aAIS=3.0)),
+                                    (256, dict(ismoth=1, Cc=1e7, dE_E=2e-6)), (400, dict(aAIS=4.0, ismoth=1)),
+                                    (128, dict(gaussFlag=1, gaussScale=0.02, aAIS=2.5))])
+def test_probe_options_match_oracle(oracle, n, opts):
+    """The probe's real-space Gaussian envelope (gaussian:), the AIS aperture with its
+    cosine edge (AIS aperture:), the smoothed aperture rim (smooth:) and the chromatic
+    defocus offset Cc dE/E, at power-of-two and 400^2 (row-transform) grids, and at a
+    centre moved by whole and fractional pixels (qs_make_probe_at)."""
+    cfg = small_config(nx=n, ny=n, pad=16, dx=0.04)
+    for k, v in opts.items():
+        setattr(cfg, k, v)
+    cx = float(np.float32(n // 2) * np.float32(0.04))
+    with _ctx(cfg) as ms:
+        g = ms.make_probe()
+        g_at = ms.make_probe(at=(cx + 3 * 0.04, cx - 1.37))
+    assert abs(np.sum(np.abs(g) ** 2) / (n * n) - 1) < 1e-5
+    assert rel_l2(g, oracle.probe(cfg)) < 2e-5
+    assert rel_l2(g_at, oracle.probe(cfg, at=(cx + 3 * 0.04, cx - 1.37))) < 2e-5
+
+
 @pytest.mark.parametrize("pot", [(160, 160), (128, 128), (200, 136), (256, 128), (128, 512), (1024, 2048),
                                  (2048, 1024)])
 def test_transmission_matches_oracle(oracle, pot):
@@ -182,12 +202,15 @@ def test_autotune_keeps_results():
     assert rel_l2(out["det"], ref["det"]) < 1e-6
 
 
-@pytest.mark.parametrize("shape", [(400, 400), (96, 160), (200, 128)])
+@pytest.mark.parametrize("shape", [(400, 400), (96, 160), (200, 128), (112, 98), (136, 88)])
 def test_generic_grid_sizes_match_oracle(oracle, shape):
     """Wave grids that are not powers of two (the reference's stem_STO_4x4.qsc uses
-    400 x 400) take the unfused path: probe, exit waves, every detector image (with a
-    shifted detector), diffraction patterns and intensities against the oracle, and
-    a run continued over a second call equal to one call over all slices."""
+    400 x 400): sizes whose prime factors are 2, 3, 5, 7 take the fused mixed-radix
+    path (radix 2, 3, 4, 5, 7, 8 passes: 400 = 4*4*5*5, 96 = 8*4*3, 160 = 8*4*5, 200 =
+    8*5*5, 112 = 4*4*7, 98 = 2*7*7), others (136 = 8*17, 88 = 8*11) the unfused path with
+    Bluestein rows: probe, exit waves, every detector image (with a shifted detector),
+    diffraction patterns and intensities against the oracle, and a run continued over a
+    second call equal to one call over all slices."""
     nx, ny = shape
     cfg = small_config(nx=nx, ny=ny, pad=24, slices=4, dx=0.03)
     V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=12)
