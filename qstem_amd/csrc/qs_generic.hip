@@ -182,12 +182,51 @@ __device__ __forceinline__ void dft7(float2* a) {
   a[4] = csub(r3, i3);
 }
 
+template <int R1, int R2, int DIR>
+__device__ __forceinline__ void dft_2stage(float2* a);
+
 template <int R, int DIR>
 __device__ __forceinline__ void dft_any(float2* a) {
   if constexpr (R == 3) dft3<DIR>(a);
   else if constexpr (R == 5) dft5<DIR>(a);
   else if constexpr (R == 7) dft7<DIR>(a);
+  else if constexpr (R == 9) dft_2stage<3, 3, DIR>(a);
+  else if constexpr (R == 25) dft_2stage<5, 5, DIR>(a);
   else dft<R, DIR>(a);
+}
+
+// Length R1*R2 DFT in registers (natural order in and out) by one Cooley-Tukey step:
+// input n = R2 n1 + n2, output k = k1 + R1 k2; R1-point DFTs over n1, twiddles
+// W_N^(n2 k1) (constants: twiddle_at of compile-time indices), R2-point DFTs over n2.
+template <int R1, int R2, int DIR>
+__device__ __forceinline__ void dft_2stage(float2* a) {
+  constexpr int N = R1 * R2;
+  float2 t[N];
+#pragma unroll
+  for (int n2 = 0; n2 < R2; n2++) {
+    float2 u[R1];
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) u[n1] = a[R2 * n1 + n2];
+    dft_any<R1, DIR>(u);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; k1++) {
+      float2 v = u[k1];
+      if ((n2 * k1) % N != 0) {
+        const float2 w = twiddle_at<N>((n2 * k1) % N);
+        v = DIR < 0 ? cmul(v, w) : cmulc(v, w);
+      }
+      t[n2 * R1 + k1] = v;
+    }
+  }
+#pragma unroll
+  for (int k1 = 0; k1 < R1; k1++) {
+    float2 u[R2];
+#pragma unroll
+    for (int n2 = 0; n2 < R2; n2++) u[n2] = t[n2 * R1 + k1];
+    dft_any<R2, DIR>(u);
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) a[k1 + R1 * k2] = u[k2];
+  }
 }
 
 // Largest LDS footprint of one block in float2 (V vectors of gen_ld(n) each), and
@@ -275,7 +314,10 @@ __device__ void gen_fft(float2* x, int V, int ld, const GenPlan& pl, const float
     else if (R == 4) gen_pass<4, DIR>(x, V, pl.n, ld, ns, tw, tid);
     else if (R == 5) gen_pass<5, DIR>(x, V, pl.n, ld, ns, tw, tid);
     else if (R == 7) gen_pass<7, DIR>(x, V, pl.n, ld, ns, tw, tid);
-    else gen_pass<8, DIR>(x, V, pl.n, ld, ns, tw, tid);
+    else if (R == 8) gen_pass<8, DIR>(x, V, pl.n, ld, ns, tw, tid);
+    else if (R == 9) gen_pass<9, DIR>(x, V, pl.n, ld, ns, tw, tid);
+    else if (R == 16) gen_pass<16, DIR>(x, V, pl.n, ld, ns, tw, tid);
+    else gen_pass<25, DIR>(x, V, pl.n, ld, ns, tw, tid);
     ns *= R;
   }
 }
@@ -291,7 +333,7 @@ constexpr int kGenPer = (kGenLds + kGenBlock - 1) / kGenBlock;
 // mode 0: the probe is the input (first slice of a fresh run); 1: the state's
 // y-spectrum (y-IFFT first); 2: exit: y-IFFT of the state into a.exitw (real space),
 // no transmit.  The transmission values are loaded with the input, before the y-IFFT.
-__global__ __launch_bounds__(kGenBlock) void k_gen_rows(GenRowArgs a) {
+__global__ __launch_bounds__(kGenBlock) __attribute__((amdgpu_waves_per_eu(4))) void k_gen_rows(GenRowArgs a) {
   extern __shared__ float2 gsm[];
   const int ny = a.ny, ld = gen_ld(ny);
   const int nrows = a.nprobe * a.nx;
@@ -452,21 +494,27 @@ bool gen_plan(int n, GenPlan* pl) {
     pl->r[pl->np++] = r;
     return true;
   };
-  // powers of two: radix 8 and 4 passes, one radix 2 when the exponent is odd and < 3
-  while (e2 >= 3 && (e2 % 2 == 1 || e2 >= 6)) {
-    if (!push(8)) return false;
-    e2 -= 3;
+  // few passes: radix 16 for powers of two (then 8, 4 or 2 for the rest), 25 and 9
+  // for pairs of fives and threes, then 5, 3, 7
+  while (e2 >= 4) {
+    if (!push(16)) return false;
+    e2 -= 4;
   }
-  while (e2 >= 2) {
-    if (!push(4)) return false;
-    e2 -= 2;
-  }
-  if (e2 == 1 && !push(2)) return false;
-  for (int r : {3, 5, 7})
+  if (e2 > 0 && !push(1 << e2)) return false;
+  for (int r : {5, 3}) {
+    while (m % (r * r) == 0) {
+      if (!push(r * r)) return false;
+      m /= r * r;
+    }
     while (m % r == 0) {
       if (!push(r)) return false;
       m /= r;
     }
+  }
+  while (m % 7 == 0) {
+    if (!push(7)) return false;
+    m /= 7;
+  }
   return m == 1 && n >= 2 && (long)gen_ld(n) <= kGenLds;
 }
 

@@ -159,7 +159,7 @@ void launch_gen_propagate(const GenPropArgs& a, int nprobe, hipStream_t st);
 constexpr int kGenMaxPasses = 16;
 struct GenPlan {
   int n, np;
-  int r[kGenMaxPasses];  // radices of the Stockham passes (2, 3, 4, 5, 7, 8), product n
+  int r[kGenMaxPasses];  // radices of the Stockham passes (2, 3, 4, 5, 7, 8, 9, 16, 25), product n
 };
 struct GenRowArgs {
   int nx, ny, nprobe, probe0, potNy, rpb, mode;
