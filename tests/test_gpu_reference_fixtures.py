@@ -16,8 +16,10 @@ stemlib.cpp:1672): a charged atom's potential is roughly (1 + q) times a
 neutral one.  With it the GPU reproduces both levels (ADF mean 0.0661 vs
 0.0673, norm at pixel (16, 2) 0.9743 vs 0.9711, image correlation 0.98); with
 neutral atoms, or ion offsets without Q5, the ADF level is 6x lower (0.0115).
-Difference kept on purpose: a 512x512 grid at the reference's dx = 0.0625 A
-(same 134 mrad band limit, 32 A instead of 25 A window).
+The grid is the reference's own 400x400 at dx = 0.0625 A (mixed-radix path).
+
+test_qstem3_reproduces_reference_outputs runs the drop-in driver on the reference's
+.qsc itself and compares the three files it writes with the same names.
 """
 import json
 import os
@@ -33,7 +35,7 @@ def _setup():
     from qstem_amd import Detector, StemConfig, scan_offsets
     from qstem_amd.specimen import build_supercell, read_cfg
     dx = 0.0625
-    nx = 512
+    nx = 400
     x0, x1, y0, y1 = 11.67, 19.71, 11.45, 19.32
     f = np.float32
     pnx = nx + int(f(f(x1) - f(x0)) / f(dx))
@@ -88,3 +90,74 @@ def test_sto_fixture_statistics():
     assert 0.9 < adf.mean() / ref_adf < 1.1
     assert abs(norm - ref_norm) < 0.01
     assert res["neutral"][0].mean() / ref_adf < 0.3
+
+
+def test_qstem3_reproduces_reference_outputs(tmp_path):
+    """qstem3 on the reference's own stem_STO_4x4.qsc, unchanged except for the frozen-
+    phonon run count (3, the fixture's params[0]) and save level 2 (the reference wrote
+    mulswav_16_2.img, which save level 1 does not produce): the same 400x400 grid, 20x20
+    scan, 40 slices, charged SrTiO3, 3 time-seeded (here: seeded) phonon runs.  The
+    three files of tests/data are compared statistically with the ones qstem3 writes
+    under the same names:
+      detector1_2.img   ADF 70-133 mrad at t = 46.86 A (last run): mean, image correlation
+      diffAvg_0_16.img  diffraction pattern of scan pixel (0, 16) averaged over the runs:
+                        total and radial profile
+      mulswav_16_2.img  exit wave of pixel (16, 2): norm
+    The detector images' headers differ from the fixture's, which an older version
+    wrote: its comment is the detector name and its dx/dy are (stop - start)/pixels of
+    the scan as written (0.402, 0.3935); the source sets "STEM image" and moves the scan
+    stops to whole potential pixels (stem3.cpp:1441-1442: 0.4, 0.3906), as qstem3 does."""
+    import subprocess
+    from qstem_amd.imgio import read_img
+    root = os.path.dirname(GOLD)
+    exe = os.path.join(os.path.dirname(root), "qstem_amd", "bin", "qstem3")
+    for f in ("SrTiO3.cfg",):
+        with open(os.path.join(GOLD, f), "rb") as src, open(tmp_path / f, "wb") as dst:
+            dst.write(src.read())
+    lines = []
+    for line in open(os.path.join(GOLD, "stem_STO_4x4.qsc")).read().splitlines():
+        if line.startswith("Runs for averaging:"):
+            line = "Runs for averaging: 3"
+        elif line.startswith("save level:"):
+            line = "save level: 2"
+        lines.append(line)
+    (tmp_path / "run.qsc").write_text("\n".join(lines) + "\n")
+    print("qstem3 on stem_STO_4x4.qsc (3 runs, save level 2)", flush=True)
+    r = subprocess.run([exe, "run.qsc"], cwd=tmp_path, capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    print(r.stdout.strip().splitlines()[-1])
+    out = tmp_path / "STO_4x4"
+    ref = json.load(open(os.path.join(GOLD, "reference_stats.json")))
+    # detector1_2.img
+    adf = read_img(str(out / "detector1_2.img"))
+    fix = read_img(os.path.join(GOLD, "detector1_2.img"))
+    # the fixture's comment is the detector name, set by the version that wrote it; the
+    # source's saveSTEMImages sets "STEM image" (stemlib.cpp:2734-2737), as qstem3 does
+    assert adf.data.shape == fix.data.shape == (20, 20) and adf.comment == "STEM image" and fix.comment == "detector1"
+    assert abs(adf.t - fix.t) < 1e-3 and adf.params[0] == 3.0
+    corr = np.corrcoef(adf.data.ravel(), fix.data.ravel())[0, 1]
+    ratio = adf.data.mean() / fix.data.mean()
+    # diffAvg_0_16.img
+    d = read_img(str(out / "diffAvg_0_16.img"))
+    rd = ref["diffAvg_0_16"]
+    assert d.data.shape == (400, 400) and abs(d.dx - rd["dx"]) < 1e-6 and abs(d.t - rd["t"]) < 1e-3
+    n = 400
+    k = (np.arange(n) - n // 2) / (n * d.dx)
+    kr = np.sqrt(k[:, None] ** 2 + k[None, :] ** 2)
+    edges = np.array(rd["radial_edges"])
+    prof = np.array([d.data[(kr >= lo) & (kr < hi)].mean() for lo, hi in zip(edges[:-1], edges[1:])])
+    want = np.array(rd["radial_mean"])
+    sum_ratio = d.data.sum() / rd["sum"]
+    strong = want > 1e-3 * want.max()
+    ring_ratio = prof[strong] / want[strong]
+    # mulswav_16_2.img
+    w = read_img(str(out / "mulswav_16_2.img"))
+    norm = float(np.sum(np.abs(w.data.astype(np.complex128)) ** 2) / w.data.size)
+    print("ADF mean ratio %.3f, correlation %.3f; diffAvg_0_16 total ratio %.4f, ring ratios %.2f..%.2f "
+          "over %d rings; exit norm %.4f (reference %.4f)" % (ratio, corr, sum_ratio, ring_ratio.min(),
+                                                                ring_ratio.max(), strong.sum(), norm,
+                                                                ref["mulswav_16_2"]["norm"]))
+    assert corr > 0.95 and 0.9 < ratio < 1.1
+    assert abs(sum_ratio - 1) < 0.02
+    assert np.all(np.abs(ring_ratio - 1) < 0.25)
+    assert abs(norm - ref["mulswav_16_2"]["norm"]) < 0.01 and abs(w.t - ref["mulswav_16_2"]["t"]) < 1e-3
