@@ -322,7 +322,10 @@ int qs_set_graph(qs_ctx *ctx, int on);
 
 /* Launch shape of the two sweeps (0 = library default): row-sweep steps per
  * wavefront (a power of two) and column tiles per block; qs_get_tuning also
- * reports the probe group size of qs_set_batch.  No reference counterpart. */
+ * reports the probe group size of qs_set_batch.  For grids on the mixed-radix path
+ * (not powers of two, prime factors 2, 3, 5, 7) the two values are the rows per block
+ * of the row kernel and the columns per tile of the column kernel (powers of two up
+ * to 16 that fit the block's LDS; QS_ERR_ARG otherwise).  No reference counterpart. */
 int qs_set_tuning(qs_ctx *ctx, int rows_per_wave, int tiles_per_block);
 int qs_get_tuning(const qs_ctx *ctx, int *rows_per_wave, int *tiles_per_block, int *batch);
 
@@ -330,8 +333,13 @@ int qs_get_tuning(const qs_ctx *ctx, int *rows_per_wave, int *tiles_per_block, i
  * probe (first 8 slices, up to 512 of the given window offsets) over probe group
  * sizes, row-sweep steps per wavefront and column tiles per block, applies the
  * fastest (as qs_set_batch + qs_set_tuning) and reports it (outputs nullable;
- * rate in probe-slices/s).  Results are cached per device and wave geometry for
- * the life of the process.  No reference counterpart. */
+ * rate in probe-slices/s).  On the mixed-radix path it tunes the group size, rows
+ * per block, columns per tile and the radix plan of each axis (few large radices,
+ * radix 8, or radices up to 8: results then differ by FP32 rounding only).  Results are cached per device model and wave geometry
+ * for the life of the process, and across processes in the JSON file named by the
+ * environment variable QSTEM_TUNE_CACHE when it is set (read at the first call,
+ * rewritten after each new measurement; no file is touched when it is unset).  No
+ * reference counterpart. */
 int qs_autotune(qs_ctx *ctx, const int *ipos, int nprobe, int *batch, int *rows_per_wave, int *tiles_per_block,
                 float *rate);
 

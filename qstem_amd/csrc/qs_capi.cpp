@@ -21,6 +21,7 @@
 #include <mutex>
 #include <string>
 #include <tuple>
+#include <unistd.h>
 #include <vector>
 
 #include "../../include/qstem.h"
@@ -168,6 +169,7 @@ struct qs_ctx {
   // rows per block of the row kernel
   bool mixed = false;
   qs::GenPlan gpx{}, gpy{};
+  int gvx = 0, gvy = 0;  // radix plan variants of gpx, gpy
   DevBuf gtiles;
   int gntiles = 0, gtc = 0, grpb = 0;
   // band limit of the transmission functions as band-compact sweeps (potNx and
@@ -234,6 +236,39 @@ static hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
   hipError_t e = b.ensure(sizeof(T) * v.size());
   if (e != hipSuccess) return e;
   return hipMemcpyAsync(b.p, v.data(), sizeof(T) * v.size(), hipMemcpyHostToDevice, st);
+}
+
+// Launch shape of the mixed-radix path: rows per block of k_gen_rows, columns per
+// tile of k_gen_cols (powers of two; 0 = the default, the most that fit the block's
+// LDS, at most 16) and the radix plan variants of x and y; rebuilds the list of
+// column tiles that hold an in-band ky column.
+static int set_mixed_shape(qs_ctx* c, int rpb, int tc, int vx, int vy) {
+  const int nx = c->p.nx, ny = c->p.ny;
+  if (rpb == 0) rpb = qs::gen_rows_per_block(ny);
+  if (tc == 0) tc = qs::gen_cols_per_tile(nx);
+  qs::GenPlan px{}, py{};
+  if ((rpb & (rpb - 1)) || (tc & (tc - 1)) || rpb < 1 || tc < 1 || rpb > 16 || tc > 16 ||
+      rpb > qs::gen_max_vectors(ny) || tc > qs::gen_max_vectors(nx) || !qs::gen_plan_variant(nx, vx, &px) ||
+      !qs::gen_plan_variant(ny, vy, &py))
+    return fail(c, QS_ERR_ARG, "mixed-radix launch shape: rows per block and columns per tile must be powers of "
+                               "two up to 16 that fit the block's LDS, plan variants 0-2");
+  std::vector<int> tiles;  // tiles of tc columns with a ky column inside the band
+  for (int y0 = 0; y0 < ny; y0 += tc) {
+    bool any = false;
+    for (int y = y0; y < std::min(ny, y0 + tc); y++) any = any || c->ky2[y] < c->k2max;
+    if (any) tiles.push_back(y0);
+  }
+  if (tiles.empty()) return fail(c, QS_ERR_ARG, "no ky column inside the band");
+  HIPCHK(c, upload(c->gtiles, tiles, c->stream));
+  HIPCHK(c, hipStreamSynchronize(c->stream));
+  c->gntiles = (int)tiles.size();
+  c->grpb = rpb;
+  c->gtc = tc;
+  c->gpx = px;
+  c->gpy = py;
+  c->gvx = vx;
+  c->gvy = vy;
+  return QS_OK;
 }
 
 static int setup_rowxform(qs_ctx* c, RowXform& r, int n) {
@@ -443,19 +478,8 @@ int qs_create(const qs_params* p, qs_ctx** out) {
     if (rc != QS_OK) return bad(rc);
     c->mixed = qs::gen_plan(nx, &c->gpx) && qs::gen_plan(ny, &c->gpy);
     if (c->mixed) {
-      c->gtc = qs::gen_cols_per_tile(nx);
-      c->grpb = qs::gen_rows_per_block(ny);
-      std::vector<int> tiles;  // tiles of gtc columns with a ky column inside the band
-      for (int y0 = 0; y0 < ny; y0 += c->gtc) {
-        bool any = false;
-        for (int y = y0; y < std::min(ny, y0 + c->gtc); y++) any = any || c->ky2[y] < c->k2max;
-        if (any) tiles.push_back(y0);
-      }
-      c->gntiles = (int)tiles.size();
-      if (c->gntiles == 0 || upload(c->gtiles, tiles, c->stream) != hipSuccess) {
-        c->err = "device allocation of the column tile list failed";
-        return bad(QS_ERR_HIP);
-      }
+      const int rc = set_mixed_shape(c, 0, 0, 0, 0);
+      if (rc != QS_OK) return bad(rc);
     }
   }
   // transmission band-limit tables (stemlib.cpp:2219-2262), float arithmetic, Q1 optional
@@ -1154,6 +1178,7 @@ int qs_share_trans(qs_ctx* dst, qs_ctx* src) {
 int qs_set_tuning(qs_ctx* c, int rows_per_wave, int tiles_per_block) {
   if (!c || rows_per_wave < 0 || tiles_per_block < 0 || (rows_per_wave & (rows_per_wave - 1)) != 0)
     return fail(c, QS_ERR_ARG, "qs_set_tuning: rows_per_wave must be 0 or a power of two, tiles_per_block >= 0");
+  if (c->mixed) return set_mixed_shape(c, rows_per_wave, tiles_per_block, c->gvx, c->gvy);
   c->rows_per_wave = rows_per_wave;
   c->tiles_per_block = tiles_per_block;
   return QS_OK;
@@ -1161,8 +1186,8 @@ int qs_set_tuning(qs_ctx* c, int rows_per_wave, int tiles_per_block) {
 
 int qs_get_tuning(const qs_ctx* c, int* rows_per_wave, int* tiles_per_block, int* batch) {
   if (!c || !rows_per_wave || !tiles_per_block || !batch) return QS_ERR_ARG;
-  *rows_per_wave = c->rows_per_wave;
-  *tiles_per_block = c->tiles_per_block;
+  *rows_per_wave = c->mixed ? c->grpb : c->rows_per_wave;
+  *tiles_per_block = c->mixed ? c->gtc : c->tiles_per_block;
   *batch = c->batch;
   return QS_OK;
 }
@@ -2235,35 +2260,109 @@ int qs_atom_gradients(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_p
 // Coordinate descent: group size first (library launch defaults), then row-sweep
 // steps per wavefront, then column tiles per block; a launch shape replaces the
 // current best only when 2% faster (timing noise).  Results are cached per
-// device and wave geometry for the life of the process.
+// device model and wave geometry for the life of the process and, when the
+// environment variable QSTEM_TUNE_CACHE names a file, in that JSON file across
+// processes (read once per process, rewritten with the union of its entries and
+// this process's after every new measurement; entries of another kernel version
+// are ignored).
 namespace {
+// bumped whenever the sweep kernels or their launch parameters change meaning
+const char* const kTuneVersion = "sweeps-3";
 struct TuneKey {
-  int dev, nx, ny, kcp, potNy, nstreams;
+  std::string device;  // gcnArchName / name / compute units
+  int nx, ny, kcp, potNy, nstreams;
   bool operator<(const TuneKey& o) const {
-    return std::tie(dev, nx, ny, kcp, potNy, nstreams) < std::tie(o.dev, o.nx, o.ny, o.kcp, o.potNy, o.nstreams);
+    return std::tie(device, nx, ny, kcp, potNy, nstreams) <
+           std::tie(o.device, o.nx, o.ny, o.kcp, o.potNy, o.nstreams);
   }
 };
 struct TuneVal {
   int batch, rw, tpb;
   float rate;
+  int vx, vy;  // mixed-radix plan variants (0 for the power-of-two sweeps)
 };
 std::mutex g_tune_mu;
 std::map<TuneKey, TuneVal> g_tune_cache;
+bool g_tune_file_read = false;
+
+std::string tune_file() {
+  const char* e = getenv("QSTEM_TUNE_CACHE");
+  return e ? std::string(e) : std::string();
+}
+
+// One entry per line, as tune_save writes them; other lines are skipped.
+void tune_load_locked(const std::string& path) {
+  FILE* f = fopen(path.c_str(), "r");
+  if (!f) return;
+  char line[1024];
+  while (fgets(line, sizeof line, f)) {
+    char dev[256] = {0}, ver[64] = {0};
+    TuneKey k;
+    TuneVal v{};
+    if (sscanf(line,
+               " {\"version\": \"%63[^\"]\", \"device\": \"%255[^\"]\", \"nx\": %d, \"ny\": %d, \"kcp\": %d, "
+               "\"potNy\": %d, \"streams\": %d, \"batch\": %d, \"rows_per_wave\": %d, \"tiles_per_block\": %d, "
+               "\"plan_x\": %d, \"plan_y\": %d, \"rate\": %f}",
+               ver, dev, &k.nx, &k.ny, &k.kcp, &k.potNy, &k.nstreams, &v.batch, &v.rw, &v.tpb, &v.vx, &v.vy,
+               &v.rate) != 13)
+      continue;
+    if (strcmp(ver, kTuneVersion) != 0 || v.batch < 1) continue;
+    k.device = dev;
+    g_tune_cache.emplace(k, v);  // this process's own measurements win
+  }
+  fclose(f);
+}
+
+// Rewrites the file with the union of its entries and the in-memory ones (a write
+// to a temporary file in the same directory, then a rename).
+void tune_save_locked(const std::string& path) {
+  tune_load_locked(path);
+  const std::string tmp = path + ".tmp" + std::to_string((long)getpid());
+  FILE* f = fopen(tmp.c_str(), "w");
+  if (!f) return;
+  fprintf(f, "{\"qstem_amd_autotune\": [\n");
+  size_t i = 0;
+  for (const auto& kv : g_tune_cache) {
+    const TuneKey& k = kv.first;
+    const TuneVal& v = kv.second;
+    fprintf(f,
+            " {\"version\": \"%s\", \"device\": \"%s\", \"nx\": %d, \"ny\": %d, \"kcp\": %d, \"potNy\": %d, "
+            "\"streams\": %d, \"batch\": %d, \"rows_per_wave\": %d, \"tiles_per_block\": %d, \"plan_x\": %d, "
+            "\"plan_y\": %d, \"rate\": %.1f}%s\n",
+            kTuneVersion, k.device.c_str(), k.nx, k.ny, k.kcp, k.potNy, k.nstreams, v.batch, v.rw, v.tpb, v.vx, v.vy,
+            v.rate, ++i < g_tune_cache.size() ? "," : "");
+  }
+  fprintf(f, "]}\n");
+  const bool ok = fclose(f) == 0;
+  if (!ok || rename(tmp.c_str(), path.c_str()) != 0) remove(tmp.c_str());
+}
 }  // namespace
 
 int qs_autotune(qs_ctx* c, const int* ipos, int nprobe, int* batch, int* rows_per_wave, int* tiles_per_block,
                 float* rate) {
   if (!c || !ipos || nprobe < 1) return fail(c, QS_ERR_ARG, "qs_autotune: bad arguments");
-  if (c->generic) return fail(c, QS_ERR_UNSUPPORTED, "qs_autotune: tunes the power-of-two sweeps only");
+  if (c->generic && !c->mixed)
+    return fail(c, QS_ERR_UNSUPPORTED,
+                "qs_autotune: tunes the power-of-two sweeps and the mixed-radix path (prime factors 2, 3, 5, 7)");
   if (trans_slices_of(c) < 1 || !c->have_probe)
     return fail(c, QS_ERR_STATE, "qs_autotune: set the potential and the probe first");
   int dev = 0;
   HIPCHK(c, hipGetDevice(&dev));
-  const TuneKey key{dev, c->p.nx, c->p.ny, c->kcp, c->p.potNy, c->nstreams};
-  TuneVal best{0, 0, 0, 0.f};
+  hipDeviceProp_t prop;
+  HIPCHK(c, hipGetDeviceProperties(&prop, dev));
+  std::string dname = std::string(prop.gcnArchName) + "/" + prop.name + "/" + std::to_string(prop.multiProcessorCount);
+  for (char& ch : dname)
+    if (ch == '"' || ch == '\\' || (unsigned char)ch < 32) ch = '_';
+  const TuneKey key{dname, c->p.nx, c->p.ny, c->mixed ? -1 : c->kcp, c->p.potNy, c->nstreams};
+  const std::string file = tune_file();
+  TuneVal best{0, 0, 0, 0.f, 0, 0};
   bool cached = false;
   {
     std::lock_guard<std::mutex> lk(g_tune_mu);
+    if (!file.empty() && !g_tune_file_read) {
+      tune_load_locked(file);
+      g_tune_file_read = true;
+    }
     auto it = g_tune_cache.find(key);
     if (it != g_tune_cache.end()) {
       best = it->second;
@@ -2276,10 +2375,17 @@ int qs_autotune(qs_ctx* c, const int* ipos, int nprobe, int* batch, int* rows_pe
     const bool graph = c->use_graph, prof = c->profiling;
     c->use_graph = false;
     c->profiling = false;
+    // mixed-radix path: rw = rows per block, tpb = columns per tile, vx/vy = plan variants
+    int vx = c->gvx, vy = c->gvy;
     auto measure = [&](int B, int rw, int tpb, float* out) -> int {
       c->batch = B;
-      c->rows_per_wave = rw;
-      c->tiles_per_block = tpb;
+      if (c->mixed) {
+        const int rc = set_mixed_shape(c, rw, tpb, vx, vy);
+        if (rc != QS_OK) return rc;
+      } else {
+        c->rows_per_wave = rw;
+        c->tiles_per_block = tpb;
+      }
       float tmin = 1e30f;
       for (int r = 0; r < 3; r++) {  // first run also pays first-touch costs
         const int rc = qs_run_stem(c, ipos, n, 0, nullptr, S, 0, nullptr, nullptr, nullptr, nullptr);
@@ -2293,7 +2399,8 @@ int qs_autotune(qs_ctx* c, const int* ipos, int nprobe, int* batch, int* rows_pe
       float r = 0.f;
       const int rc = measure(B, rw, tpb, &r);
       if (rc != QS_OK) return rc;
-      if (r > best.rate * (best.batch == B ? 1.02f : 1.0f)) best = TuneVal{B, rw, tpb, r};  // launch shapes: 2% hysteresis
+      if (r > best.rate * (best.batch == B ? 1.02f : 1.0f))  // launch shapes: 2% hysteresis
+        best = TuneVal{B, rw, tpb, r, vx, vy};
       return QS_OK;
     };
     int rc = QS_OK;
@@ -2301,20 +2408,43 @@ int qs_autotune(qs_ctx* c, const int* ipos, int nprobe, int* batch, int* rows_pe
     for (int B : Bs)
       if (rc == QS_OK && (B <= n || B == Bs[0])) rc = consider(std::min(B, n), 0, 0);
     const int B0 = best.batch;
-    for (int rw : {1, 2, 4, 8})
-      if (rc == QS_OK) rc = consider(B0, rw, best.tpb);
-    const int rw0 = best.rw;
-    for (int tpb : {1, 2, 4})
-      if (rc == QS_OK) rc = consider(B0, rw0, tpb);
+    if (c->mixed) {
+      for (int rw : {1, 2, 4, 8, 16})
+        if (rc == QS_OK && rw <= qs::gen_max_vectors(c->p.ny)) rc = consider(B0, rw, best.tpb);
+      for (int tc : {2, 4, 8, 16})
+        if (rc == QS_OK && tc <= qs::gen_max_vectors(c->p.nx)) rc = consider(B0, best.rw, tc);
+      for (int v = 0; v < qs::kGenPlanVariants && rc == QS_OK; v++) {  // x plan, then y plan
+        vx = v;
+        vy = best.vy;
+        rc = consider(B0, best.rw, best.tpb);
+      }
+      for (int v = 0; v < qs::kGenPlanVariants && rc == QS_OK; v++) {
+        vx = best.vx;
+        vy = v;
+        rc = consider(B0, best.rw, best.tpb);
+      }
+    } else {
+      for (int rw : {1, 2, 4, 8})
+        if (rc == QS_OK) rc = consider(B0, rw, best.tpb);
+      const int rw0 = best.rw;
+      for (int tpb : {1, 2, 4})
+        if (rc == QS_OK) rc = consider(B0, rw0, tpb);
+    }
     c->use_graph = graph;
     c->profiling = prof;
     if (rc != QS_OK) return rc;
     std::lock_guard<std::mutex> lk(g_tune_mu);
     g_tune_cache[key] = best;
+    if (!file.empty()) tune_save_locked(file);
   }
   c->batch = best.batch;
-  c->rows_per_wave = best.rw;
-  c->tiles_per_block = best.tpb;
+  if (c->mixed) {
+    const int rc = set_mixed_shape(c, best.rw, best.tpb, best.vx, best.vy);
+    if (rc != QS_OK) return rc;
+  } else {
+    c->rows_per_wave = best.rw;
+    c->tiles_per_block = best.tpb;
+  }
   drop_graph(c);
   if (batch) *batch = best.batch;
   if (rows_per_wave) *rows_per_wave = best.rw;

@@ -481,7 +481,10 @@ __global__ __launch_bounds__(kGenBlock) void k_gen_cols(GenColArgs a) {
   }
 }
 
-bool gen_plan(int n, GenPlan* pl) {
+bool gen_plan(int n, GenPlan* pl) { return gen_plan_variant(n, 0, pl); }
+
+bool gen_plan_variant(int n, int variant, GenPlan* pl) {
+  if (variant < 0 || variant >= kGenPlanVariants) return false;
   int m = n, e2 = 0;
   pl->n = n;
   pl->np = 0;
@@ -494,15 +497,28 @@ bool gen_plan(int n, GenPlan* pl) {
     pl->r[pl->np++] = r;
     return true;
   };
-  // few passes: radix 16 for powers of two (then 8, 4 or 2 for the rest), 25 and 9
-  // for pairs of fives and threes, then 5, 3, 7
-  while (e2 >= 4) {
-    if (!push(16)) return false;
-    e2 -= 4;
+  // variant 0: few passes: radix 16 for powers of two (then 8, 4 or 2 for the rest),
+  // 25 and 9 for pairs of fives and threes, then 5, 3, 7; variant 1: radix 8 for powers
+  // of two, 25 and 9; variant 2: radices 2-8 only (4 and 8 for powers of two)
+  const int big = variant == 0 ? 4 : 3;  // log2 of the power-of-two radix
+  if (variant == 2) {
+    while (e2 >= 3 && (e2 % 2 == 1 || e2 >= 6)) {
+      if (!push(8)) return false;
+      e2 -= 3;
+    }
+    while (e2 >= 2) {
+      if (!push(4)) return false;
+      e2 -= 2;
+    }
+  } else {
+    while (e2 >= big) {
+      if (!push(1 << big)) return false;
+      e2 -= big;
+    }
   }
   if (e2 > 0 && !push(1 << e2)) return false;
   for (int r : {5, 3}) {
-    while (m % (r * r) == 0) {
+    while (variant != 2 && m % (r * r) == 0) {
       if (!push(r * r)) return false;
       m /= r * r;
     }
@@ -527,6 +543,7 @@ static int gen_pow2_fit(int n) {
 }
 int gen_rows_per_block(int ny) { return gen_pow2_fit(ny); }
 int gen_cols_per_tile(int nx) { return gen_pow2_fit(nx); }
+int gen_max_vectors(int n) { return kGenLds / gen_ld(n); }
 
 void launch_gen_rows(const GenRowArgs& a, hipStream_t st) {
   const int nrows = a.nprobe * a.nx;

@@ -194,8 +194,13 @@ struct GenColArgs {
   const float2* tw;     // W_nx^m, m < nx
 };
 bool gen_plan(int n, GenPlan* pl);  // false: n has a prime factor above 7
-int gen_rows_per_block(int ny);
-int gen_cols_per_tile(int nx);
+// radix plan variants (0 = gen_plan's): 0 few passes (radix 16, 25, 9 first),
+// 1 radix 8 for powers of two, 2 radices 2-8 only
+constexpr int kGenPlanVariants = 3;
+bool gen_plan_variant(int n, int variant, GenPlan* pl);
+int gen_rows_per_block(int ny);   // default rows per block of k_gen_rows (power of two)
+int gen_cols_per_tile(int nx);    // default columns per tile of k_gen_cols (power of two)
+int gen_max_vectors(int n);       // most vectors of length n one block holds in LDS
 void launch_gen_rows(const GenRowArgs& a, hipStream_t st);
 void launch_gen_cols(const GenColArgs& a, int ntiles, hipStream_t st);
 

@@ -243,18 +243,94 @@ def test_generic_grid_sizes_match_oracle(oracle, shape):
     assert rel_l2(cont["exit"], out["exit"]) < 1e-6
 
 
-def test_generic_grid_refusals():
-    """The adjoint, uneven batches, the autotuner and the no-allocation run need the
-    power-of-two sweeps: they refuse other grids with an error, before any launch."""
+@pytest.mark.parametrize("n", [96, 88])
+def test_generic_grid_refusals(n):
+    """The adjoint, uneven batches and the no-allocation run need the power-of-two
+    sweeps, and the autotuner the power-of-two or the mixed-radix path (88 = 8*11 has
+    neither): they refuse other grids with an error, before any launch."""
     from qstem_amd import QstemError, run_groups
-    cfg = small_config(nx=96, ny=96, pad=8, slices=2, dx=0.05)
+    cfg = small_config(nx=n, ny=n, pad=8, slices=2, dx=0.05)
     with _ctx(cfg) as ms:
         ms.set_potential(smooth_potential(2, cfg.potNx, cfg.potNy, seed=3))
         ms.make_probe()
         ipos = np.zeros((1, 2), np.int32)
-        for call in (lambda: ms.adjoint(ipos, np.zeros((1, 96, 96), np.float32)),
-                     lambda: ms.run_stem_noalloc(ipos),
-                     lambda: ms.autotune(ipos),
-                     lambda: run_groups([(ms, ipos, None)])):
+        calls = [lambda: ms.adjoint(ipos, np.zeros((1, n, n), np.float32)),
+                 lambda: ms.run_stem_noalloc(ipos),
+                 lambda: run_groups([(ms, ipos, None)])]
+        if n == 88:
+            calls.append(lambda: ms.autotune(ipos))
+        for call in calls:
             with pytest.raises(QstemError, match="power"):
                 call()
+
+
+_TUNE_SCRIPT = r"""
+import json, sys, time
+sys.path.insert(0, sys.argv[1]); sys.path.insert(0, sys.argv[1] + "/tests")
+from qstem_amd import Multislice
+from cases import probe_positions, small_config, smooth_potential
+cfg = small_config(nx=256, ny=256, pad=32, slices=8, dx=0.03)
+with Multislice(cfg) as ms:
+    ms.set_potential(smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=4))
+    ms.make_probe()
+    t0 = time.time()
+    best = ms.autotune(probe_positions(cfg, 64, seed=2))
+    best["seconds"] = time.time() - t0
+print(json.dumps(best))
+"""
+
+
+def test_autotune_disk_cache(tmp_path):
+    """QSTEM_TUNE_CACHE names a JSON file: the first process measures and writes it; a
+    second process reads it and applies the stored launch shape without measuring."""
+    import json
+    import os
+    import subprocess
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    path = tmp_path / "tune.json"
+    env = dict(os.environ, QSTEM_TUNE_CACHE=str(path))
+    runs = []
+    for i in range(2):
+        print("autotune process %d" % i, flush=True)
+        r = subprocess.run([sys.executable, "-c", _TUNE_SCRIPT, root], env=env, capture_output=True, text=True,
+                           timeout=300)
+        assert r.returncode == 0, r.stderr[-2000:]
+        runs.append(json.loads(r.stdout.strip().splitlines()[-1]))
+        if i == 0:
+            entries = json.load(open(path))["qstem_amd_autotune"]
+            assert len(entries) == 1 and entries[0]["nx"] == 256 and entries[0]["batch"] == runs[0]["batch"]
+    a, b = runs
+    assert (a["batch"], a["rows_per_wave"], a["tiles_per_block"]) == (b["batch"], b["rows_per_wave"],
+                                                                       b["tiles_per_block"])
+    assert abs(a["rate"] - b["rate"]) <= 1e-6 * a["rate"] + 0.1 and b["seconds"] < 0.5 * a["seconds"]
+
+
+def test_autotune_mixed_radix_path(oracle):
+    """On a 400 x 400 grid the autotuner chooses group size, rows per block, columns per
+    tile and the radix plans; the tuned run agrees with the default one and with the
+    oracle, and qs_set_tuning refuses shapes that do not fit."""
+    from qstem_amd import QstemError
+    cfg = small_config(nx=400, ny=400, pad=24, slices=4, dx=0.03)
+    V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=5)
+    ipos = probe_positions(cfg, 6, seed=4)
+    bins, nbins = oracle.record_bins(cfg.slices, outputInterval=2)
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        psi0 = ms.make_probe()
+        t32 = ms.get_trans()
+        ref = ms.run_stem(ipos, bins=bins, nbins=nbins, want_exit=True)
+        best = ms.autotune(ipos)
+        print("mixed-radix tuning", best)
+        tuned = ms.run_stem(ipos, bins=bins, nbins=nbins, want_exit=True)
+        with pytest.raises(QstemError):
+            ms.set_tuning(3, 0)
+        with pytest.raises(QstemError):
+            ms.set_tuning(32, 0)
+        ms.set_tuning(2, 4)
+        other = ms.run_stem(ipos, bins=bins, nbins=nbins, want_exit=True)
+    assert rel_l2(tuned["exit"], ref["exit"]) < 1e-5 and rel_l2(other["exit"], ref["exit"]) < 1e-5
+    assert rel_l2(tuned["det"], ref["det"]) < 1e-5
+    ex, det, _ = oracle.multislice(cfg, t32.astype(np.complex128), ipos, psi0.astype(np.complex128), bins=bins,
+                                   nbins=nbins)
+    assert rel_l2(tuned["exit"], ex) < TOL
