@@ -20,8 +20,10 @@ sequence), the batched/per-probe ratio, the adjoint/forward time ratio of the
 sample of probes, the roofline of the slice-step kernel pair, the plain CPU
 FP32 multislice (oracle/qs_cpu32.c, OpenMP over probes) timed on the host cores
 on a sample of this workload and on the CPU configuration (128^2, 20 slices,
-16x16 scan), and, as extra keys, the uneven frozen-phonon batch and the 4096^2
-TEM workload with its checkpointed adjoint.
+16x16 scan), and, as extra keys, the uneven frozen-phonon batch, the reference's
+own example configuration (stem_STO_4x4.qsc: charged SrTiO3, 400^2 wave, 20x20
+scan, 3 phonon configurations) and the 4096^2 TEM workload with its checkpointed
+adjoint.
 """
 import argparse
 import json
@@ -133,6 +135,59 @@ def uneven_workload(nconfig=16, steps=3, streams=0):
             "potential_build_s_per_config": round(t_pot / nconfig, 3),
             "transmission_GB": round(trans_bytes / 1e9, 2),
             "adf_mean_per_size": {str(n): float(np.mean(o["det"][-1, 1])) for n, o in zip(ctxs, outs)}}
+
+
+def reference_config_workload(runs=3):
+    """The reference's own example, stem3/debug_data/stem_STO_4x4.qsc, through the
+    library as qstem3 runs it: charged SrTiO3 (Sr +2, Ti +4, O -2; Debye-Waller factors
+    of SrTiO3.cfg) 9 x 9 x 20 cells, 400^2 wave at 0.0625 A (mixed-radix path), 200 kV,
+    40 slices of 1.9525 A, 20 x 20 scan over [11.67, 19.71] x [11.45, 19.32] A, four
+    annular detectors, outputs every 8 slices, Einstein phonons at 300 K, `runs`
+    configurations, each built on the GPU (ion-offset LUTs) and run over all 400
+    positions.  Returns probe-slices/s of the multislice (device time) and the wall
+    time per configuration including the potential build."""
+    import torch
+    from qstem_amd import Detector, Multislice, StemConfig, record_bins, scan_offsets
+    from qstem_amd.specimen import build_supercell, perovskite_cell
+    f = np.float32
+    nx, dx, nslice = 400, 0.0625, 40
+    x0, x1, y0, y1 = 11.67, 19.71, 11.45, 19.32
+    npx = int((f(x1) - f(x0)) / f(dx))
+    npy = int((f(y1) - f(y0)) / f(dx))
+    sx1, sy1 = f(f(x0) + f(dx) * npx), f(f(y0) + f(dx) * npy)
+    dets = [Detector(70.0, 133.0, "detector1"), Detector(50.0, 70.0, "detector2"), Detector(30.0, 50.0, "detector3"),
+            Detector(0.0, 40.0, "detector4")]
+    cfg = StemConfig(nx=nx, ny=nx, resolutionX=dx, resolutionY=dx, v0=200.0, slices=nslice, sliceThickness=1.9525,
+                     potNx=nx + npx, potNy=nx + npy, by=float(f(35.145)), bandlimittrans=0, Cs=float(f(f(0.05) * 1e7)),
+                     df0=float(f(10.0 * f(-13.7))), alpha=15.0, ismoth=1, Cc=1e7, electronScale=1.0, compat=3,
+                     detectors=dets)
+    cell = perovskite_cell(charges=(2.0, 4.0, -2.0))
+    ipos = scan_offsets(x0, sx1, 20, y0, sy1, 20, dx, dx, cfg.potNx, cfg.potNy, nx, nx)
+    offx = float(f(np.float64(f(x0)) - 0.5 * nx * np.float64(f(dx))))
+    offy = float(f(np.float64(f(y0)) - 0.5 * nx * np.float64(f(dx))))
+    bins, nbins = record_bins(nslice, outputInterval=8)
+    configs = [build_supercell(cell, 9, 9, 20, tds=True, temperature=300.0, seed=k + 1)[0] for k in range(runs + 1)]
+    with Multislice(cfg) as ms:
+        def run(atoms):
+            ms.build_potential(atoms, atomRadius=5.0, potOffsetX=offx, potOffsetY=offy, czOffset=0.976, tds=1)
+            ms.make_probe()
+            out = ms.run_stem(ipos, bins=bins, nbins=nbins)
+            return ms.last_run_ms, out
+        run(configs[0])  # warm-up
+        torch.cuda.synchronize()
+        dev_ms = []
+        t0 = time.perf_counter()
+        for atoms in configs[1:]:
+            ms_dev, out = run(atoms)
+            dev_ms.append(ms_dev)
+        torch.cuda.synchronize()
+        wall = (time.perf_counter() - t0) / runs
+    adf = out["det"][2, 0]
+    return {"metric": "probe-slices/s, the reference's stem_STO_4x4.qsc (400^2, 20x20 scan, 40 slices)",
+            "value": round(len(ipos) * nslice / (np.mean(dev_ms) * 1e-3), 1), "unit": "probe-slices/s",
+            "runs": runs, "multislice_ms_per_run": round(float(np.mean(dev_ms)), 2),
+            "wall_s_per_run_with_potential": round(wall, 3), "atoms": int(len(configs[0])),
+            "adf_70_133_mean_t46.86": round(float(adf.mean()), 4)}
 
 
 def tem_workload(nslice=200, streams=0):
@@ -611,6 +666,7 @@ def main():
         ms.close()
         del trans
         result["uneven"] = uneven_workload(steps=2)
+        result["reference_config"] = reference_config_workload()
         result["tem"] = tem_workload(nslice=a.tem_slices)
     print(json.dumps(result), flush=True)
 
