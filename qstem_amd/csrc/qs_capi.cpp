@@ -93,6 +93,28 @@ void host_fft_pow2(std::vector<std::complex<double>>& a, int sign) {
 
 bool is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 
+// Unnormalised DFT of any length whose prime factors are small (recursive
+// decimation in time by the smallest prime factor), for host set-up tables.
+void host_fft_any(std::vector<std::complex<double>>& a, int sign) {
+  const int n = (int)a.size();
+  if (n <= 1) return;
+  int p = 2;
+  while (n % p) p++;
+  const int m = n / p;
+  std::vector<std::vector<std::complex<double>>> sub(p, std::vector<std::complex<double>>(m));
+  for (int r = 0; r < p; r++)
+    for (int k = 0; k < m; k++) sub[r][k] = a[r + (size_t)p * k];
+  for (auto& v : sub) host_fft_any(v, sign);
+  for (int k = 0; k < n; k++) {
+    std::complex<double> acc = 0.0;
+    for (int r = 0; r < p; r++) {
+      const double ang = sign * 2.0 * M_PI * (double)(((long long)r * k) % n) / n;
+      acc += sub[r][k % m] * std::complex<double>(cos(ang), sin(ang));
+    }
+    a[k] = acc;
+  }
+}
+
 double wavelength(double kev) {  // stem3/stemutil.cpp:714-725
   const double emass = 510.99906, hc = 12.3984244;
   return hc / sqrt(kev * (2.0 * emass + kev));
@@ -111,6 +133,9 @@ std::vector<float2> twiddles(int n) {
 // power-of-two Stockham transform or Bluestein through M = 2^k >= 2n-1.
 struct RowXform {
   int n = 0, M = 0;
+  bool gen = false;   // through the mixed-radix LDS passes (direct when M == 0, else Bluestein)
+  qs::GenPlan plan{};
+  int rpb = 0;
   DevBuf tw, chirp_f, vhat_f, chirp_b, vhat_b;
 };
 
@@ -273,15 +298,37 @@ static int set_mixed_shape(qs_ctx* c, int rpb, int tc, int vx, int vy) {
 
 static int setup_rowxform(qs_ctx* c, RowXform& r, int n) {
   r.n = n;
+  r.gen = false;
   if (is_pow2(n) && n >= 64 && n <= 4096) {
     r.M = 0;
     HIPCHK(c, upload(r.tw, twiddles(n), c->stream));
     return QS_OK;
   }
+  if (qs::gen_plan(n, &r.plan)) {  // prime factors 2, 3, 5, 7: direct mixed-radix passes
+    r.gen = true;
+    r.M = 0;
+    r.rpb = qs::gen_rows_per_block(n);
+    HIPCHK(c, upload(r.tw, twiddles(n), c->stream));
+    return QS_OK;
+  }
+  // Bluestein through a power of two M >= 2n-1 up to 4096 (the optimised fixed-plan
+  // transforms: measured faster than the LDS passes at the shorter length 2187 for
+  // n = 1088), else through the smallest length M >= 2n-1 with prime factors 2, 3, 5, 7
+  // that one block holds in LDS
   int M = 64;
   while (M < 2 * n - 1) M <<= 1;
-  if (M > 4096)
-    return fail(c, QS_ERR_UNSUPPORTED, "transmission size " + std::to_string(n) +
+  int Mg = 2 * n - 1;
+  qs::GenPlan pg{};
+  while (M > 4096 && Mg <= 8 * n && !qs::gen_plan(Mg, &pg)) Mg++;
+  const bool use_gen = M > 4096 && Mg <= 8 * n && qs::gen_plan(Mg, &pg);
+  if (use_gen) {
+    r.gen = true;
+    r.plan = pg;
+    r.rpb = qs::gen_rows_per_block(Mg);
+    M = Mg;
+  }
+  if (!r.gen && M > 4096)
+    return fail(c, QS_ERR_UNSUPPORTED, "transform length " + std::to_string(n) +
                                            " needs a Bluestein length above 4096 (only sizes <= 2048 or powers of two "
                                            "<= 4096 are supported)");
   r.M = M;
@@ -297,7 +344,8 @@ static int setup_rowxform(qs_ctx* c, RowXform& r, int n) {
     }
     v[0] = std::conj(cd[0]);
     for (int k = 1; k < n; k++) v[k] = v[M - k] = std::conj(cd[k]);
-    host_fft_pow2(v, -1);
+    if (is_pow2(M)) host_fft_pow2(v, -1);
+    else host_fft_any(v, -1);
     for (int k = 0; k < M; k++) vh[k] = make_float2((float)v[k].real(), (float)v[k].imag());
     HIPCHK(c, upload(dir < 0 ? r.chirp_f : r.chirp_b, ch, c->stream));
     HIPCHK(c, upload(dir < 0 ? r.vhat_f : r.vhat_b, vh, c->stream));
@@ -306,7 +354,20 @@ static int setup_rowxform(qs_ctx* c, RowXform& r, int n) {
 }
 
 static void run_rowxform(qs_ctx* c, RowXform& r, float2* data, long nrows, int dir, float scale) {
-  if (r.M == 0) qs::launch_rows_fft(data, nrows, r.n, dir, r.tw.as<float2>(), scale, c->stream);
+  if (r.gen) {
+    qs::GenRowXformArgs a{};
+    a.data = data;
+    a.nrows = nrows;
+    a.n = r.n;
+    a.M = r.M;
+    a.rpb = r.rpb;
+    a.scale = scale;
+    a.plan = r.plan;
+    a.tw = r.tw.as<float2>();
+    a.chirp = (dir < 0 ? r.chirp_f : r.chirp_b).as<float2>();
+    a.vhat = (dir < 0 ? r.vhat_f : r.vhat_b).as<float2>();
+    qs::launch_gen_rowxform(a, dir, c->stream);
+  } else if (r.M == 0) qs::launch_rows_fft(data, nrows, r.n, dir, r.tw.as<float2>(), scale, c->stream);
   else
     qs::launch_rows_bluestein(data, nrows, r.n, r.M, (dir < 0 ? r.chirp_f : r.chirp_b).as<float2>(),
                               (dir < 0 ? r.vhat_f : r.vhat_b).as<float2>(), r.tw.as<float2>(), scale, c->stream);

@@ -481,6 +481,73 @@ __global__ __launch_bounds__(kGenBlock) void k_gen_cols(GenColArgs a) {
   }
 }
 
+// Row transforms of rows of length n (contiguous, nrows of them) through the
+// mixed-radix LDS passes: a.rpb rows per block (a power of two), thread t on row
+// t / TPR and elements t % TPR + k TPR.
+//   direct (a.M == 0): length-n transform in direction DIR (n's plan), times scale;
+//   Bluestein (a.M > 0): chirp[k] x row, zero-padded to M, FFT_M, x vhat, IFFT_M,
+//   x chirp x scale/M (a.plan is M's; chirp[k] = exp(DIR i pi k^2/n), vhat = FFT_M of
+//   the wrapped conj(chirp)).
+template <int DIR>
+__global__ __launch_bounds__(kGenBlock) void k_gen_rowxform(GenRowXformArgs a) {
+  extern __shared__ float2 gsm[];
+  const int L = a.M > 0 ? a.M : a.n, ld = gen_ld(L);
+  const long r0 = (long)blockIdx.x * a.rpb;
+  const int V = (int)min((long)a.rpb, a.nrows - r0);
+  const int lt = 31 - __clz(kGenBlock / a.rpb);
+  const int vr = threadIdx.x >> lt, yt = threadIdx.x & ((1 << lt) - 1);
+  const bool rowok = vr < V;
+  float2* row = a.data + (r0 + (rowok ? vr : 0)) * a.n;
+  float2* lrow = gsm + vr * ld;
+  float2 sv[kGenPer];
+#pragma unroll
+  for (int k = 0; k < kGenPer; k++) {
+    const int y = yt + (k << lt);
+    sv[k] = make_float2(0.f, 0.f);
+    if (rowok && y < a.n) {
+      sv[k] = row[y];
+      if (a.M > 0) sv[k] = cmul(sv[k], a.chirp[y]);
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < kGenPer; k++) {
+    const int y = yt + (k << lt);
+    if (rowok && y < L) lrow[lp(y)] = sv[k];
+  }
+  __syncthreads();
+  float s = a.scale;
+  if (a.M > 0) {
+    gen_fft<-1>(gsm, V, ld, a.plan, a.tw);
+#pragma unroll
+    for (int k = 0; k < kGenPer; k++) {
+      const int y = yt + (k << lt);
+      if (rowok && y < L) lrow[lp(y)] = cmul(lrow[lp(y)], a.vhat[y]);
+    }
+    __syncthreads();
+    gen_fft<+1>(gsm, V, ld, a.plan, a.tw);
+    s /= (float)a.M;
+  } else {
+    gen_fft<DIR>(gsm, V, ld, a.plan, a.tw);
+  }
+#pragma unroll
+  for (int k = 0; k < kGenPer; k++) {
+    const int y = yt + (k << lt);
+    if (rowok && y < a.n) {
+      float2 v = lrow[lp(y)];
+      if (a.M > 0) v = cmul(v, a.chirp[y]);
+      row[y] = cscale(v, s);
+    }
+  }
+}
+
+void launch_gen_rowxform(const GenRowXformArgs& a, int dir, hipStream_t st) {
+  const int L = a.M > 0 ? a.M : a.n;
+  const unsigned grid = (unsigned)((a.nrows + a.rpb - 1) / a.rpb);
+  const size_t lds = sizeof(float2) * a.rpb * gen_ld(L);
+  if (dir < 0) hipLaunchKernelGGL(k_gen_rowxform<-1>, dim3(grid), dim3(kGenBlock), lds, st, a);
+  else hipLaunchKernelGGL(k_gen_rowxform<+1>, dim3(grid), dim3(kGenBlock), lds, st, a);
+}
+
 bool gen_plan(int n, GenPlan* pl) { return gen_plan_variant(n, 0, pl); }
 
 bool gen_plan_variant(int n, int variant, GenPlan* pl) {

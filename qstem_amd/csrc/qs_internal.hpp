@@ -201,6 +201,19 @@ bool gen_plan_variant(int n, int variant, GenPlan* pl);
 int gen_rows_per_block(int ny);   // default rows per block of k_gen_rows (power of two)
 int gen_cols_per_tile(int nx);    // default columns per tile of k_gen_cols (power of two)
 int gen_max_vectors(int n);       // most vectors of length n one block holds in LDS
+// Row transforms through the mixed-radix LDS passes (qs_generic.hip): length n
+// directly (M == 0, plan of n) or Bluestein through a smooth length M (plan of M).
+struct GenRowXformArgs {
+  float2* data;            // [nrows][n]
+  long nrows;
+  int n, M, rpb;           // rpb: rows per block, a power of two (gen_rows_per_block of n or M)
+  float scale;
+  GenPlan plan;
+  const float2* tw;        // W_L^m, L = M or n
+  const float2* chirp;     // Bluestein: [n], exp(DIR i pi k^2/n)
+  const float2* vhat;      // Bluestein: [M]
+};
+void launch_gen_rowxform(const GenRowXformArgs& a, int dir, hipStream_t st);
 void launch_gen_rows(const GenRowArgs& a, hipStream_t st);
 void launch_gen_cols(const GenColArgs& a, int ntiles, hipStream_t st);
 

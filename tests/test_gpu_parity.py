@@ -50,10 +50,12 @@ def test_probe_options_match_oracle(oracle, n, opts):
 
 
 @pytest.mark.parametrize("pot", [(160, 160), (128, 128), (200, 136), (256, 128), (128, 512), (1024, 2048),
-                                 (2048, 1024)])
+                                 (2048, 1024), (2056, 136)])
 def test_transmission_matches_oracle(oracle, pot):
-    """Power-of-two grids take the band-compact sweeps, the others row transforms
-    (Bluestein where needed) and transposes."""
+    """Power-of-two grids take the band-compact sweeps, the others row transforms and
+    transposes: mixed-radix LDS passes for lengths with prime factors 2, 3, 5, 7 (160,
+    200), Bluestein through a power of two up to 4096 (136 = 8*17), and beyond that
+    Bluestein through the LDS passes (2056 = 8*257: length 4116 = 4*3*7^3)."""
     cfg = small_config(nx=128, ny=128, pad=0)
     cfg.potNx, cfg.potNy = pot
     V = smooth_potential(3, pot[0], pot[1], seed=3)
