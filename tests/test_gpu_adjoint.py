@@ -181,7 +181,7 @@ def test_torch_autograd_function(oracle, device):
         ms.set_potential(V)
         ms.make_probe()
         ref = ms.adjoint(ipos, target_np)
-        assert abs(float(loss) - ref["loss"]) <= 1e-6 * ref["loss"]
+        assert abs(float(loss.detach()) - ref["loss"]) <= 1e-6 * ref["loss"]
         assert Vt.grad.device == Vt.device
         assert np.allclose(Vt.grad.cpu().numpy(), ref["grad_V"], rtol=1e-5, atol=1e-6 * np.abs(ref["grad_V"]).max())
         assert abs(float(df.grad) - ref["grad_df0"]) <= 1e-5 * abs(ref["grad_df0"])
@@ -301,3 +301,43 @@ def test_band_limit_overlap_keeps_gradients(nx, slices):
         assert abs(o["loss"] / ref["loss"] - 1) < 1e-6, key
         g0 = ref["grad_V"]
         assert np.linalg.norm(o["grad_V"] - g0) <= 1e-6 * np.linalg.norm(g0), key
+
+
+def test_atoms_autograd_function():
+    """atoms_multislice_loss: dL/d(x, y) through torch autograd equals qs_atom_gradients
+    after qs_adjoint on the same build, and a small step along -grad lowers the loss
+    (the target is the pattern of the atoms shifted by 0.05 A)."""
+    import torch
+    from qstem_amd import Multislice
+    from qstem_amd.autograd import atoms_multislice_loss
+    cfg = _cfg(1, nx=128, pad=24, slices=4)
+    cfg.sliceThickness = 2.0
+    rng = np.random.default_rng(5)
+    n_atoms = 8
+    xyz = np.column_stack([rng.uniform(2.0, 5.5, n_atoms), rng.uniform(2.0, 5.5, n_atoms),
+                           rng.uniform(0.5, 7.5, n_atoms)])
+    Z = np.array([8, 22, 38, 8, 22, 38, 8, 22])
+    atoms = np.column_stack([xyz, np.full(n_atoms, 0.5), np.ones(n_atoms), np.zeros(n_atoms), Z])
+    atoms = atoms[np.argsort(atoms[:, 2], kind="stable")]
+    ipos = np.array([[0, 0], [12, 8], [24, 24]], np.int32)
+    R = 2.5
+    with Multislice(cfg) as ms:
+        shifted = atoms.copy()
+        shifted[:, 0] += 0.05
+        ms.build_potential(shifted, atomRadius=R)
+        ms.make_probe()
+        ex = ms.run_stem(ipos, want_exit=True, want_det=False)["exit"]
+        target = (np.abs(np.fft.fft2(ex.astype(np.complex128), axes=(1, 2)) / (128 * 128)) ** 2).astype(np.float32)
+        xy = torch.tensor(atoms[:, :2], dtype=torch.float64, requires_grad=True)
+        loss = atoms_multislice_loss(xy, ms, atoms, ipos, target, atomRadius=R)
+        loss.backward()
+        ms.build_potential(atoms, atomRadius=R)
+        ms.make_probe()
+        ref = ms.adjoint(ipos, target)
+        g_ref = ms.atom_gradients(atoms)[:, :2]
+        assert abs(float(loss.detach()) - ref["loss"]) <= 1e-6 * ref["loss"]
+        assert np.allclose(xy.grad.numpy(), g_ref, rtol=1e-5, atol=1e-6 * np.abs(g_ref).max())
+        step = 0.2 * float(loss.detach()) / float((xy.grad ** 2).sum())
+        xy2 = (xy - step * xy.grad).detach()
+        loss2 = atoms_multislice_loss(xy2, ms, atoms, ipos, target, atomRadius=R)
+        assert float(loss2) < float(loss.detach())
