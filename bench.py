@@ -137,7 +137,7 @@ def uneven_workload(nconfig=16, steps=3, streams=0):
             "adf_mean_per_size": {str(n): float(np.mean(o["det"][-1, 1])) for n, o in zip(ctxs, outs)}}
 
 
-def reference_config_workload(runs=3):
+def reference_config_workload(runs=3, cpu_probes=64):
     """The reference's own example, stem3/debug_data/stem_STO_4x4.qsc, through the
     library as qstem3 runs it: charged SrTiO3 (Sr +2, Ti +4, O -2; Debye-Waller factors
     of SrTiO3.cfg) 9 x 9 x 20 cells, 400^2 wave at 0.0625 A (mixed-radix path), 200 kV,
@@ -145,7 +145,9 @@ def reference_config_workload(runs=3):
     annular detectors, outputs every 8 slices, Einstein phonons at 300 K, `runs`
     configurations, each built on the GPU (ion-offset LUTs) and run over all 400
     positions.  Returns probe-slices/s of the multislice (device time) and the wall
-    time per configuration including the potential build."""
+    time per configuration including the potential build, and the plain CPU FP32
+    multislice (oracle/qs_cpu32.c, one probe per OpenMP thread as doSTEM) on the last
+    configuration's transmission functions for cpu_probes of the scan positions."""
     import torch
     from qstem_amd import Detector, Multislice, StemConfig, record_bins, scan_offsets
     from qstem_amd.specimen import build_supercell, perovskite_cell
@@ -182,12 +184,26 @@ def reference_config_workload(runs=3):
             dev_ms.append(ms_dev)
         torch.cuda.synchronize()
         wall = (time.perf_counter() - t0) / runs
+        cpu = None
+        if cpu_probes > 0:
+            trans = ms.get_trans().astype(np.complex128)
+            psi0 = ms.make_probe().astype(np.complex128)
+            model, ncpu = cpu_info()
+            threads = min(ncpu, cpu_probes)
+            sel = ipos[np.linspace(0, len(ipos) - 1, cpu_probes).astype(int)]
+            rate, cwall, cdet = cpu32_rate(cfg, trans, psi0, sel, bins, nbins, threads)
+            ref = ms.run_stem(sel, bins=bins, nbins=nbins)["det"]
+            err = float(np.max([np.linalg.norm(cdet[b, d] - ref[b, d]) / max(np.linalg.norm(ref[b, d]), 1e-30)
+                                for b in range(nbins) for d in range(len(dets))]))
+            cpu = {"value": round(rate, 1), "unit": "probe-slices/s", "cores": threads, "kind": "port",
+                   "sample": "%d of the 400 positions x 40 slices (%.2f s wall)" % (cpu_probes, cwall),
+                   "cpu_model": model, "max_rel_err_gpu_vs_cpu": err}
     adf = out["det"][2, 0]
     return {"metric": "probe-slices/s, the reference's stem_STO_4x4.qsc (400^2, 20x20 scan, 40 slices)",
             "value": round(len(ipos) * nslice / (np.mean(dev_ms) * 1e-3), 1), "unit": "probe-slices/s",
             "runs": runs, "multislice_ms_per_run": round(float(np.mean(dev_ms)), 2),
             "wall_s_per_run_with_potential": round(wall, 3), "atoms": int(len(configs[0])),
-            "adf_70_133_mean_t46.86": round(float(adf.mean()), 4)}
+            "adf_70_133_mean_t46.86": round(float(adf.mean()), 4), "cpu_baseline": cpu}
 
 
 def tem_workload(nslice=200, streams=0):

@@ -7,8 +7,12 @@ import pytest
 from cases import probe_positions, rel_l2, small_config, smooth_potential
 
 
-@pytest.mark.parametrize("shape,slices", [((128, 128), 6), ((256, 256), 3), ((64, 128), 4)])
+@pytest.mark.parametrize("shape,slices", [((128, 128), 6), ((256, 256), 3), ((64, 128), 4), ((400, 400), 2),
+                                          ((90, 150), 3), ((48, 120), 2)])
 def test_cpu32_matches_oracle(oracle, shape, slices):
+    """Powers of two, and the mixed-radix sizes (radix 3 and 5 passes, and row and
+    column blocks narrower than 16 and 32 at the edge: 400 = 4*4*5*5, 90 = 2*3*3*5,
+    150 = 2*3*5*5, 48 = 4*4*3, 120 = 4*2*3*5)."""
     nx, ny = shape
     cfg = small_config(nx=nx, ny=ny, pad=24, slices=slices, dx=0.03)
     V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=5)
