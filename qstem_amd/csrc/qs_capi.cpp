@@ -573,6 +573,11 @@ int qs_create(const qs_params* p, qs_ctx** out) {
       while (bb < pny / 2 - 1 && ky2t[pny - 1 - bb] < k2m) bb++;
       bool contiguous = ba >= 0;
       for (int i = 0; i < pny && contiguous; i++) contiguous = ((i <= ba) || (i >= pny - bb)) == (ky2t[i] < k2m);
+      // the sweep kernels never compute the slots inside (N/3, N - N/3); with Q1 and
+      // by > potSizeY the negative-ky half of the band reaches into them, and the
+      // full-transform band limit below takes over
+      const int third = pny / 3;
+      contiguous = contiguous && ba <= third && bb <= third;
       if (contiguous) {
         c->bl.band_a = ba;
         c->bl.band_b = bb;

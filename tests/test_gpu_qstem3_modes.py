@@ -233,3 +233,88 @@ def test_qstem3_sequence_repeats(oracle, tmp_path):
             img = read_img(str(out / name))
             e = rel_l2(img.data, want[t, di].reshape(4, 3))
             assert e < TOL, (name, e)
+
+
+
+def _small_oracle(oracle, pot, offx, offy, dets=()):
+    """stem_small.qsc's specimen (STO_neutral.cfg 4x4x3 cells, 6 slices, 128^2 at 0.05 A,
+    radius 3 A, band-limited transmission): config and transmission functions."""
+    from qstem_amd import StemConfig
+    from qstem_amd.specimen import build_supercell, read_cfg
+    atoms, ext = build_supercell(read_cfg(os.path.join(GOLD, "STO_neutral.cfg")), 4, 4, 3)
+    cfg = StemConfig(nx=128, ny=128, resolutionX=0.05, resolutionY=0.05, v0=200.0, slices=6, sliceThickness=1.9525,
+                     potNx=pot, potNy=pot, by=float(f(ext[1])), bandlimittrans=1, Cs=float(f(f(0.001) * 1e7)),
+                     df0=float(f(10.0 * f(-5.0))), alpha=20.0, detectors=list(dets), electronScale=1.0, compat=3)
+    g = oracle.lut_geometry(3.0, cfg.resolutionX, cfg.sliceThickness)
+    keys = []
+    for a in atoms:
+        if int(a[6]) not in [k[0] for k in keys]:
+            keys.append((int(a[6]), float(f(a[3]))))
+    luts = oracle.atom_luts(g, [k[0] for k in keys], [k[1] for k in keys], cfg.sliceThickness)
+    kinds = [[k[0] for k in keys].index(int(a[6])) for a in atoms]
+    V = oracle.make_slices(cfg, g, luts, kinds, [tuple(a) for a in atoms], offx, offy, atomRadius=3.0)
+    return cfg, oracle.make_trans(cfg, V), ext
+
+
+def _run_small(tmp_path, mode, extra):
+    """stem_small.qsc in another mode; extra "key: value" lines replace the file's line
+    of that key (readparam reads the first match) or are appended."""
+    shutil.copy(os.path.join(GOLD, "STO_neutral.cfg"), tmp_path / "STO_neutral.cfg")
+    keys = dict(line.split(":", 1) for line in extra)
+    keys["mode"] = " " + mode
+    _write_qsc(os.path.join(GOLD, "stem_small.qsc"), tmp_path / "run.qsc", **{k: v.strip() for k, v in keys.items()})
+    r = subprocess.run([EXE, "run.qsc"], cwd=tmp_path, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    return tmp_path / "out_small"
+
+
+def test_qstem3_tem_tilted_beam(oracle, tmp_path):
+    """TEM with beam tilt and tilt back (doTEM, stem3.cpp:2403-2420, 2494-2512): the
+    incident wave exp(i (ktx x + kty y)), kt = 2 pi sin(tilt)/lambda, with the
+    reference's y = resolutionY (ix - nx/2), and the exit wave.img multiplied by cos
+    (real part) and sin (imaginary part) of the back-tilt phase, as the reference
+    writes them; wave_t.img at slices 1, 3, 5 before the back tilt."""
+    from qstem_amd.imgio import read_img
+    out = _run_small(tmp_path, "TEM", ["Beam tilt X: 0.2 deg", "Beam tilt Y: 0.1 deg", "Tilt back: yes"])
+    cx = f(f(f(4 * 3.905)) / f(2.0))
+    off = f(np.float64(cx) - 0.5 * np.float64(f(128 * f(0.05))))
+    cfg, trans, _ = _small_oracle(oracle, 128, float(off), float(off))
+    lam = oracle.wavelength(200.0)
+    pi = 3.1415926535897  # the driver's constant, as stem3.cpp's
+    tx, ty = f(np.float64(f(0.2)) * pi / 180.0), f(np.float64(f(0.1)) * pi / 180.0)  # "deg" values in float
+    ktx, kty = 2 * pi * np.sin(np.float64(tx)) / lam, 2 * pi * np.sin(np.float64(ty)) / lam
+    ix = np.arange(128)
+    xq = (f(0.05) * (ix - 64).astype(np.float32)).astype(np.float64)
+    phase = ktx * xq[:, None] + kty * xq[:, None] * np.ones((1, 128))  # y from ix, as the reference
+    psi0 = np.cos(phase).astype(np.float32) + 1j * np.sin(phase).astype(np.float32)
+    res = _pieces(oracle, cfg, trans, psi0.astype(np.complex128), [2, 4, 6])
+    for t, (psi, _) in enumerate(res):
+        w = read_img(str(out / ("wave_%d.img" % t)))
+        assert rel_l2(w.data, psi) < TOL, (t, rel_l2(w.data, psi))
+    back = -ktx * xq[:, None] - kty * xq[:, None] * np.ones((1, 128))
+    exit_ref = res[-1][0].real * np.cos(back) + 1j * res[-1][0].imag * np.sin(back)
+    w = read_img(str(out / "wave.img"))
+    assert rel_l2(w.data, exit_ref) < TOL
+
+
+def test_qstem3_cbed_save_level_2(oracle, tmp_path):
+    """CBED with save level 2: per slice, collectIntensity's diff_t.img of the k-space
+    pattern is overwritten by the second collectIntensity call on the real-space wave
+    (stemlib.cpp:2476-2482), so diff_t.img ends as fftshift(|psi|^2)/sqrt(N) of the
+    wave after the bin's last slice; wave_t.img holds that wave."""
+    from qstem_amd.imgio import read_img
+    out = _run_small(tmp_path, "CBED", ["save level: 2"])
+    x0 = f(5.8575)
+    off = f(np.float64(x0) - 0.5 * 128 * np.float64(f(0.05)))
+    cfg, trans, _ = _small_oracle(oracle, 128, float(off), float(off))
+    psi0 = oracle.probe(cfg, at=(float(x0 - off), float(x0 - off)))
+    res = _pieces(oracle, cfg, trans, psi0, [2, 4, 5, 6])  # bins: slices 1, 3, 4 (Q4), 5
+    for t, k in ((0, 0), (1, 1), (2, 3)):  # interimWave: slices 1, 3, 5 (slice 5 is t = 5 // 2 = 2)
+        w = read_img(str(out / ("wave_%d.img" % t)))
+        assert rel_l2(w.data, res[k][0]) < TOL, (t, rel_l2(w.data, res[k][0]))
+    for t in range(4):
+        psi = res[t][0]
+        d = read_img(str(out / ("diff_%d.img" % t))).data
+        real_int = np.fft.fftshift(np.abs(psi) ** 2) / 128.0
+        # an intensity carries twice the wave's relative error
+        assert rel_l2(d, real_int) < 2 * TOL, (t, rel_l2(d, real_int))
