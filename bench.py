@@ -436,10 +436,13 @@ def main():
     ap.add_argument("--no-extra", action="store_true", help="skip the uneven-batch and TEM records")
     ap.add_argument("--no-graph", action="store_true", help="time eager launches only")
     ap.add_argument("--tem-slices", type=int, default=200, help="slices of the tem workload")
-    ap.add_argument("--workload", default="batched", choices=["batched", "uneven", "tem", "copy", "copyrate"],
+    ap.add_argument("--workload", default="batched",
+                    choices=["batched", "uneven", "tem", "reference", "copy", "copyrate"],
                     help="batched: the headline line; uneven: frozen-phonon uneven batch; tem: 4096^2 plane wave "
-                         "with the checkpointed adjoint (own JSON lines); copy: a plain 512 MiB device copy "
-                         "(calibrates the FETCH_SIZE / WRITE_SIZE counters); copyrate: plain device copy rates")
+                         "with the checkpointed adjoint; reference: stem_STO_4x4.qsc's 400^2 scan (mixed-radix "
+                         "path; --cpu-sample positions for the CPU leg, 0 = skip) (own JSON lines); copy: a plain "
+                         "512 MiB device copy (calibrates the FETCH_SIZE / WRITE_SIZE counters); copyrate: plain "
+                         "device copy rates")
     ap.add_argument("--rows-per-wave", type=int, default=0, help="with --group: row-sweep steps per wavefront")
     ap.add_argument("--tiles-per-block", type=int, default=0, help="with --group: column tiles per block")
     ap.add_argument("--streams", type=int, default=0, choices=[0, 1, 2],
@@ -450,6 +453,9 @@ def main():
               file=sys.stderr)
         sys.exit(2)
 
+    if a.workload == "reference":
+        print(json.dumps(reference_config_workload(cpu_probes=a.cpu_sample)), flush=True)
+        return
     if a.workload == "uneven":
         print(json.dumps(uneven_workload(streams=a.streams)), flush=True)
         return
