@@ -197,6 +197,7 @@ struct qs_ctx {
   int gvx = 0, gvy = 0;  // radix plan variants of gpx, gpy
   DevBuf gtiles;
   int gntiles = 0, gtc = 0, grpb = 0;
+  int gkc = 0;  // entries of gtiles (tiles, or in-band columns for the compiled column kernel)
   // band limit of the transmission functions as band-compact sweeps (potNx and
   // potNy powers of two): row FFT into the ky band, column FFT / disk mask with
   // 1/(potNx potNy) / column IFFT, row IFFT; else row transforms and transposes
@@ -265,28 +266,35 @@ static hipError_t upload(DevBuf& b, const std::vector<T>& v, hipStream_t st) {
 
 // Launch shape of the mixed-radix path: rows per block of k_gen_rows, columns per
 // tile of k_gen_cols (powers of two; 0 = the default, the most that fit the block's
-// LDS, at most 16) and the radix plan variants of x and y; rebuilds the list of
-// column tiles that hold an in-band ky column.
+// LDS, at most 16) and the radix plan variants of x and y.  Variant
+// kGenPlanCompiled selects the wavefront-resident kernels of a compiled plan for
+// that axis (rows: rpb = rows per transform slot, a power of two up to 16, default
+// 2; columns: tc unused).  Rebuilds the column work list: tiles of tc columns that
+// hold an in-band ky column, or, for the compiled column kernel, the in-band ky
+// columns themselves.
 static int set_mixed_shape(qs_ctx* c, int rpb, int tc, int vx, int vy) {
   const int nx = c->p.nx, ny = c->p.ny;
-  if (rpb == 0) rpb = qs::gen_rows_per_block(ny);
+  const bool cx = vx == qs::kGenPlanCompiled, cy = vy == qs::kGenPlanCompiled;
+  if (rpb == 0) rpb = cy ? 2 : qs::gen_rows_per_block(ny);
   if (tc == 0) tc = qs::gen_cols_per_tile(nx);
   qs::GenPlan px{}, py{};
   if ((rpb & (rpb - 1)) || (tc & (tc - 1)) || rpb < 1 || tc < 1 || rpb > 16 || tc > 16 ||
-      rpb > qs::gen_max_vectors(ny) || tc > qs::gen_max_vectors(nx) || !qs::gen_plan_variant(nx, vx, &px) ||
+      (!cy && rpb > qs::gen_max_vectors(ny)) || tc > qs::gen_max_vectors(nx) || !qs::gen_plan_variant(nx, vx, &px) ||
       !qs::gen_plan_variant(ny, vy, &py))
     return fail(c, QS_ERR_ARG, "mixed-radix launch shape: rows per block and columns per tile must be powers of "
-                               "two up to 16 that fit the block's LDS, plan variants 0-2");
-  std::vector<int> tiles;  // tiles of tc columns with a ky column inside the band
-  for (int y0 = 0; y0 < ny; y0 += tc) {
+                               "two up to 16 that fit the block's LDS, plan variants 0-2 (3: compiled plans only)");
+  const int w = cx ? 1 : tc;
+  std::vector<int> tiles;  // tiles of w columns with a ky column inside the band
+  for (int y0 = 0; y0 < ny; y0 += w) {
     bool any = false;
-    for (int y = y0; y < std::min(ny, y0 + tc); y++) any = any || c->ky2[y] < c->k2max;
+    for (int y = y0; y < std::min(ny, y0 + w); y++) any = any || c->ky2[y] < c->k2max;
     if (any) tiles.push_back(y0);
   }
   if (tiles.empty()) return fail(c, QS_ERR_ARG, "no ky column inside the band");
   HIPCHK(c, upload(c->gtiles, tiles, c->stream));
   HIPCHK(c, hipStreamSynchronize(c->stream));
-  c->gntiles = (int)tiles.size();
+  c->gkc = (int)tiles.size();
+  c->gntiles = cx ? (c->gkc + qs::mix_cols_per_block(nx) - 1) / qs::mix_cols_per_block(nx) : c->gkc;
   c->grpb = rpb;
   c->gtc = tc;
   c->gpx = px;
@@ -539,7 +547,11 @@ int qs_create(const qs_params* p, qs_ctx** out) {
     if (rc != QS_OK) return bad(rc);
     c->mixed = qs::gen_plan(nx, &c->gpx) && qs::gen_plan(ny, &c->gpy);
     if (c->mixed) {
-      const int rc = set_mixed_shape(c, 0, 0, 0, 0);
+      // compiled plans by default; QSTEM_COMPILED_PLANS=0 keeps the runtime-planned kernels
+      const char* cp = getenv("QSTEM_COMPILED_PLANS");
+      const bool comp = !(cp && strcmp(cp, "0") == 0);
+      const int rc = set_mixed_shape(c, 0, 0, comp && qs::mix_compiled(nx) ? qs::kGenPlanCompiled : 0,
+                                     comp && qs::mix_compiled(ny) ? qs::kGenPlanCompiled : 0);
       if (rc != QS_OK) return bad(rc);
     }
   }
@@ -1558,7 +1570,7 @@ static int run_mixed(qs_ctx* c, const int* ipos, const int* config, int nprobe, 
   qs::GenColArgs ca{};
   ca.nx = nx;
   ca.ny = ny;
-  ca.tc = c->gtc;
+  ca.tc = c->gvx == qs::kGenPlanCompiled ? c->gkc : c->gtc;
   ca.state = c->state.as<float2>();
   ca.tiles = c->gtiles.as<int>();
   ca.px = c->px.as<float2>();
@@ -1584,12 +1596,14 @@ static int run_mixed(qs_ctx* c, const int* ipos, const int* config, int nprobe, 
     for (int s = 0; s < nslice; s++) {
       ra.mode = (s == 0 && start == 0) ? 0 : 1;
       ra.trans = trans_base(c) + (size_t)(slice0 + s) * slice;
-      qs::launch_gen_rows(ra, st);
+      if (c->gvy == qs::kGenPlanCompiled) qs::launch_mix_rows(ra, st);
+      else qs::launch_gen_rows(ra, st);
       const int bin = want_det && bins ? bins[s] : -1;
       const bool last = s == nslice - 1, tot = last && int_intensity;
       ca.rec = (bin >= 0 || tot) ? 1 : 0;
       ca.diff = (last && diff) ? c->diff.as<float>() : nullptr;
-      qs::launch_gen_cols(ca, c->gntiles, st);
+      if (c->gvx == qs::kGenPlanCompiled) qs::launch_mix_cols(ca, c->gntiles, st);
+      else qs::launch_gen_cols(ca, c->gntiles, st);
       if (ca.rec)
         qs::launch_reduce_partials(ca.partial, nb, c->gntiles, bin >= 0 ? p.ndet : 0, p.electronScale,
                                    bin >= 0 ? c->det.as<float>() + (size_t)bin * p.ndet * nprobe : nullptr, nprobe,
@@ -1597,7 +1611,8 @@ static int run_mixed(qs_ctx* c, const int* ipos, const int* config, int nprobe, 
     }
     if (exit_wave) {  // the real-space exit waves: y-IFFT of the state, no transmit
       ra.mode = 2;
-      qs::launch_gen_rows(ra, st);
+      if (c->gvy == qs::kGenPlanCompiled) qs::launch_mix_rows(ra, st);
+      else qs::launch_gen_rows(ra, st);
     }
   }
   HIPCHK(c, hipGetLastError());
@@ -2333,7 +2348,7 @@ int qs_atom_gradients(qs_ctx* c, const qs_atom* atoms, int natom, const qs_pot_p
 // are ignored).
 namespace {
 // bumped whenever the sweep kernels or their launch parameters change meaning
-const char* const kTuneVersion = "sweeps-3";
+const char* const kTuneVersion = "sweeps-4";
 struct TuneKey {
   std::string device;  // gcnArchName / name / compute units
   int nx, ny, kcp, potNy, nstreams;
@@ -2477,17 +2492,23 @@ int qs_autotune(qs_ctx* c, const int* ipos, int nprobe, int* batch, int* rows_pe
     if (c->mixed) {
       for (int rw : {1, 2, 4, 8, 16})
         if (rc == QS_OK && rw <= qs::gen_max_vectors(c->p.ny)) rc = consider(B0, rw, best.tpb);
-      for (int tc : {2, 4, 8, 16})
-        if (rc == QS_OK && tc <= qs::gen_max_vectors(c->p.nx)) rc = consider(B0, best.rw, tc);
+      for (int tc : {2, 4, 8, 16})  // (the compiled column kernel has no tiles)
+        if (rc == QS_OK && vx != qs::kGenPlanCompiled && tc <= qs::gen_max_vectors(c->p.nx))
+          rc = consider(B0, best.rw, tc);
+      qs::GenPlan pl{};
       for (int v = 0; v < qs::kGenPlanVariants && rc == QS_OK; v++) {  // x plan, then y plan
+        if (!qs::gen_plan_variant(c->p.nx, v, &pl)) continue;
         vx = v;
         vy = best.vy;
         rc = consider(B0, best.rw, best.tpb);
       }
       for (int v = 0; v < qs::kGenPlanVariants && rc == QS_OK; v++) {
+        if (!qs::gen_plan_variant(c->p.ny, v, &pl)) continue;
         vx = best.vx;
         vy = v;
-        rc = consider(B0, best.rw, best.tpb);
+        // rows per block of the runtime kernels and rows per slot of the compiled one
+        // differ in meaning: each plan with its own default
+        rc = consider(B0, (v == qs::kGenPlanCompiled) != (best.vy == qs::kGenPlanCompiled) ? 0 : best.rw, best.tpb);
       }
     } else {
       for (int rw : {1, 2, 4, 8})

@@ -191,6 +191,7 @@ __device__ __forceinline__ void dft_any(float2* a) {
   else if constexpr (R == 5) dft5<DIR>(a);
   else if constexpr (R == 7) dft7<DIR>(a);
   else if constexpr (R == 9) dft_2stage<3, 3, DIR>(a);
+  else if constexpr (R == 20) dft_2stage<4, 5, DIR>(a);
   else if constexpr (R == 25) dft_2stage<5, 5, DIR>(a);
   else dft<R, DIR>(a);
 }
@@ -548,10 +549,298 @@ void launch_gen_rowxform(const GenRowXformArgs& a, int dir, hipStream_t st) {
   else hipLaunchKernelGGL(k_gen_rowxform<+1>, dim3(grid), dim3(kGenBlock), lds, st, a);
 }
 
+// ============================================================================
+// Wavefront-resident mixed-radix sweeps (compiled plans).  The runtime-planned
+// kernels above share every Stockham pass among all threads of a block: two block
+// barriers per pass, and threads idle wherever the radix leaves the butterfly
+// count short of the block (half of them in the radix-25 pass of 400).  For a
+// length with a compiled two-pass plan N = R0 x R1, a transform is owned by T = N/E
+// lanes of one wavefront, each holding E values in registers (element t + T m in
+// v[m], the layout of coalesced row loads); the one exchange between the passes
+// goes through the transform's own LDS slot and needs wavefront-level ordering
+// only.  64/T transforms share a wavefront; lanes past them mirror the last
+// transform's lanes (same inputs, same values written to the same LDS words) and
+// store nothing.  State layout, modes and outputs are those of k_gen_rows and
+// k_gen_cols; the column kernel reads its column straight from the state rows
+// (the three columns of a wavefront are adjacent, so each row access is one
+// 24-byte piece).
+
+// E values per lane, passes of radix R0 then R1; LD: float2 per transform slot in LDS
+template <int N> struct MixPlan { static constexpr int E = 0; };
+template <> struct MixPlan<400> { static constexpr int E = 20, R0 = 20, R1 = 20, LD = 420; };
+
+namespace {
+
+constexpr int kMixWaves = 4;  // wavefronts per block
+template <int N> constexpr int mix_slots() { return 64 / (N / MixPlan<N>::E); }  // transforms per wavefront
+template <int N> constexpr int mix_ld() { return MixPlan<N>::LD; }
+// LDS position of the value between the passes with Stockham index i = j R0 + k
+// (butterfly j of the first pass, its output k): k (N/R0 + 1) + j, the first pass's
+// outputs transposed with one float2 of padding per row.  At 400 (20 x 20, three
+// transforms per wavefront, slot stride 420 = 4 mod 32) the first pass's writes (lanes
+// of a transform on consecutive j) and the second pass's reads (stride 21, odd) fall on
+// distinct eight-byte bank pairs in every 16-lane write group and 32-lane read group,
+// across the three transforms of a wavefront too; four blocks of twelve slots fit the
+// CU's 160 KB.
+template <int N>
+__device__ __forceinline__ int mix_pos(int i) {
+  constexpr int R0 = MixPlan<N>::R0;
+  return (i % R0) * (N / R0 + 1) + i / R0;
+}
+
+// Buffer-resource access (MUBUF): a per-lane byte offset plus a wavefront-uniform
+// one, so that the E strided column elements of a lane need one address register.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t mix_rsrc(const void* base, long bytes) {
+  const int n = bytes < 0x7fffffffL ? (int)bytes : 0x7fffffff;
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, n, 0x00020000);
+}
+__device__ __forceinline__ float2 mix_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ void mix_store(float2 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
+  using U2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0u, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, voff, soff, 0);
+}
+
+// Length-N transform of v (element t + T m in v[m], in and out) in two Stockham
+// passes: radix R0 over registers into the LDS slot (butterfly j = t + T b reads
+// elements j + k N/R0 and writes j R0 + k), then radix R1 from the slot (reads
+// j + k N/R1, twiddles W_N^(k (j mod R0)) from the table tw = W_N^m, m < N (k (j mod
+// R0) < N), writes j + k R0 = t + T (b + k NB1)).
+template <int N, int DIR>
+__device__ __forceinline__ void mix_fft(float2* v, float2* slot, const float2* __restrict__ tw, int t) {
+  using P = MixPlan<N>;
+  constexpr int E = P::E, T = N / E, R0 = P::R0, R1 = P::R1, NB0 = E / R0, NB1 = E / R1;
+  static_assert(R0 * R1 == N && E % R0 == 0 && E % R1 == 0 && R0 == T * NB1, "two-pass plan");
+  float2 a[NB0][R0];
+#pragma unroll
+  for (int b = 0; b < NB0; b++) {
+#pragma unroll
+    for (int k = 0; k < R0; k++) a[b][k] = v[b + k * NB0];
+    dft_any<R0, DIR>(a[b]);
+  }
+  wave_lds_sync();  // the slot's earlier reads are done
+#pragma unroll
+  for (int b = 0; b < NB0; b++) {
+    const int j = t + T * b;
+#pragma unroll
+    for (int k = 0; k < R0; k++) slot[mix_pos<N>(j * R0 + k)] = a[b][k];
+  }
+  wave_lds_sync();
+#pragma unroll
+  for (int b = 0; b < NB1; b++) {
+    const int j = t + T * b;
+    // opaque per call: the twiddle loads are neither hoisted out of the callers'
+    // loops nor shared between the two transforms of a kernel (registers held)
+    int jm = j % R0;
+    asm volatile("" : "+v"(jm));
+    float2 c[R1];
+#pragma unroll
+    for (int k = 0; k < R1; k++) c[k] = slot[mix_pos<N>(j + k * (N / R1))];
+    // w^k loaded for k < 4 and k a multiple of 4, the others as w^(k & ~3) w^(k & 3)
+    // (7 loads instead of 19 for radix 20: fewer registers in flight)
+    float2 pw[R1];
+#pragma unroll
+    for (int k = 1; k < R1; k++)
+      if (k < 4 || (k & 3) == 0) pw[k] = tw[k * jm];
+#pragma unroll
+    for (int k = 5; k < R1; k++)
+      if (k & 3) pw[k] = cmul(pw[k & ~3], pw[k & 3]);
+#pragma unroll
+    for (int k = 1; k < R1; k++) c[k] = DIR < 0 ? cmul(c[k], pw[k]) : cmulc(c[k], pw[k]);
+    dft_any<R1, DIR>(c);
+#pragma unroll
+    for (int k = 0; k < R1; k++) v[b + k * NB1] = c[k];
+  }
+}
+
+}  // namespace
+
+// Rows (length ny = N): transform slot g of the launch advances rows g*a.rpb ..
+// g*a.rpb + a.rpb - 1 (row = probe * nx + x).  Modes as k_gen_rows: 0 probe input,
+// 1 state input (y-IFFT first), 2 exit (y-IFFT into a.exitw, no transmit).
+template <int N>
+__global__ __launch_bounds__(64 * kMixWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_mix_rows(GenRowArgs a) {
+  using P = MixPlan<N>;
+  constexpr int E = P::E, T = N / E, S = mix_slots<N>(), LD = mix_ld<N>();
+  __shared__ float2 lds[kMixWaves * S * LD];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int t = lane % T;
+  const int sub = min(lane / T, S - 1);  // lanes past S*T mirror transform S-1
+  const bool act = lane < S * T;
+  float2* slot = lds + (wv * S + sub) * LD;
+  unsigned band = 0;  // bit m: element t + T m is a ky column inside the band
+#pragma unroll
+  for (int m = 0; m < E; m++) band |= (a.ky2[t + T * m] < a.k2max ? 1u : 0u) << m;
+  const int nrows = a.nprobe * a.nx;
+  const int g = (blockIdx.x * kMixWaves + wv) * S + sub;
+  for (int i = 0; i < a.rpb; i++) {
+    const int r0 = g * a.rpb + i;
+    const bool ok = act && r0 < nrows;
+    const int r = min(r0, nrows - 1);
+    const int p = r / a.nx, x = r - p * a.nx, pg = a.probe0 + p;
+    const size_t rowe = ((size_t)pg * a.nx + x) * N;
+    float2 v[E];
+#pragma unroll
+    for (int m = 0; m < E; m++) {
+      const int y = t + T * m;
+      if (a.mode == 0) v[m] = a.probe[(size_t)x * N + y];
+      else v[m] = ((band >> m) & 1) ? a.state[rowe + y] : make_float2(0.f, 0.f);
+    }
+    if (a.mode != 0) mix_fft<N, +1>(v, slot, a.tw, t);
+    if (a.mode == 2) {
+      if (ok) {
+#pragma unroll
+        for (int m = 0; m < E; m++) a.exitw[rowe + t + T * m] = v[m];
+      }
+      continue;
+    }
+    // transmit (stemlib.cpp:2880-2887): the window's row, loaded after the IFFT (its
+    // registers are not held through the transform)
+    const int2 ip = a.ipos[pg];
+    const float2* trow =
+        a.trans + (a.cfg ? (long long)a.cfg[pg] * a.cfg_stride : 0LL) + (size_t)(x + ip.x) * a.potNy + ip.y;
+#pragma unroll
+    for (int m = 0; m < E; m++) v[m] = cmul(v[m], trow[t + T * m]);
+    mix_fft<N, -1>(v, slot, a.tw, t);
+    if (ok) {
+#pragma unroll
+      for (int m = 0; m < E; m++)
+        if ((band >> m) & 1) a.state[rowe + t + T * m] = v[m];
+    }
+  }
+}
+
+// Columns (length nx = N): block b of probe p (a one-dimensional grid of nb blocks
+// per probe) has transform slots ci = 12 b .. 12 b + 11 and slot ci takes band column
+// a.tiles[ci] (a.tc of them, ascending), read straight from the state rows (the three
+// columns of a wavefront are adjacent: one 24-byte piece per row access).  Twelve
+// columns do not fill the rows' 128-byte lines, so the blocks of kMixGroup consecutive
+// b (48 columns: three lines) are placed on one XCD back to back, where they share its
+// L2 (blocks are dispatched round-robin over the 8 XCDs); otherwise every line is
+// fetched, and written back in pieces, by two XCDs.  x-FFT, propagator, band limit,
+// detector sums into partial[p][b][kMaxDet + 1] and the diffraction pattern as
+// k_gen_cols, x-IFFT.  ND: most detectors (a.ndet <= ND), so that only ND + 1 sums are
+// held in registers.
+constexpr int kMixGroup = 4;
+template <int N, int ND>
+__global__ __launch_bounds__(64 * kMixWaves) __attribute__((amdgpu_waves_per_eu(ND <= 4 ? 4 : 3))) void k_mix_cols(GenColArgs a) {
+  using P = MixPlan<N>;
+  constexpr int E = P::E, T = N / E, S = mix_slots<N>(), LD = mix_ld<N>();
+  // register slots m whose kx = t + T m lies, for every lane, strictly inside
+  // (N/3, N - N/3): outside the 2/3 band, zero after the propagator
+  constexpr int A3 = N / 3;
+  __shared__ float2 lds[kMixWaves * S * LD];
+  __shared__ float red[ND + 1][kMixWaves];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  const int t = lane % T;
+  const int sub = min(lane / T, S - 1);
+  const bool act = lane < S * T;
+  float2* slot = lds + (wv * S + sub) * LD;
+  // logical block: groups of kMixGroup consecutive ones per XCD (the last partial
+  // round of 8 kMixGroup blocks keeps the dispatch order)
+  const int nblk = (int)gridDim.x, blk = (int)blockIdx.x;
+  int lb = blk;
+  if (blk < (nblk / (8 * kMixGroup)) * (8 * kMixGroup)) {
+    const int xcd = blk & 7, slot8 = blk >> 3;
+    lb = (slot8 / kMixGroup) * (8 * kMixGroup) + xcd * kMixGroup + slot8 % kMixGroup;
+  }
+  const int nb = (a.tc + kMixWaves * S - 1) / (kMixWaves * S);  // blocks per probe
+  const int p = lb / nb, b = lb - p * nb, ny = a.ny;
+  const int ci = (b * kMixWaves + wv) * S + sub;
+  const bool ok = act && ci < a.tc;
+  const int ky = a.tiles[min(ci, a.tc - 1)];
+  // this probe's state; element x of the column at byte (x ny + ky) 8 (below 2^31: nx, ny <= 2048)
+  const auto rs = mix_rsrc(a.state + (size_t)(a.probe0 + p) * N * ny, (long)N * ny * 8);
+  const unsigned voff = (unsigned)(t * ny + ky) * 8u, mstep = (unsigned)(T * ny) * 8u;
+  float2 v[E];
+#pragma unroll
+  for (int m = 0; m < E; m++) v[m] = mix_load(rs, voff, m * mstep);
+  mix_fft<N, -1>(v, slot, a.tw, t);
+  const float2 pyk = a.py[ky];
+  const float ky2 = a.ky2[ky];
+  float dsum[ND + 1];  // detector sums of this lane's elements; dsum[ND]: the total
+#pragma unroll
+  for (int d = 0; d <= ND; d++) dsum[d] = 0.f;
+#pragma unroll
+  for (int m = 0; m < E; m++) {
+    if (T * m > A3 && T * m + T - 1 < N - A3) {  // in the band gap for every lane
+      v[m] = make_float2(0.f, 0.f);
+      continue;
+    }
+    const int kx = t + T * m;
+    const float kx2 = a.kx2[kx];
+    const bool in = kx2 < a.k2max && (kx2 + ky2) < a.k2max;
+    v[m] = in ? cmul(cmul(v[m], pyk), a.px[kx]) : make_float2(0.f, 0.f);
+    const float e = cnorm(v[m]);
+    if (a.rec) {
+      dsum[ND] += e;
+#pragma unroll
+      for (int d = 0; d < ND; d++) {
+        if (d >= a.ndet) break;
+        float k2 = kx2 + ky2;
+        if (a.det_sx[d] != 0 || a.det_sy[d] != 0) {
+          int sx = kx - a.det_sx[d], sy = ky - a.det_sy[d];
+          sx += sx < 0 ? N : 0;
+          sy += sy < 0 ? ny : 0;
+          k2 = a.kx2[sx] + a.ky2[sy];
+        }
+        dsum[d] += (k2 >= a.det_k2in[d] && k2 <= a.det_k2out[d]) ? e : 0.f;
+      }
+    }
+    if (a.diff && ok) {
+      const int hx = kx + N / 2, hy = ky + ny / 2;
+      a.diff[((size_t)(a.probe0 + p) * N + (hx >= N ? hx - N : hx)) * ny + (hy >= ny ? hy - ny : hy)] =
+          e * a.diff_scale;
+    }
+  }
+  if (a.rec) {
+#pragma unroll
+    for (int d = 0; d <= ND; d++) {
+      if (d < ND && d >= a.ndet) continue;
+      float s = ok ? dsum[d] : 0.f;  // mirrored and surplus slots add nothing
+      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+      if (lane == 0) red[d][wv] = s;
+    }
+    __syncthreads();
+    if (threadIdx.x < a.ndet || threadIdx.x == ND) {
+      float s = 0.f;
+#pragma unroll
+      for (int w = 0; w < kMixWaves; w++) s += red[threadIdx.x][w];
+      a.partial[((size_t)p * nb + b) * (kMaxDet + 1) + (threadIdx.x == ND ? kMaxDet : threadIdx.x)] = s;
+    }
+  }
+  mix_fft<N, +1>(v, slot, a.tw, t);
+  if (ok) {
+#pragma unroll
+    for (int m = 0; m < E; m++) mix_store(v[m], rs, voff, m * mstep);
+  }
+}
+
+bool mix_compiled(int n) { return MixPlan<400>::E > 0 && n == 400; }
+int mix_cols_per_block(int n) { return mix_compiled(n) ? kMixWaves * mix_slots<400>() : 0; }
+
+void launch_mix_rows(const GenRowArgs& a, hipStream_t st) {
+  const long nrows = (long)a.nprobe * a.nx;
+  if (a.ny == 400) {
+    const long per = (long)kMixWaves * mix_slots<400>() * a.rpb;
+    hipLaunchKernelGGL(k_mix_rows<400>, dim3((unsigned)((nrows + per - 1) / per)), dim3(64 * kMixWaves), 0, st, a);
+  }
+}
+
+void launch_mix_cols(const GenColArgs& a, int nblocks, hipStream_t st) {
+  const dim3 grid((unsigned)(nblocks * a.nprobe)), block(64 * kMixWaves);
+  if (a.nx == 400) {
+    if (a.ndet <= 4) hipLaunchKernelGGL((k_mix_cols<400, 4>), grid, block, 0, st, a);
+    else hipLaunchKernelGGL((k_mix_cols<400, kMaxDet>), grid, block, 0, st, a);
+  }
+}
+
 bool gen_plan(int n, GenPlan* pl) { return gen_plan_variant(n, 0, pl); }
 
 bool gen_plan_variant(int n, int variant, GenPlan* pl) {
   if (variant < 0 || variant >= kGenPlanVariants) return false;
+  if (variant == kGenPlanCompiled) return mix_compiled(n) && gen_plan_variant(n, 0, pl);
   int m = n, e2 = 0;
   pl->n = n;
   pl->np = 0;

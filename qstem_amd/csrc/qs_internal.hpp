@@ -195,9 +195,17 @@ struct GenColArgs {
 };
 bool gen_plan(int n, GenPlan* pl);  // false: n has a prime factor above 7
 // radix plan variants (0 = gen_plan's): 0 few passes (radix 16, 25, 9 first),
-// 1 radix 8 for powers of two, 2 radices 2-8 only
-constexpr int kGenPlanVariants = 3;
+// 1 radix 8 for powers of two, 2 radices 2-8 only, 3 (kGenPlanCompiled) the
+// wavefront-resident kernels with a compiled plan (k_mix_rows / k_mix_cols; the
+// lengths of mix_compiled only; the GenPlan is then variant 0's, unused)
+constexpr int kGenPlanVariants = 4, kGenPlanCompiled = 3;
 bool gen_plan_variant(int n, int variant, GenPlan* pl);
+bool mix_compiled(int n);         // n has a compiled two-pass plan (400 = 20 x 20)
+int mix_cols_per_block(int n);    // band columns per block of k_mix_cols<n>
+// a.rpb: rows per transform slot (k_mix_rows, ny = a.ny)
+void launch_mix_rows(const GenRowArgs& a, hipStream_t st);
+// a.tiles: the in-band ky columns (a.tc of them), nblocks = ceil(a.tc / mix_cols_per_block(nx))
+void launch_mix_cols(const GenColArgs& a, int nblocks, hipStream_t st);
 int gen_rows_per_block(int ny);   // default rows per block of k_gen_rows (power of two)
 int gen_cols_per_tile(int nx);    // default columns per tile of k_gen_cols (power of two)
 int gen_max_vectors(int n);       // most vectors of length n one block holds in LDS

@@ -207,7 +207,7 @@ def test_autotune_keeps_results():
     assert rel_l2(out["det"], ref["det"]) < 1e-6
 
 
-@pytest.mark.parametrize("shape", [(400, 400), (96, 160), (200, 128), (112, 98), (136, 88)])
+@pytest.mark.parametrize("shape", [(400, 400), (400, 160), (160, 400), (96, 160), (200, 128), (112, 98), (136, 88)])
 def test_generic_grid_sizes_match_oracle(oracle, shape):
     """Wave grids that are not powers of two (the reference's stem_STO_4x4.qsc uses
     400 x 400): sizes whose prime factors are 2, 3, 5, 7 take the fused mixed-radix
@@ -217,6 +217,7 @@ def test_generic_grid_sizes_match_oracle(oracle, shape):
     diffraction patterns and intensities against the oracle, and a run continued over a
     second call equal to one call over all slices."""
     nx, ny = shape
+    print("generic grid", shape, flush=True)
     cfg = small_config(nx=nx, ny=ny, pad=24, slices=4, dx=0.03)
     V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=12)
     V[2:] = V[:2]  # a continued call restarts at slice 0: slices 2, 3 repeat 0, 1
@@ -246,6 +247,40 @@ def test_generic_grid_sizes_match_oracle(oracle, shape):
     norm = np.sum(np.abs(ex) ** 2, axis=(1, 2)) / (nx * ny)
     assert np.max(np.abs(out["intensity"] - norm) / norm) < TOL
     assert rel_l2(cont["exit"], out["exit"]) < 1e-6
+
+
+@pytest.mark.parametrize("ndet", [4, 6])
+def test_compiled_plans_equal_runtime_plans(oracle, monkeypatch, ndet):
+    """400 has a compiled two-pass plan (20 x 20, wavefront-resident kernels, the
+    default); QSTEM_COMPILED_PLANS=0 at context creation keeps the runtime-planned
+    kernels.  Both agree to FP32 rounding on every output, over 300 probes in groups of
+    128 (the last group partial, rows and columns not multiples of a block's), with up
+    to four detectors (the column kernel's small variant) and with six (two shifted)."""
+    from qstem_amd import Detector
+    cfg = small_config(nx=400, ny=400, pad=40, slices=3, dx=0.03)
+    if ndet == 6:
+        cfg.detectors = cfg.detectors + [Detector(5.0, 15.0, "D5"), Detector(30.0, 90.0, "D6", -1.5, 2.0)]
+    assert len(cfg.detectors) == ndet
+    V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=21)
+    ipos = probe_positions(cfg, 300, seed=8)
+    bins, nbins = oracle.record_bins(cfg.slices, outputInterval=1)
+    res = []
+    for flag in ("1", "0"):
+        monkeypatch.setenv("QSTEM_COMPILED_PLANS", flag)
+        print("compiled plans", flag, flush=True)
+        with _ctx(cfg) as ms:
+            ms.set_potential(V)
+            ms.make_probe()
+            ms.set_batch(128)
+            res.append(ms.run_stem(ipos, bins=bins, nbins=nbins, want_diff=True, want_exit=True,
+                                   want_intensity=True))
+    a, b = res
+    assert rel_l2(a["exit"], b["exit"]) < 2e-6
+    assert rel_l2(a["diff"], b["diff"]) < 4e-6
+    for k in range(nbins):
+        for d in range(a["det"].shape[1]):
+            assert rel_l2(a["det"][k, d], b["det"][k, d]) < 4e-6, (k, d)
+    assert np.max(np.abs(a["intensity"] - b["intensity"]) / b["intensity"]) < 4e-6
 
 
 @pytest.mark.parametrize("n", [96, 88])
