@@ -713,12 +713,14 @@ __global__ __launch_bounds__(64 * kMixWaves) __attribute__((amdgpu_waves_per_eu(
 
 // Columns (length nx = N): block b of probe p (a one-dimensional grid of nb blocks
 // per probe) has transform slots ci = 12 b .. 12 b + 11 and slot ci takes band column
-// a.tiles[ci] (a.tc of them, ascending), read straight from the state rows (the three
-// columns of a wavefront are adjacent: one 24-byte piece per row access).  Twelve
-// columns do not fill the rows' 128-byte lines, so the blocks of kMixGroup consecutive
-// b (48 columns: three lines) are placed on one XCD back to back, where they share its
-// L2 (blocks are dispatched round-robin over the 8 XCDs); otherwise every line is
-// fetched, and written back in pieces, by two XCDs.  x-FFT, propagator, band limit,
+// a.tiles[ci] (a.tc of them, ascending).  The block's columns are staged through the
+// slots in LDS: each state row's piece of the twelve columns is read, and written back,
+// by twelve consecutive lanes (one contiguous piece per row and instruction; read
+// straight by the transforms' lanes, every lane would touch twenty rows per access and
+// the rows' lines would be fetched and written back in 24-byte pieces).  Twelve columns
+// do not fill the rows' 128-byte lines, so the blocks of kMixGroup consecutive b (48
+// columns: three lines) are placed on one XCD back to back, where they share its L2
+// (blocks are dispatched round-robin over the 8 XCDs).  x-FFT, propagator, band limit,
 // detector sums into partial[p][b][kMaxDet + 1] and the diffraction pattern as
 // k_gen_cols, x-IFFT.  ND: most detectors (a.ndet <= ND), so that only ND + 1 sums are
 // held in registers.
@@ -750,12 +752,22 @@ __global__ __launch_bounds__(64 * kMixWaves) __attribute__((amdgpu_waves_per_eu(
   const int ci = (b * kMixWaves + wv) * S + sub;
   const bool ok = act && ci < a.tc;
   const int ky = a.tiles[min(ci, a.tc - 1)];
-  // this probe's state; element x of the column at byte (x ny + ky) 8 (below 2^31: nx, ny <= 2048)
+  // this probe's state; element x of column ky at byte (x ny + ky) 8 (below 2^31: nx, ny <= 2048)
   const auto rs = mix_rsrc(a.state + (size_t)(a.probe0 + p) * N * ny, (long)N * ny * 8);
-  const unsigned voff = (unsigned)(t * ny + ky) * 8u, mstep = (unsigned)(T * ny) * 8u;
+  // staging: lane (sc, sx) moves block column sc (slot sc) of rows sx + SR i
+  constexpr int CPB = kMixWaves * S, SR = 64 * kMixWaves / CPB;
+  const int sc = threadIdx.x % CPB, sx = threadIdx.x / CPB;
+  const int sci = b * CPB + sc;
+  const bool sok = sx < SR && sci < a.tc;
+  const unsigned soff = sok ? (unsigned)(sx * ny + a.tiles[sci]) * 8u : 0u, sstep = (unsigned)(SR * ny) * 8u;
+  float2* const sslot = lds + sc * LD;
+  if (sok) {
+    for (int i = 0; i * SR + sx < N; i++) sslot[sx + SR * i] = mix_load(rs, soff, i * sstep);
+  }
+  __syncthreads();
   float2 v[E];
 #pragma unroll
-  for (int m = 0; m < E; m++) v[m] = mix_load(rs, voff, m * mstep);
+  for (int m = 0; m < E; m++) v[m] = slot[t + T * m];
   mix_fft<N, -1>(v, slot, a.tw, t);
   const float2 pyk = a.py[ky];
   const float ky2 = a.ky2[ky];
@@ -811,9 +823,12 @@ __global__ __launch_bounds__(64 * kMixWaves) __attribute__((amdgpu_waves_per_eu(
     }
   }
   mix_fft<N, +1>(v, slot, a.tw, t);
-  if (ok) {
+  wave_lds_sync();  // the transform's last reads of its slot are done
 #pragma unroll
-    for (int m = 0; m < E; m++) mix_store(v[m], rs, voff, m * mstep);
+  for (int m = 0; m < E; m++) slot[t + T * m] = v[m];
+  __syncthreads();
+  if (sok) {
+    for (int i = 0; i * SR + sx < N; i++) mix_store(sslot[sx + SR * i], rs, soff, i * sstep);
   }
 }
 
