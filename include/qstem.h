@@ -325,7 +325,10 @@ int qs_set_graph(qs_ctx *ctx, int on);
  * reports the probe group size of qs_set_batch.  For grids on the mixed-radix path
  * (not powers of two, prime factors 2, 3, 5, 7) the two values are the rows per block
  * of the row kernel and the columns per tile of the column kernel (powers of two up
- * to 16 that fit the block's LDS; QS_ERR_ARG otherwise).  No reference counterpart. */
+ * to 16 that fit the block's LDS; QS_ERR_ARG otherwise); where a length has a compiled
+ * plan (400: wavefront-resident kernels, the default unless QSTEM_COMPILED_PLANS=0 is
+ * set when the context is created) the first is the rows per transform slot of the
+ * row kernel and the second is unused.  No reference counterpart. */
 int qs_set_tuning(qs_ctx *ctx, int rows_per_wave, int tiles_per_block);
 int qs_get_tuning(const qs_ctx *ctx, int *rows_per_wave, int *tiles_per_block, int *batch);
 
@@ -335,7 +338,8 @@ int qs_get_tuning(const qs_ctx *ctx, int *rows_per_wave, int *tiles_per_block, i
  * fastest (as qs_set_batch + qs_set_tuning) and reports it (outputs nullable;
  * rate in probe-slices/s).  On the mixed-radix path it tunes the group size, rows
  * per block, columns per tile and the radix plan of each axis (few large radices,
- * radix 8, or radices up to 8: results then differ by FP32 rounding only).  Results are cached per device model and wave geometry
+ * radix 8, radices up to 8, or the compiled plan where the length has one: results
+ * then differ by FP32 rounding only).  Results are cached per device model and wave geometry
  * for the life of the process, and across processes in the JSON file named by the
  * environment variable QSTEM_TUNE_CACHE when it is set (read at the first call,
  * rewritten after each new measurement; no file is touched when it is unset).  No
