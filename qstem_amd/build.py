@@ -21,6 +21,16 @@ BIN = os.path.join(HERE, "bin")
 ROOT = os.path.dirname(HERE)
 
 SIZES = (64, 128, 256, 512, 1024, 2048, 4096)
+
+
+def mix_sizes():
+    """Lengths with a compiled mixed-radix plan: the QS_MIX_SIZES list of qs_internal.hpp."""
+    import re
+    text = open(os.path.join(CSRC, "qs_internal.hpp")).read()
+    m = re.search(r"#define QS_MIX_SIZES\(X\)((?: X\(\d+\))+)", text)
+    if not m:
+        raise RuntimeError("QS_MIX_SIZES not found in qs_internal.hpp")
+    return tuple(int(n) for n in re.findall(r"X\((\d+)\)", m.group(1)))
 ARCH = os.environ.get("QS_OFFLOAD_ARCH", "gfx950")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 # packed FP32 instructions are disabled on purpose: some of their forms have
@@ -58,6 +68,10 @@ def build_lib(force=True, jobs=8):
         src = os.path.join(CSRC, "qs_sizes.hip")
         obj = os.path.join(OBJ, "qs_sizes_%d.o" % n)
         jobs_list.append((obj, [HIPCC] + HIPFLAGS + ["-DQS_N=%d" % n, "-c", src, "-o", obj], [src] + headers))
+    for n in mix_sizes():
+        src = os.path.join(CSRC, "qs_mix.hip")
+        obj = os.path.join(OBJ, "qs_mix_%d.o" % n)
+        jobs_list.append((obj, [HIPCC] + HIPFLAGS + ["-DQS_MIX_N=%d" % n, "-c", src, "-o", obj], [src] + headers))
     for name in ("qs_setup.hip", "qs_capi.cpp", "qs_potential.hip", "qs_adjoint.hip", "qs_generic.hip"):
         src = os.path.join(CSRC, name)
         if not os.path.exists(src):

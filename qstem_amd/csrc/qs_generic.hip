@@ -13,6 +13,7 @@
 
 #include <algorithm>
 
+#include "qs_dft.hpp"
 #include "qs_fft.hpp"
 #include "qs_internal.hpp"
 
@@ -122,113 +123,6 @@ void launch_gen_propagate(const GenPropArgs& a, int nprobe, hipStream_t st) {
 // twiddle table W_n^m = exp(-2 pi i m/n), m < n.
 
 namespace {
-
-template <int DIR>
-__device__ __forceinline__ float2 rot_i(float2 v) {  // DIR * i * v
-  return DIR < 0 ? make_float2(v.y, -v.x) : make_float2(-v.y, v.x);
-}
-
-template <int DIR>
-__device__ __forceinline__ void dft3(float2* a) {
-  const float h = 0.86602540378443865f;  // sin(2 pi/3)
-  const float2 b = cadd(a[1], a[2]), d = csub(a[1], a[2]);
-  const float2 t = make_float2(a[0].x - 0.5f * b.x, a[0].y - 0.5f * b.y);
-  const float2 u = rot_i<DIR>(cscale(d, h));
-  a[0] = cadd(a[0], b);
-  a[1] = cadd(t, u);
-  a[2] = csub(t, u);
-}
-
-template <int DIR>
-__device__ __forceinline__ void dft5(float2* a) {
-  const float c1 = 0.30901699437494742f, c2 = -0.80901699437494742f;  // cos(2 pi/5), cos(4 pi/5)
-  const float s1 = 0.95105651629515357f, s2 = 0.58778525229247313f;   // sin(2 pi/5), sin(4 pi/5)
-  const float2 b1 = cadd(a[1], a[4]), b2 = cadd(a[2], a[3]);
-  const float2 d1 = csub(a[1], a[4]), d2 = csub(a[2], a[3]);
-  const float2 a0 = a[0];
-  const float2 r1 = make_float2(a0.x + c1 * b1.x + c2 * b2.x, a0.y + c1 * b1.y + c2 * b2.y);
-  const float2 r2 = make_float2(a0.x + c2 * b1.x + c1 * b2.x, a0.y + c2 * b1.y + c1 * b2.y);
-  const float2 i1 = rot_i<DIR>(make_float2(s1 * d1.x + s2 * d2.x, s1 * d1.y + s2 * d2.y));
-  const float2 i2 = rot_i<DIR>(make_float2(s2 * d1.x - s1 * d2.x, s2 * d1.y - s1 * d2.y));
-  a[0] = cadd(a0, cadd(b1, b2));
-  a[1] = cadd(r1, i1);
-  a[4] = csub(r1, i1);
-  a[2] = cadd(r2, i2);
-  a[3] = csub(r2, i2);
-}
-
-template <int DIR>
-__device__ __forceinline__ void dft7(float2* a) {
-  const float c1 = 0.62348980185873353f, c2 = -0.22252093395631440f, c3 = -0.90096886790241913f;
-  const float s1 = 0.78183148246802981f, s2 = 0.97492791218182361f, s3 = 0.43388373911755812f;
-  const float2 b1 = cadd(a[1], a[6]), b2 = cadd(a[2], a[5]), b3 = cadd(a[3], a[4]);
-  const float2 d1 = csub(a[1], a[6]), d2 = csub(a[2], a[5]), d3 = csub(a[3], a[4]);
-  const float2 a0 = a[0];
-  // X_k = a0 + sum_j cos(2 pi jk/7) b_j +- DIR i sum_j sin(2 pi jk/7) d_j, k = 1, 2, 3
-  auto re = [&](float x1, float x2, float x3) {
-    return make_float2(a0.x + x1 * b1.x + x2 * b2.x + x3 * b3.x, a0.y + x1 * b1.y + x2 * b2.y + x3 * b3.y);
-  };
-  auto im = [&](float y1, float y2, float y3) {
-    return rot_i<DIR>(make_float2(y1 * d1.x + y2 * d2.x + y3 * d3.x, y1 * d1.y + y2 * d2.y + y3 * d3.y));
-  };
-  const float2 r1 = re(c1, c2, c3), r2 = re(c2, c3, c1), r3 = re(c3, c1, c2);
-  const float2 i1 = im(s1, s2, s3), i2 = im(s2, -s3, -s1), i3 = im(s3, -s1, s2);
-  a[0] = cadd(a0, cadd(b1, cadd(b2, b3)));
-  a[1] = cadd(r1, i1);
-  a[6] = csub(r1, i1);
-  a[2] = cadd(r2, i2);
-  a[5] = csub(r2, i2);
-  a[3] = cadd(r3, i3);
-  a[4] = csub(r3, i3);
-}
-
-template <int R1, int R2, int DIR>
-__device__ __forceinline__ void dft_2stage(float2* a);
-
-template <int R, int DIR>
-__device__ __forceinline__ void dft_any(float2* a) {
-  if constexpr (R == 3) dft3<DIR>(a);
-  else if constexpr (R == 5) dft5<DIR>(a);
-  else if constexpr (R == 7) dft7<DIR>(a);
-  else if constexpr (R == 9) dft_2stage<3, 3, DIR>(a);
-  else if constexpr (R == 20) dft_2stage<4, 5, DIR>(a);
-  else if constexpr (R == 25) dft_2stage<5, 5, DIR>(a);
-  else dft<R, DIR>(a);
-}
-
-// Length R1*R2 DFT in registers (natural order in and out) by one Cooley-Tukey step:
-// input n = R2 n1 + n2, output k = k1 + R1 k2; R1-point DFTs over n1, twiddles
-// W_N^(n2 k1) (constants: twiddle_at of compile-time indices), R2-point DFTs over n2.
-template <int R1, int R2, int DIR>
-__device__ __forceinline__ void dft_2stage(float2* a) {
-  constexpr int N = R1 * R2;
-  float2 t[N];
-#pragma unroll
-  for (int n2 = 0; n2 < R2; n2++) {
-    float2 u[R1];
-#pragma unroll
-    for (int n1 = 0; n1 < R1; n1++) u[n1] = a[R2 * n1 + n2];
-    dft_any<R1, DIR>(u);
-#pragma unroll
-    for (int k1 = 0; k1 < R1; k1++) {
-      float2 v = u[k1];
-      if ((n2 * k1) % N != 0) {
-        const float2 w = twiddle_at<N>((n2 * k1) % N);
-        v = DIR < 0 ? cmul(v, w) : cmulc(v, w);
-      }
-      t[n2 * R1 + k1] = v;
-    }
-  }
-#pragma unroll
-  for (int k1 = 0; k1 < R1; k1++) {
-    float2 u[R2];
-#pragma unroll
-    for (int n2 = 0; n2 < R2; n2++) u[n2] = t[n2 * R1 + k1];
-    dft_any<R2, DIR>(u);
-#pragma unroll
-    for (int k2 = 0; k2 < R2; k2++) a[k1 + R1 * k2] = u[k2];
-  }
-}
 
 // Largest LDS footprint of one block in float2 (V vectors of gen_ld(n) each), and
 // the block size.
@@ -550,305 +444,38 @@ void launch_gen_rowxform(const GenRowXformArgs& a, int dir, hipStream_t st) {
 }
 
 // ============================================================================
-// Wavefront-resident mixed-radix sweeps (compiled plans).  The runtime-planned
-// kernels above share every Stockham pass among all threads of a block: two block
-// barriers per pass, and threads idle wherever the radix leaves the butterfly
-// count short of the block (half of them in the radix-25 pass of 400).  For a
-// length with a compiled two-pass plan N = R0 x R1, a transform is owned by T = N/E
-// lanes of one wavefront, each holding E values in registers (element t + T m in
-// v[m], the layout of coalesced row loads); the one exchange between the passes
-// goes through the transform's own LDS slot and needs wavefront-level ordering
-// only.  64/T transforms share a wavefront; lanes past them mirror the last
-// transform's lanes (same inputs, same values written to the same LDS words) and
-// store nothing.  State layout, modes and outputs are those of k_gen_rows and
-// k_gen_cols; the column kernel reads its column straight from the state rows
-// (the three columns of a wavefront are adjacent, so each row access is one
-// 24-byte piece).
+// Compiled-plan sweeps (qs_mix.hpp, one translation unit per length): dispatch.
+#define QS_MIX_DECL(n)                                                   \
+  void mix_rows_##n(const GenRowArgs& a, hipStream_t st);                \
+  void mix_cols_##n(const GenColArgs& a, int nblocks, hipStream_t st);   \
+  int mix_cpb_##n();
+QS_MIX_SIZES(QS_MIX_DECL)
+#undef QS_MIX_DECL
 
-// E values per lane, passes of radix R0 then R1; LD: float2 per transform slot in LDS
-template <int N> struct MixPlan { static constexpr int E = 0; };
-template <> struct MixPlan<400> { static constexpr int E = 20, R0 = 20, R1 = 20, LD = 420; };
-
-namespace {
-
-constexpr int kMixWaves = 4;  // wavefronts per block
-template <int N> constexpr int mix_slots() { return 64 / (N / MixPlan<N>::E); }  // transforms per wavefront
-template <int N> constexpr int mix_ld() { return MixPlan<N>::LD; }
-// LDS position of the value between the passes with Stockham index i = j R0 + k
-// (butterfly j of the first pass, its output k): k (N/R0 + 1) + j, the first pass's
-// outputs transposed with one float2 of padding per row.  At 400 (20 x 20, three
-// transforms per wavefront, slot stride 420 = 4 mod 32) the first pass's writes (lanes
-// of a transform on consecutive j) and the second pass's reads (stride 21, odd) fall on
-// distinct eight-byte bank pairs in every 16-lane write group and 32-lane read group,
-// across the three transforms of a wavefront too; four blocks of twelve slots fit the
-// CU's 160 KB.
-template <int N>
-__device__ __forceinline__ int mix_pos(int i) {
-  constexpr int R0 = MixPlan<N>::R0;
-  return (i % R0) * (N / R0 + 1) + i / R0;
+bool mix_compiled(int n) {
+#define QS_MIX_CASE(m) if (n == m) return true;
+  QS_MIX_SIZES(QS_MIX_CASE)
+#undef QS_MIX_CASE
+  return false;
 }
 
-// Buffer-resource access (MUBUF): a per-lane byte offset plus a wavefront-uniform
-// one, so that the E strided column elements of a lane need one address register.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t mix_rsrc(const void* base, long bytes) {
-  const int n = bytes < 0x7fffffffL ? (int)bytes : 0x7fffffff;
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, n, 0x00020000);
+int mix_cols_per_block(int n) {
+#define QS_MIX_CASE(m) if (n == m) return mix_cpb_##m();
+  QS_MIX_SIZES(QS_MIX_CASE)
+#undef QS_MIX_CASE
+  return 0;
 }
-__device__ __forceinline__ float2 mix_load(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
-}
-__device__ __forceinline__ void mix_store(float2 v, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff) {
-  using U2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0u, 0, 0));
-  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, voff, soff, 0);
-}
-
-// Length-N transform of v (element t + T m in v[m], in and out) in two Stockham
-// passes: radix R0 over registers into the LDS slot (butterfly j = t + T b reads
-// elements j + k N/R0 and writes j R0 + k), then radix R1 from the slot (reads
-// j + k N/R1, twiddles W_N^(k (j mod R0)) from the table tw = W_N^m, m < N (k (j mod
-// R0) < N), writes j + k R0 = t + T (b + k NB1)).
-template <int N, int DIR>
-__device__ __forceinline__ void mix_fft(float2* v, float2* slot, const float2* __restrict__ tw, int t) {
-  using P = MixPlan<N>;
-  constexpr int E = P::E, T = N / E, R0 = P::R0, R1 = P::R1, NB0 = E / R0, NB1 = E / R1;
-  static_assert(R0 * R1 == N && E % R0 == 0 && E % R1 == 0 && R0 == T * NB1, "two-pass plan");
-  float2 a[NB0][R0];
-#pragma unroll
-  for (int b = 0; b < NB0; b++) {
-#pragma unroll
-    for (int k = 0; k < R0; k++) a[b][k] = v[b + k * NB0];
-    dft_any<R0, DIR>(a[b]);
-  }
-  wave_lds_sync();  // the slot's earlier reads are done
-#pragma unroll
-  for (int b = 0; b < NB0; b++) {
-    const int j = t + T * b;
-#pragma unroll
-    for (int k = 0; k < R0; k++) slot[mix_pos<N>(j * R0 + k)] = a[b][k];
-  }
-  wave_lds_sync();
-#pragma unroll
-  for (int b = 0; b < NB1; b++) {
-    const int j = t + T * b;
-    // opaque per call: the twiddle loads are neither hoisted out of the callers'
-    // loops nor shared between the two transforms of a kernel (registers held)
-    int jm = j % R0;
-    asm volatile("" : "+v"(jm));
-    float2 c[R1];
-#pragma unroll
-    for (int k = 0; k < R1; k++) c[k] = slot[mix_pos<N>(j + k * (N / R1))];
-    // w^k loaded for k < 4 and k a multiple of 4, the others as w^(k & ~3) w^(k & 3)
-    // (7 loads instead of 19 for radix 20: fewer registers in flight)
-    float2 pw[R1];
-#pragma unroll
-    for (int k = 1; k < R1; k++)
-      if (k < 4 || (k & 3) == 0) pw[k] = tw[k * jm];
-#pragma unroll
-    for (int k = 5; k < R1; k++)
-      if (k & 3) pw[k] = cmul(pw[k & ~3], pw[k & 3]);
-#pragma unroll
-    for (int k = 1; k < R1; k++) c[k] = DIR < 0 ? cmul(c[k], pw[k]) : cmulc(c[k], pw[k]);
-    dft_any<R1, DIR>(c);
-#pragma unroll
-    for (int k = 0; k < R1; k++) v[b + k * NB1] = c[k];
-  }
-}
-
-}  // namespace
-
-// Rows (length ny = N): transform slot g of the launch advances rows g*a.rpb ..
-// g*a.rpb + a.rpb - 1 (row = probe * nx + x).  Modes as k_gen_rows: 0 probe input,
-// 1 state input (y-IFFT first), 2 exit (y-IFFT into a.exitw, no transmit).
-template <int N>
-__global__ __launch_bounds__(64 * kMixWaves) __attribute__((amdgpu_waves_per_eu(4))) void k_mix_rows(GenRowArgs a) {
-  using P = MixPlan<N>;
-  constexpr int E = P::E, T = N / E, S = mix_slots<N>(), LD = mix_ld<N>();
-  __shared__ float2 lds[kMixWaves * S * LD];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int t = lane % T;
-  const int sub = min(lane / T, S - 1);  // lanes past S*T mirror transform S-1
-  const bool act = lane < S * T;
-  float2* slot = lds + (wv * S + sub) * LD;
-  unsigned band = 0;  // bit m: element t + T m is a ky column inside the band
-#pragma unroll
-  for (int m = 0; m < E; m++) band |= (a.ky2[t + T * m] < a.k2max ? 1u : 0u) << m;
-  const int nrows = a.nprobe * a.nx;
-  const int g = (blockIdx.x * kMixWaves + wv) * S + sub;
-  for (int i = 0; i < a.rpb; i++) {
-    const int r0 = g * a.rpb + i;
-    const bool ok = act && r0 < nrows;
-    const int r = min(r0, nrows - 1);
-    const int p = r / a.nx, x = r - p * a.nx, pg = a.probe0 + p;
-    const size_t rowe = ((size_t)pg * a.nx + x) * N;
-    float2 v[E];
-#pragma unroll
-    for (int m = 0; m < E; m++) {
-      const int y = t + T * m;
-      if (a.mode == 0) v[m] = a.probe[(size_t)x * N + y];
-      else v[m] = ((band >> m) & 1) ? a.state[rowe + y] : make_float2(0.f, 0.f);
-    }
-    if (a.mode != 0) mix_fft<N, +1>(v, slot, a.tw, t);
-    if (a.mode == 2) {
-      if (ok) {
-#pragma unroll
-        for (int m = 0; m < E; m++) a.exitw[rowe + t + T * m] = v[m];
-      }
-      continue;
-    }
-    // transmit (stemlib.cpp:2880-2887): the window's row, loaded after the IFFT (its
-    // registers are not held through the transform)
-    const int2 ip = a.ipos[pg];
-    const float2* trow =
-        a.trans + (a.cfg ? (long long)a.cfg[pg] * a.cfg_stride : 0LL) + (size_t)(x + ip.x) * a.potNy + ip.y;
-#pragma unroll
-    for (int m = 0; m < E; m++) v[m] = cmul(v[m], trow[t + T * m]);
-    mix_fft<N, -1>(v, slot, a.tw, t);
-    if (ok) {
-#pragma unroll
-      for (int m = 0; m < E; m++)
-        if ((band >> m) & 1) a.state[rowe + t + T * m] = v[m];
-    }
-  }
-}
-
-// Columns (length nx = N): block b of probe p (a one-dimensional grid of nb blocks
-// per probe) has transform slots ci = 12 b .. 12 b + 11 and slot ci takes band column
-// a.tiles[ci] (a.tc of them, ascending).  The block's columns are staged through the
-// slots in LDS: each state row's piece of the twelve columns is read, and written back,
-// by twelve consecutive lanes (one contiguous piece per row and instruction; read
-// straight by the transforms' lanes, every lane would touch twenty rows per access and
-// the rows' lines would be fetched and written back in 24-byte pieces).  Twelve columns
-// do not fill the rows' 128-byte lines, so the blocks of kMixGroup consecutive b (48
-// columns: three lines) are placed on one XCD back to back, where they share its L2
-// (blocks are dispatched round-robin over the 8 XCDs).  x-FFT, propagator, band limit,
-// detector sums into partial[p][b][kMaxDet + 1] and the diffraction pattern as
-// k_gen_cols, x-IFFT.  ND: most detectors (a.ndet <= ND), so that only ND + 1 sums are
-// held in registers.
-constexpr int kMixGroup = 4;
-template <int N, int ND>
-__global__ __launch_bounds__(64 * kMixWaves) __attribute__((amdgpu_waves_per_eu(ND <= 4 ? 4 : 3))) void k_mix_cols(GenColArgs a) {
-  using P = MixPlan<N>;
-  constexpr int E = P::E, T = N / E, S = mix_slots<N>(), LD = mix_ld<N>();
-  // register slots m whose kx = t + T m lies, for every lane, strictly inside
-  // (N/3, N - N/3): outside the 2/3 band, zero after the propagator
-  constexpr int A3 = N / 3;
-  __shared__ float2 lds[kMixWaves * S * LD];
-  __shared__ float red[ND + 1][kMixWaves];
-  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  const int t = lane % T;
-  const int sub = min(lane / T, S - 1);
-  const bool act = lane < S * T;
-  float2* slot = lds + (wv * S + sub) * LD;
-  // logical block: groups of kMixGroup consecutive ones per XCD (the last partial
-  // round of 8 kMixGroup blocks keeps the dispatch order)
-  const int nblk = (int)gridDim.x, blk = (int)blockIdx.x;
-  int lb = blk;
-  if (blk < (nblk / (8 * kMixGroup)) * (8 * kMixGroup)) {
-    const int xcd = blk & 7, slot8 = blk >> 3;
-    lb = (slot8 / kMixGroup) * (8 * kMixGroup) + xcd * kMixGroup + slot8 % kMixGroup;
-  }
-  const int nb = (a.tc + kMixWaves * S - 1) / (kMixWaves * S);  // blocks per probe
-  const int p = lb / nb, b = lb - p * nb, ny = a.ny;
-  const int ci = (b * kMixWaves + wv) * S + sub;
-  const bool ok = act && ci < a.tc;
-  const int ky = a.tiles[min(ci, a.tc - 1)];
-  // this probe's state; element x of column ky at byte (x ny + ky) 8 (below 2^31: nx, ny <= 2048)
-  const auto rs = mix_rsrc(a.state + (size_t)(a.probe0 + p) * N * ny, (long)N * ny * 8);
-  // staging: lane (sc, sx) moves block column sc (slot sc) of rows sx + SR i
-  constexpr int CPB = kMixWaves * S, SR = 64 * kMixWaves / CPB;
-  const int sc = threadIdx.x % CPB, sx = threadIdx.x / CPB;
-  const int sci = b * CPB + sc;
-  const bool sok = sx < SR && sci < a.tc;
-  const unsigned soff = sok ? (unsigned)(sx * ny + a.tiles[sci]) * 8u : 0u, sstep = (unsigned)(SR * ny) * 8u;
-  float2* const sslot = lds + sc * LD;
-  if (sok) {
-    for (int i = 0; i * SR + sx < N; i++) sslot[sx + SR * i] = mix_load(rs, soff, i * sstep);
-  }
-  __syncthreads();
-  float2 v[E];
-#pragma unroll
-  for (int m = 0; m < E; m++) v[m] = slot[t + T * m];
-  mix_fft<N, -1>(v, slot, a.tw, t);
-  const float2 pyk = a.py[ky];
-  const float ky2 = a.ky2[ky];
-  float dsum[ND + 1];  // detector sums of this lane's elements; dsum[ND]: the total
-#pragma unroll
-  for (int d = 0; d <= ND; d++) dsum[d] = 0.f;
-#pragma unroll
-  for (int m = 0; m < E; m++) {
-    if (T * m > A3 && T * m + T - 1 < N - A3) {  // in the band gap for every lane
-      v[m] = make_float2(0.f, 0.f);
-      continue;
-    }
-    const int kx = t + T * m;
-    const float kx2 = a.kx2[kx];
-    const bool in = kx2 < a.k2max && (kx2 + ky2) < a.k2max;
-    v[m] = in ? cmul(cmul(v[m], pyk), a.px[kx]) : make_float2(0.f, 0.f);
-    const float e = cnorm(v[m]);
-    if (a.rec) {
-      dsum[ND] += e;
-#pragma unroll
-      for (int d = 0; d < ND; d++) {
-        if (d >= a.ndet) break;
-        float k2 = kx2 + ky2;
-        if (a.det_sx[d] != 0 || a.det_sy[d] != 0) {
-          int sx = kx - a.det_sx[d], sy = ky - a.det_sy[d];
-          sx += sx < 0 ? N : 0;
-          sy += sy < 0 ? ny : 0;
-          k2 = a.kx2[sx] + a.ky2[sy];
-        }
-        dsum[d] += (k2 >= a.det_k2in[d] && k2 <= a.det_k2out[d]) ? e : 0.f;
-      }
-    }
-    if (a.diff && ok) {
-      const int hx = kx + N / 2, hy = ky + ny / 2;
-      a.diff[((size_t)(a.probe0 + p) * N + (hx >= N ? hx - N : hx)) * ny + (hy >= ny ? hy - ny : hy)] =
-          e * a.diff_scale;
-    }
-  }
-  if (a.rec) {
-#pragma unroll
-    for (int d = 0; d <= ND; d++) {
-      if (d < ND && d >= a.ndet) continue;
-      float s = ok ? dsum[d] : 0.f;  // mirrored and surplus slots add nothing
-      for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
-      if (lane == 0) red[d][wv] = s;
-    }
-    __syncthreads();
-    if (threadIdx.x < a.ndet || threadIdx.x == ND) {
-      float s = 0.f;
-#pragma unroll
-      for (int w = 0; w < kMixWaves; w++) s += red[threadIdx.x][w];
-      a.partial[((size_t)p * nb + b) * (kMaxDet + 1) + (threadIdx.x == ND ? kMaxDet : threadIdx.x)] = s;
-    }
-  }
-  mix_fft<N, +1>(v, slot, a.tw, t);
-  wave_lds_sync();  // the transform's last reads of its slot are done
-#pragma unroll
-  for (int m = 0; m < E; m++) slot[t + T * m] = v[m];
-  __syncthreads();
-  if (sok) {
-    for (int i = 0; i * SR + sx < N; i++) mix_store(sslot[sx + SR * i], rs, soff, i * sstep);
-  }
-}
-
-bool mix_compiled(int n) { return MixPlan<400>::E > 0 && n == 400; }
-int mix_cols_per_block(int n) { return mix_compiled(n) ? kMixWaves * mix_slots<400>() : 0; }
 
 void launch_mix_rows(const GenRowArgs& a, hipStream_t st) {
-  const long nrows = (long)a.nprobe * a.nx;
-  if (a.ny == 400) {
-    const long per = (long)kMixWaves * mix_slots<400>() * a.rpb;
-    hipLaunchKernelGGL(k_mix_rows<400>, dim3((unsigned)((nrows + per - 1) / per)), dim3(64 * kMixWaves), 0, st, a);
-  }
+#define QS_MIX_CASE(m) if (a.ny == m) return mix_rows_##m(a, st);
+  QS_MIX_SIZES(QS_MIX_CASE)
+#undef QS_MIX_CASE
 }
 
 void launch_mix_cols(const GenColArgs& a, int nblocks, hipStream_t st) {
-  const dim3 grid((unsigned)(nblocks * a.nprobe)), block(64 * kMixWaves);
-  if (a.nx == 400) {
-    if (a.ndet <= 4) hipLaunchKernelGGL((k_mix_cols<400, 4>), grid, block, 0, st, a);
-    else hipLaunchKernelGGL((k_mix_cols<400, kMaxDet>), grid, block, 0, st, a);
-  }
+#define QS_MIX_CASE(m) if (a.nx == m) return mix_cols_##m(a, nblocks, st);
+  QS_MIX_SIZES(QS_MIX_CASE)
+#undef QS_MIX_CASE
 }
 
 bool gen_plan(int n, GenPlan* pl) { return gen_plan_variant(n, 0, pl); }

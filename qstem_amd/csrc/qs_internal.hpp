@@ -200,7 +200,10 @@ bool gen_plan(int n, GenPlan* pl);  // false: n has a prime factor above 7
 // lengths of mix_compiled only; the GenPlan is then variant 0's, unused)
 constexpr int kGenPlanVariants = 4, kGenPlanCompiled = 3;
 bool gen_plan_variant(int n, int variant, GenPlan* pl);
-bool mix_compiled(int n);         // n has a compiled two-pass plan (400 = 20 x 20)
+// Lengths with a compiled radix plan (qs_mix.hpp; one translation unit qs_mix.hip per
+// length, which the build reads from this list).
+#define QS_MIX_SIZES(X) X(320) X(384) X(400) X(448) X(600) X(768)
+bool mix_compiled(int n);         // n has a compiled radix plan (QS_MIX_SIZES)
 int mix_cols_per_block(int n);    // band columns per block of k_mix_cols<n>
 // a.rpb: rows per transform slot (k_mix_rows, ny = a.ny)
 void launch_mix_rows(const GenRowArgs& a, hipStream_t st);
