@@ -25,12 +25,17 @@ namespace qs {
 // Compiled plans: E values per lane, NP passes of radix R0, R1 (, R2), each dividing
 // E (so every lane owns E/R butterflies of a pass); LD float2 per transform slot.
 template <int N> struct MixPlan { static constexpr int E = 0; };
+template <> struct MixPlan<288> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 6, R2 = 2, LD = 297; };
+template <> struct MixPlan<300> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 5, R2 = 2, LD = 309; };
 template <> struct MixPlan<320> { static constexpr int E = 20, NP = 3, R0 = 20, R1 = 4, R2 = 4, LD = 331; };
+template <> struct MixPlan<360> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 6, R2 = 2, LD = 371; };
 template <> struct MixPlan<384> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 8, R2 = 2, LD = 397; };
 template <> struct MixPlan<400> { static constexpr int E = 20, NP = 2, R0 = 20, R1 = 20, R2 = 1, LD = 420; };
+template <> struct MixPlan<441> { static constexpr int E = 21, NP = 3, R0 = 21, R1 = 7, R2 = 3, LD = 455; };
 template <> struct MixPlan<448> { static constexpr int E = 28, NP = 3, R0 = 28, R1 = 4, R2 = 4, LD = 463; };
 template <> struct MixPlan<600> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 10, R2 = 2, LD = 619; };
 template <> struct MixPlan<768> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 8, R2 = 4, LD = 793; };
+template <> struct MixPlan<900> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 15, R2 = 2, LD = 929; };
 
 constexpr int kMixWaves = 4;  // wavefronts per block
 template <int N> constexpr int mix_slots() { return 64 / (N / MixPlan<N>::E); }  // transforms per wavefront
