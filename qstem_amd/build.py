@@ -27,7 +27,7 @@ def mix_sizes():
     """Lengths with a compiled mixed-radix plan: the QS_MIX_SIZES list of qs_internal.hpp."""
     import re
     text = open(os.path.join(CSRC, "qs_internal.hpp")).read()
-    m = re.search(r"#define QS_MIX_SIZES\(X\)((?: X\(\d+\))+)", text)
+    m = re.search(r"#define QS_MIX_SIZES\(X\)((?:[ \t]*\\\n|[ \t]*X\(\d+\))+)", text)
     if not m:
         raise RuntimeError("QS_MIX_SIZES not found in qs_internal.hpp")
     return tuple(int(n) for n in re.findall(r"X\((\d+)\)", m.group(1)))

@@ -3,7 +3,7 @@
 // The runtime-planned kernels of qs_generic.hip share every Stockham pass among all
 // threads of a block: two block barriers per pass, and threads idle wherever the
 // radix leaves the butterfly count short of the block (half of them in the radix-25
-// pass of 400).  For a length with a compiled plan of two or three passes, a
+// pass of 400).  For a length with a compiled plan of two to four passes, a
 // transform is owned by T = N/E lanes of one wavefront, each holding E values in
 // registers (element t + T m in v[m], the layout of coalesced row loads); the
 // exchanges between the passes go through the transform's own LDS slot and need
@@ -22,20 +22,25 @@
 
 namespace qs {
 
-// Compiled plans: E values per lane, NP passes of radix R0, R1 (, R2), each dividing
-// E (so every lane owns E/R butterflies of a pass); LD float2 per transform slot.
+// Compiled plans: E values per lane, NP passes of radix R0, R1 (, R2, R3), each
+// dividing E (so every lane owns E/R butterflies of a pass); LD float2 per transform
+// slot.
 template <int N> struct MixPlan { static constexpr int E = 0; };
-template <> struct MixPlan<288> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 6, R2 = 2, LD = 297; };
-template <> struct MixPlan<300> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 5, R2 = 2, LD = 309; };
-template <> struct MixPlan<320> { static constexpr int E = 20, NP = 3, R0 = 20, R1 = 4, R2 = 4, LD = 331; };
-template <> struct MixPlan<360> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 6, R2 = 2, LD = 371; };
-template <> struct MixPlan<384> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 8, R2 = 2, LD = 397; };
-template <> struct MixPlan<400> { static constexpr int E = 20, NP = 2, R0 = 20, R1 = 20, R2 = 1, LD = 420; };
-template <> struct MixPlan<441> { static constexpr int E = 21, NP = 3, R0 = 21, R1 = 7, R2 = 3, LD = 455; };
-template <> struct MixPlan<448> { static constexpr int E = 28, NP = 3, R0 = 28, R1 = 4, R2 = 4, LD = 463; };
-template <> struct MixPlan<600> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 10, R2 = 2, LD = 619; };
-template <> struct MixPlan<768> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 8, R2 = 4, LD = 793; };
-template <> struct MixPlan<900> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 15, R2 = 2, LD = 929; };
+template <> struct MixPlan<288> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 6, R2 = 2, R3 = 1, LD = 297; };
+template <> struct MixPlan<300> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 5, R2 = 2, R3 = 1, LD = 309; };
+template <> struct MixPlan<320> { static constexpr int E = 20, NP = 3, R0 = 20, R1 = 4, R2 = 4, R3 = 1, LD = 331; };
+template <> struct MixPlan<360> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 6, R2 = 2, R3 = 1, LD = 371; };
+template <> struct MixPlan<384> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 8, R2 = 2, R3 = 1, LD = 397; };
+template <> struct MixPlan<400> { static constexpr int E = 20, NP = 2, R0 = 20, R1 = 20, R2 = 1, R3 = 1, LD = 420; };
+template <> struct MixPlan<441> { static constexpr int E = 21, NP = 3, R0 = 21, R1 = 7, R2 = 3, R3 = 1, LD = 455; };
+template <> struct MixPlan<448> { static constexpr int E = 28, NP = 3, R0 = 28, R1 = 4, R2 = 4, R3 = 1, LD = 463; };
+template <> struct MixPlan<600> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 10, R2 = 2, R3 = 1, LD = 619; };
+template <> struct MixPlan<640> { static constexpr int E = 20, NP = 4, R0 = 20, R1 = 4, R2 = 4, R3 = 2, LD = 661; };
+template <> struct MixPlan<768> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 8, R2 = 4, R3 = 1, LD = 793; };
+template <> struct MixPlan<900> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 15, R2 = 2, R3 = 1, LD = 929; };
+template <> struct MixPlan<1280> { static constexpr int E = 20, NP = 4, R0 = 20, R1 = 4, R2 = 4, R3 = 4, LD = 1321; };
+template <> struct MixPlan<1536> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 8, R2 = 8, R3 = 1, LD = 1585; };
+template <> struct MixPlan<1800> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 30, R2 = 2, R3 = 1, LD = 1857; };
 
 constexpr int kMixWaves = 4;  // wavefronts per block
 template <int N> constexpr int mix_slots() { return 64 / (N / MixPlan<N>::E); }  // transforms per wavefront
@@ -49,7 +54,7 @@ template <int N> constexpr int mix_cpb() { return kMixWaves * mix_slots<N>(); } 
 //    and the second pass's reads (stride 21, odd) fall on distinct eight-byte bank
 //    pairs in every 16-lane write and 32-lane read group, across the three
 //    transforms of a wavefront too, and four blocks of twelve slots fit the CU's LDS.
-//  Three passes: i + i/32 (one float2 of padding every 32).
+//  Three or four passes: i + i/32 (one float2 of padding every 32).
 template <int N>
 __device__ __forceinline__ int mix_pos(int i) {
   using P = MixPlan<N>;
@@ -123,11 +128,17 @@ __device__ __forceinline__ void mix_fft(float2* v, float2* slot, const float2* _
     static_assert(P::R0 * P::R1 == N, "two-pass plan");
     mix_pass<N, P::R0, 1, DIR, true, false>(v, slot, tw, t);
     mix_pass<N, P::R1, P::R0, DIR, false, true>(v, slot, tw, t);
-  } else {
+  } else if constexpr (P::NP == 3) {
     static_assert(P::R0 * P::R1 * P::R2 == N, "three-pass plan");
     mix_pass<N, P::R0, 1, DIR, true, false>(v, slot, tw, t);
     mix_pass<N, P::R1, P::R0, DIR, false, false>(v, slot, tw, t);
     mix_pass<N, P::R2, P::R0 * P::R1, DIR, false, true>(v, slot, tw, t);
+  } else {
+    static_assert(P::NP == 4 && P::R0 * P::R1 * P::R2 * P::R3 == N, "four-pass plan");
+    mix_pass<N, P::R0, 1, DIR, true, false>(v, slot, tw, t);
+    mix_pass<N, P::R1, P::R0, DIR, false, false>(v, slot, tw, t);
+    mix_pass<N, P::R2, P::R0 * P::R1, DIR, false, false>(v, slot, tw, t);
+    mix_pass<N, P::R3, P::R0 * P::R1 * P::R2, DIR, false, true>(v, slot, tw, t);
   }
 }
 

@@ -208,7 +208,7 @@ def test_autotune_keeps_results():
 
 
 @pytest.mark.parametrize("shape", [(400, 400), (400, 160), (160, 400), (384, 384), (320, 448), (600, 768), (300, 441),
-                                   (96, 160), (200, 128), (112, 98), (136, 88)])
+                                   (1280, 640), (96, 160), (200, 128), (112, 98), (136, 88)])
 def test_generic_grid_sizes_match_oracle(oracle, shape):
     """Wave grids that are not powers of two (the reference's stem_STO_4x4.qsc uses
     400 x 400): sizes whose prime factors are 2, 3, 5, 7 take the fused mixed-radix
@@ -216,9 +216,9 @@ def test_generic_grid_sizes_match_oracle(oracle, shape):
     8*5*5, 112 = 4*4*7, 98 = 2*7*7), others (136 = 8*17, 88 = 8*11) the unfused path with
     Bluestein rows: probe, exit waves, every detector image (with a shifted detector),
     diffraction patterns and intensities against the oracle, and a run continued over a
-    second call equal to one call over all slices.  300, 320, 384, 400, 441, 448, 600
-    and 768 run the compiled plans (wavefront-resident kernels) on their axes, 160 the
-    runtime plan."""
+    second call equal to one call over all slices.  300, 320, 384, 400, 441, 448, 600,
+    640, 768 and 1280 run the compiled plans (wavefront-resident kernels) on their axes,
+    160 the runtime plan."""
     nx, ny = shape
     print("generic grid", shape, flush=True)
     cfg = small_config(nx=nx, ny=ny, pad=24, slices=4, dx=0.03)
@@ -253,11 +253,13 @@ def test_generic_grid_sizes_match_oracle(oracle, shape):
 
 
 @pytest.mark.parametrize("n,ndet", [(400, 4), (400, 6), (288, 4), (300, 4), (320, 4), (360, 4), (384, 6), (441, 4),
-                                    (448, 4), (600, 4), (768, 4), (900, 4)])
+                                    (448, 4), (600, 4), (640, 4), (768, 4), (900, 4), (1280, 4), (1536, 4),
+                                    (1800, 6)])
 def test_compiled_plans_equal_runtime_plans(oracle, monkeypatch, n, ndet):
     """Lengths with a compiled plan (400 = 20 x 20; 288 = 24 x 6 x 2, 300 = 30 x 5 x 2,
     320 = 20 x 4 x 4, 360 = 30 x 6 x 2, 384 = 24 x 8 x 2, 441 = 21 x 7 x 3, 448 = 28 x 4
-    x 4, 600 = 30 x 10 x 2, 768 = 24 x 8 x 4, 900 = 30 x 15 x 2: wavefront-resident
+    x 4, 600 = 30 x 10 x 2, 640 = 20 x 4 x 4 x 2, 768 = 24 x 8 x 4, 900 = 30 x 15 x 2,
+    1280 = 20 x 4 x 4 x 4, 1536 = 24 x 8 x 8, 1800 = 30 x 30 x 2: wavefront-resident
     kernels, the default); QSTEM_COMPILED_PLANS=0 at context creation keeps the runtime-planned
     kernels.  Both agree to FP32 rounding on every output, over 300 probes in groups of
     128 (the last group partial, rows and columns not multiples of a block's), with up

@@ -26,7 +26,7 @@ def _plans():
 
 def _sizes():
     text = open(os.path.join(CSRC, "qs_internal.hpp")).read()
-    m = re.search(r"#define QS_MIX_SIZES\(X\)((?: X\(\d+\))+)", text)
+    m = re.search(r"#define QS_MIX_SIZES\(X\)((?:[ \t]*\\\n|[ \t]*X\(\d+\))+)", text)
     return [int(n) for n in re.findall(r"X\((\d+)\)", m.group(1))]
 
 
@@ -42,7 +42,7 @@ def _pos(p, n, i):
 def _transform(p, n, x, sign):
     e = p["E"]
     t_n = n // e
-    radices = [p["R0"], p["R1"]] + ([p["R2"]] if p["NP"] == 3 else [])
+    radices = [p["R0"], p["R1"], p["R2"], p["R3"]][:p["NP"]]
     v = np.array([[x[t + t_n * m] for m in range(e)] for t in range(t_n)], dtype=complex)
     slot = np.full(p["LD"], np.nan + 0j)
     ns = 1
@@ -79,8 +79,9 @@ def test_table_matches_the_compiled_list():
 @pytest.mark.parametrize("n", sorted(PLANS))
 def test_plan_layout(n):
     p = PLANS[n]
-    radices = [p["R0"], p["R1"]] + ([p["R2"]] if p["NP"] == 3 else [])
-    assert np.prod(radices) == n and p["NP"] in (2, 3)
+    radices = [p["R0"], p["R1"], p["R2"], p["R3"]][:p["NP"]]
+    assert np.prod(radices) == n and p["NP"] in (2, 3, 4)
+    assert all(r == 1 for r in [p["R0"], p["R1"], p["R2"], p["R3"]][p["NP"]:])
     assert all(p["E"] % r == 0 for r in radices)
     t_n = n // p["E"]
     assert n % p["E"] == 0 and t_n <= 64 and p["E"] <= 32
