@@ -326,9 +326,10 @@ int qs_set_graph(qs_ctx *ctx, int on);
  * (not powers of two, prime factors 2, 3, 5, 7) the two values are the rows per block
  * of the row kernel and the columns per tile of the column kernel (powers of two up
  * to 16 that fit the block's LDS; QS_ERR_ARG otherwise); where a length has a compiled
- * plan (400: wavefront-resident kernels, the default unless QSTEM_COMPILED_PLANS=0 is
- * set when the context is created) the first is the rows per transform slot of the
- * row kernel and the second is unused.  No reference counterpart. */
+ * plan (288, 300, 320, 360, 384, 400, 441, 448, 600, 640, 768, 900, 1280, 1536, 1800:
+ * wavefront-resident kernels, the default unless QSTEM_COMPILED_PLANS=0 is set when
+ * the context is created) the first is the rows per transform slot of the row kernel
+ * and the second is unused.  No reference counterpart. */
 int qs_set_tuning(qs_ctx *ctx, int rows_per_wave, int tiles_per_block);
 int qs_get_tuning(const qs_ctx *ctx, int *rows_per_wave, int *tiles_per_block, int *batch);
 
