@@ -326,7 +326,8 @@ int qs_set_graph(qs_ctx *ctx, int on);
  * (not powers of two, prime factors 2, 3, 5, 7) the two values are the rows per block
  * of the row kernel and the columns per tile of the column kernel (powers of two up
  * to 16 that fit the block's LDS; QS_ERR_ARG otherwise); where a length has a compiled
- * plan (288, 300, 320, 360, 384, 400, 441, 448, 600, 640, 768, 900, 1280, 1536, 1800:
+ * plan (288, 300, 320, 360, 384, 400, 441, 448, 500, 540, 600, 640, 768, 900, 1280, 1350,
+ * 1536, 1800:
  * wavefront-resident kernels, the default unless QSTEM_COMPILED_PLANS=0 is set when
  * the context is created) the first is the rows per transform slot of the row kernel
  * and the second is unused.  No reference counterpart. */

@@ -201,9 +201,11 @@ bool gen_plan(int n, GenPlan* pl);  // false: n has a prime factor above 7
 constexpr int kGenPlanVariants = 4, kGenPlanCompiled = 3;
 bool gen_plan_variant(int n, int variant, GenPlan* pl);
 // Lengths with a compiled radix plan (qs_mix.hpp; one translation unit qs_mix.hip per
-// length, which the build reads from this list).
+// length, which the build reads from this list): those measured faster than the
+// runtime-planned kernels (720, 800, 1200 and 1620 have plans too and were not).
 #define QS_MIX_SIZES(X) \
-  X(288) X(300) X(320) X(360) X(384) X(400) X(441) X(448) X(600) X(640) X(768) X(900) X(1280) X(1536) X(1800)
+  X(288) X(300) X(320) X(360) X(384) X(400) X(441) X(448) X(500) X(540) X(600) X(640) X(768) X(900) X(1280) \
+  X(1350) X(1536) X(1800)
 bool mix_compiled(int n);         // n has a compiled radix plan (QS_MIX_SIZES)
 int mix_cols_per_block(int n);    // band columns per block of k_mix_cols<n>
 // a.rpb: rows per transform slot (k_mix_rows, ny = a.ny)

@@ -34,11 +34,14 @@ template <> struct MixPlan<384> { static constexpr int E = 24, NP = 3, R0 = 24, 
 template <> struct MixPlan<400> { static constexpr int E = 20, NP = 2, R0 = 20, R1 = 20, R2 = 1, R3 = 1, LD = 420; };
 template <> struct MixPlan<441> { static constexpr int E = 21, NP = 3, R0 = 21, R1 = 7, R2 = 3, R3 = 1, LD = 455; };
 template <> struct MixPlan<448> { static constexpr int E = 28, NP = 3, R0 = 28, R1 = 4, R2 = 4, R3 = 1, LD = 463; };
+template <> struct MixPlan<500> { static constexpr int E = 10, NP = 3, R0 = 10, R1 = 10, R2 = 5, R3 = 1, LD = 517; };
+template <> struct MixPlan<540> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 6, R2 = 3, R3 = 1, LD = 557; };
 template <> struct MixPlan<600> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 10, R2 = 2, R3 = 1, LD = 619; };
 template <> struct MixPlan<640> { static constexpr int E = 20, NP = 4, R0 = 20, R1 = 4, R2 = 4, R3 = 2, LD = 661; };
 template <> struct MixPlan<768> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 8, R2 = 4, R3 = 1, LD = 793; };
 template <> struct MixPlan<900> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 15, R2 = 2, R3 = 1, LD = 929; };
 template <> struct MixPlan<1280> { static constexpr int E = 20, NP = 4, R0 = 20, R1 = 4, R2 = 4, R3 = 4, LD = 1321; };
+template <> struct MixPlan<1350> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 15, R2 = 3, R3 = 1, LD = 1393; };
 template <> struct MixPlan<1536> { static constexpr int E = 24, NP = 3, R0 = 24, R1 = 8, R2 = 8, R3 = 1, LD = 1585; };
 template <> struct MixPlan<1800> { static constexpr int E = 30, NP = 3, R0 = 30, R1 = 30, R2 = 2, R3 = 1, LD = 1857; };
 

@@ -253,14 +253,13 @@ def test_generic_grid_sizes_match_oracle(oracle, shape):
 
 
 @pytest.mark.parametrize("n,ndet", [(400, 4), (400, 6), (288, 4), (300, 4), (320, 4), (360, 4), (384, 6), (441, 4),
-                                    (448, 4), (600, 4), (640, 4), (768, 4), (900, 4), (1280, 4), (1536, 4),
-                                    (1800, 6)])
+                                    (448, 4), (500, 4), (540, 4), (600, 4), (640, 4), (768, 4), (900, 4), (1280, 4),
+                                    (1350, 4), (1536, 4), (1800, 6)])
 def test_compiled_plans_equal_runtime_plans(oracle, monkeypatch, n, ndet):
-    """Lengths with a compiled plan (400 = 20 x 20; 288 = 24 x 6 x 2, 300 = 30 x 5 x 2,
-    320 = 20 x 4 x 4, 360 = 30 x 6 x 2, 384 = 24 x 8 x 2, 441 = 21 x 7 x 3, 448 = 28 x 4
-    x 4, 600 = 30 x 10 x 2, 640 = 20 x 4 x 4 x 2, 768 = 24 x 8 x 4, 900 = 30 x 15 x 2,
-    1280 = 20 x 4 x 4 x 4, 1536 = 24 x 8 x 8, 1800 = 30 x 30 x 2: wavefront-resident
-    kernels, the default); QSTEM_COMPILED_PLANS=0 at context creation keeps the runtime-planned
+    """Lengths with a compiled plan (the MixPlan table of qs_mix.hpp: 400 = 20 x 20, the
+    others three or four passes such as 384 = 24 x 8 x 2 or 1200 = 30 x 10 x 2 x 2;
+    wavefront-resident kernels, the default); QSTEM_COMPILED_PLANS=0 at context
+    creation keeps the runtime-planned
     kernels.  Both agree to FP32 rounding on every output, over 300 probes in groups of
     128 (the last group partial, rows and columns not multiples of a block's), with up
     to four detectors (the column kernel's small variant) and with six (two shifted)."""
