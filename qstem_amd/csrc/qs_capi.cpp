@@ -157,7 +157,6 @@ struct qs_ctx {
   int nstreams = 2;
   hipStream_t stream2 = nullptr;
   hipEvent_t evfork = nullptr, evjoin = nullptr;
-  hipEvent_t evgt[2] = {nullptr, nullptr}, evbl[2] = {nullptr, nullptr};  // adjoint: dL/dt buffers ready / consumed
   float last_ms = 0.f;
   std::string err;
   int batch = 0;
@@ -205,7 +204,7 @@ struct qs_ctx {
     bool on = false;
     int band_a = 0, band_b = 0, kc = 0, kcp = 0, ntiles = 0;
   } bl;
-  DevBuf blpx, blpy, blstate, blstate2;
+  DevBuf blpx, blpy, blstate;
   // potential builder
   std::vector<double> scat;       // 93 x 123 Rez table
   std::vector<double> scat_offs;  // 93 x 123 ion-offset table (scatParOffs)
@@ -229,7 +228,7 @@ struct qs_ctx {
   bool have_probe = false;
   // adjoint
   std::vector<DevBuf> ckpt;
-  DevBuf gx, target, lossp, gT, gT2, gprobe, gV, dsum, adj_ipos;
+  DevBuf gx, target, lossp, gT, gprobe, gV, dsum, adj_ipos;
   // forward pass kept by qs_forward_save for qs_adjoint_saved (checkpoints, the
   // adjoint seed in gx, the loss); cleared by every change of potential, probe or
   // optics (with the graph) and by the reverse pass itself
@@ -454,11 +453,7 @@ int qs_create(const qs_params* p, qs_ctx** out) {
   if (hipStreamCreate(&c->stream) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
       hipEventCreate(&c->ev1) != hipSuccess || hipStreamCreate(&c->stream2) != hipSuccess ||
       hipEventCreateWithFlags(&c->evfork, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evjoin, hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evgt[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evgt[1], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evbl[0], hipEventDisableTiming) != hipSuccess ||
-      hipEventCreateWithFlags(&c->evbl[1], hipEventDisableTiming) != hipSuccess) {
+      hipEventCreateWithFlags(&c->evjoin, hipEventDisableTiming) != hipSuccess) {
     c->err = "hipStreamCreate/hipEventCreate failed (no HIP device?)";
     return bad(QS_ERR_HIP);
   }
@@ -638,8 +633,8 @@ void qs_destroy(qs_ctx* c) {
                     &c->reach, &c->tcount, &c->tstart, &c->tlist,
                     &c->lx.tw, &c->lx.chirp_f, &c->lx.vhat_f, &c->lx.chirp_b, &c->lx.vhat_b,
                     &c->lz.tw, &c->lz.chirp_f, &c->lz.vhat_f, &c->lz.chirp_b, &c->lz.vhat_b,
-                    &c->gx, &c->target, &c->lossp, &c->gT, &c->gT2, &c->gprobe, &c->gV, &c->dsum,
-                    &c->blpx, &c->blpy, &c->blstate, &c->blstate2, &c->adj_ipos,
+                    &c->gx, &c->target, &c->lossp, &c->gT, &c->gprobe, &c->gV, &c->dsum,
+                    &c->blpx, &c->blpy, &c->blstate, &c->adj_ipos,
                     &c->wx.tw, &c->wx.chirp_f, &c->wx.vhat_f, &c->wx.chirp_b, &c->wx.vhat_b,
                     &c->wy.tw, &c->wy.chirp_f, &c->wy.vhat_f, &c->wy.chirp_b, &c->wy.vhat_b, &c->gscratch, &c->gtiles};
   for (DevBuf* b : bufs) b->release();
@@ -652,10 +647,6 @@ void qs_destroy(qs_ctx* c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->evfork) (void)hipEventDestroy(c->evfork);
   if (c->evjoin) (void)hipEventDestroy(c->evjoin);
-  for (int k = 0; k < 2; k++) {
-    if (c->evgt[k]) (void)hipEventDestroy(c->evgt[k]);
-    if (c->evbl[k]) (void)hipEventDestroy(c->evbl[k]);
-  }
   if (c->stream) (void)hipStreamDestroy(c->stream);
   if (c->stream2) (void)hipStreamDestroy(c->stream2);
   delete c;
@@ -672,8 +663,10 @@ static constexpr size_t kBandScratchBytes = size_t(1) << 30;
 // FFT / propagator 1/(nx ny) inside the disk k^2 < k2max / column IFFT, row IFFT
 // written back to `tr` (rows mode 2) or, with gv given, turned into
 // dL/dV = sigma Im(g conj(exp(i sigma V))) of the slices' potential vpot (mode 4).
+// rows_done (one slice, gv given): the first pass already ran (the adjoint's row sweep
+// stored the row FFT band-compact into *scratch); tr is not used.
 static int bl_sweeps(qs_ctx* c, float2* tr, int nslice, const float* vpot, float* gv, float sigma,
-                     hipStream_t st = nullptr, DevBuf* scratch = nullptr) {
+                     hipStream_t st = nullptr, DevBuf* scratch = nullptr, bool rows_done = false) {
   const qs_params& p = c->p;
   const int nx = p.potNx, ny = p.potNy;
   const size_t slice = (size_t)nx * ny;
@@ -681,6 +674,10 @@ static int bl_sweeps(qs_ctx* c, float2* tr, int nslice, const float* vpot, float
   const int q = (int)std::max<size_t>(1, std::min<size_t>((size_t)nslice, kBandScratchBytes / per));
   if (!st) st = c->stream;
   if (!scratch) scratch = &c->blstate;
+  if (rows_done && (nslice != 1 || !gv || !scratch->p || scratch->bytes < per)) {
+    c->err = "band limit: the fused first pass needs one slice, dL/dV and its state";
+    return QS_ERR_STATE;
+  }
   HIPCHK(c, scratch->ensure(per * q));
   qs::SweepArgs a{};
   a.nx = nx;
@@ -703,13 +700,13 @@ static int bl_sweeps(qs_ctx* c, float2* tr, int nslice, const float* vpot, float
   a.gv_sigma = sigma;
   for (int s0 = 0; s0 < nslice; s0 += q) {
     a.nprobe = std::min(q, nslice - s0);
-    a.probe = tr + (size_t)s0 * slice;
-    a.out_wave = tr + (size_t)s0 * slice;
+    a.probe = tr ? tr + (size_t)s0 * slice : nullptr;
+    a.out_wave = tr ? tr + (size_t)s0 * slice : nullptr;
     if (gv) {
       a.vpot = vpot + (size_t)s0 * slice;
       a.gv = gv + (size_t)s0 * slice;
     }
-    qs::launch_sweep_rows(a, 3, st);
+    if (!rows_done) qs::launch_sweep_rows(a, 3, st);
     qs::launch_sweep_cols(a, false, false, st);
     qs::launch_sweep_rows(a, gv ? 4 : 2, st);
   }
@@ -2107,7 +2104,7 @@ static int adj_forward(qs_ctx* c, const char* who, const int* ipos, int nprobe, 
   qs::launch_fill_zero(A.gx, st_elems, c->stream);
   A.f.trans = trans_base(c) + (size_t)(S - 1) * slice_elems;
   A.xprev = c->state.as<float2>();
-  qs::launch_adj_cols(A, true, false, c->stream);
+  qs::launch_adj_cols(A, true, c->stream);
   // loss = sum over band pixels (partials) + sum of target^2 outside the band
   qs::launch_target_outside_band(c->target.as<float>(), nprobe, p.nx, p.ny, c->band_a, c->band_b,
                                  c->dsum.as<double>() + 2, c->stream);
@@ -2158,16 +2155,25 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
   const float mm0 = 1.0F + p.v0 / 511.0F;
   const double sigma = mm0 * wavelength(p.v0);
   HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * gslices));
-  // band-limited dL/dV slice by slice: the band limit of slice s runs on the second
-  // stream while the reverse sweeps of slice s-1 run on the first, from two dL/dt
-  // buffers used in turn
-  const bool overlap = grad_each && p.bandlimittrans && c->nstreams > 1 && S > 1;
-  if (overlap) HIPCHK(c, c->gT2.ensure(sizeof(float2) * slice_elems));
-  bool pending[2] = {false, false};  // a band limit on the second stream still reads buffer k
+  // band-limited dL/dV slice by slice, on the context stream after the slice's reverse
+  // sweeps (its first pass fused into the reverse row sweep where the window is the
+  // whole slice)
   if (grad_V) HIPCHK(c, c->gV.ensure(sizeof(float) * slice_elems * S));
   HIPCHK(c, c->gprobe.ensure(sizeof(float2) * (size_t)p.nx * p.ny));
   qs::AdjArgs A = adj_args(c, nprobe);
   const bool whole = A.single_writer == 2;
+  // one window covers the slice and the band limit runs as band-compact sweeps: each
+  // row of dL/dt is complete in the reverse row sweep, which then also does the band
+  // limit's first pass (row FFT into the band-compact state); its buffers are allocated
+  // before the loop (the row sweep writes them)
+  const bool fuse = grad_each && p.bandlimittrans && c->bl.on && whole;
+  if (fuse) {
+    const size_t bl_per = sizeof(float2) * (size_t)p.potNx * c->bl.kcp;
+    HIPCHK(c, c->blstate.ensure(std::max(c->blstate.bytes, bl_per)));
+    A.bl_band_a = c->bl.band_a;
+    A.bl_gap = p.potNy - c->bl.band_b - c->bl.band_a - 1;
+    A.bl_kcp = c->bl.kcp;
+  }
   HIPCHK(c, hipEventRecord(c->ev0, c->stream));
   if (!whole && gslices > 1) qs::launch_cplx_fill_zero(c->gT.as<float>(), 2 * slice_elems * S, c->stream);
   qs::launch_cplx_fill_zero(c->gprobe.as<float>(), 2 * (size_t)p.nx * p.ny, c->stream);
@@ -2186,28 +2192,20 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
       prev = w.state;
     }
     for (int s = s1; s >= s0; s--) {
-      const int b = overlap ? (s & 1) : 0;
       A.f.trans = trans_base(c) + (size_t)s * slice_elems;
-      A.gT = (b ? c->gT2 : c->gT).as<float>() + (gslices > 1 ? 2 * (size_t)s * slice_elems : 0);
+      A.gT = c->gT.as<float>() + (gslices > 1 ? 2 * (size_t)s * slice_elems : 0);
+      A.blstate = fuse ? c->blstate.as<float2>() : nullptr;
       A.xprev = nullptr;
-      if (s < S - 1) qs::launch_adj_cols(A, false, overlap, c->stream);
+      if (s < S - 1) qs::launch_adj_cols(A, false, c->stream);
       if (s > 0) A.xprev = (s - 1 >= s0) ? c->ckpt[nseg + (s - 1 - s0)].as<float2>() : c->ckpt[j].as<float2>();
-      if (pending[b]) HIPCHK(c, hipStreamWaitEvent(c->stream, c->evbl[b], 0));  // slice s+2's band limit read it
       if (grad_each && !whole) qs::launch_cplx_fill_zero(A.gT, 2 * slice_elems, c->stream);
       qs::launch_adj_rows(A, s == 0, c->stream);
       if (grad_each) {
         const float* vs = c->V.as<float>() + (size_t)s * slice_elems;
         float* gs = c->gV.as<float>() + (size_t)s * slice_elems;
-        if (overlap) {
-          HIPCHK(c, hipEventRecord(c->evgt[b], c->stream));
-          HIPCHK(c, hipStreamWaitEvent(c->stream2, c->evgt[b], 0));
-          const int rc = bl_sweeps(c, reinterpret_cast<float2*>(A.gT), 1, vs, gs, (float)sigma, c->stream2,
-                                   b ? &c->blstate2 : &c->blstate);
-          if (rc != QS_OK) return rc;
-          HIPCHK(c, hipEventRecord(c->evbl[b], c->stream2));
-          pending[b] = true;
-        } else if (p.bandlimittrans) {
-          const int rc = bl_sweeps(c, reinterpret_cast<float2*>(A.gT), 1, vs, gs, (float)sigma);
+        if (p.bandlimittrans) {
+          const int rc = bl_sweeps(c, fuse ? nullptr : reinterpret_cast<float2*>(A.gT), 1, vs, gs, (float)sigma,
+                                   nullptr, nullptr, fuse);
           if (rc != QS_OK) return rc;
         } else {
           qs::launch_grad_v(reinterpret_cast<const float2*>(A.gT), vs, gs, slice_elems, sigma, c->stream);
@@ -2216,10 +2214,6 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
     }
   }
   HIPCHK(c, hipGetLastError());
-  if (overlap) {  // join: the first stream waits for the last band limits
-    HIPCHK(c, hipEventRecord(c->evjoin, c->stream2));
-    HIPCHK(c, hipStreamWaitEvent(c->stream, c->evjoin, 0));
-  }
   // dL/dV of every slice at once: band limit (self-adjoint) of dL/dt, then sigma Im(g conj(u))
   if (grad_V && !grad_each) {
     int rc = bandlimit_buf(c, c->gT.as<float2>(), S);

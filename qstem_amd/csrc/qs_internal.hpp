@@ -60,6 +60,12 @@ struct AdjArgs {
   float* gprobe;            // [nx][ny][2] sum over probes of the gradient w.r.t. psi_0 (atomics)
   int single_writer;        // 1: one probe per launch, gT updated without atomics;
                             // 2: and its window is the whole slice, gT stored (no prior clear)
+  // With single_writer 2 and the transmission band limit on band-compact sweeps: the
+  // row sweep stores the row FFT of each dL/dt row, band-compact in the potential
+  // grid's band (bl_band_a, bl_gap, rows of bl_kcp), into blstate instead of the row
+  // into gT (the first pass of the band limit); nullptr: gT as above.
+  float2* blstate;
+  int bl_band_a, bl_gap, bl_kcp;
 };
 
 struct ProbeArgs {
@@ -251,9 +257,7 @@ void launch_probe_window(const ProbeArgs& a, float2* psi, double* partial, hipSt
 void launch_fill_zero(float2* p, size_t n, hipStream_t st);
 hipError_t set_col_lds_limits();
 void launch_adj_rows(const AdjArgs& a, bool first_slice, hipStream_t st);
-// direct: allow the register-staged variant where the size has one (chosen when the
-// band limit of dL/dt runs beside the reverse sweeps on the second stream)
-void launch_adj_cols(const AdjArgs& a, bool last_slice, bool direct, hipStream_t st);
+void launch_adj_cols(const AdjArgs& a, bool last_slice, hipStream_t st);
 void launch_grad_v(const float2* gu, const float* V, float* gV, size_t n, double sigma, hipStream_t st);
 void launch_target_outside_band(const float* target, int nprobe, int nx, int ny, int band_a, int band_b,
                                 double* out, hipStream_t st);

@@ -163,7 +163,7 @@ __global__ void k_fill_zero(float2* __restrict__ p, size_t n) {
   void rows_fft_##N(float2*, long, int, const float2*, float, hipStream_t);                         \
   void bluestein_##N(float2*, long, int, const float2*, const float2*, const float2*, float, hipStream_t); \
   void adj_rows_##N(const AdjArgs&, bool, hipStream_t);                                             \
-  void adj_cols_##N(const AdjArgs&, bool, bool, hipStream_t);
+  void adj_cols_##N(const AdjArgs&, bool, hipStream_t);
 QS_DECL(64)
 QS_DECL(128)
 QS_DECL(256)
@@ -234,8 +234,8 @@ void launch_adj_rows(const AdjArgs& a, bool first_slice, hipStream_t st) {
 #undef C
 }
 
-void launch_adj_cols(const AdjArgs& a, bool last_slice, bool direct, hipStream_t st) {
-#define C(N) adj_cols_##N(a, last_slice, direct, st)
+void launch_adj_cols(const AdjArgs& a, bool last_slice, hipStream_t st) {
+#define C(N) adj_cols_##N(a, last_slice, st)
   QS_SWITCH(a.f.nx, C)
 #undef C
 }

@@ -18,9 +18,7 @@ hipError_t QS_CAT(set_col_lds_, QS_N)() {
   return e == hipSuccess ? set_adj_lds_t<QS_N>() : e;
 }
 void QS_CAT(adj_rows_, QS_N)(const AdjArgs& a, bool first, hipStream_t st) { launch_adj_rows_t<QS_N>(a, first, st); }
-void QS_CAT(adj_cols_, QS_N)(const AdjArgs& a, bool last, bool direct, hipStream_t st) {
-  launch_adj_cols_t<QS_N>(a, last, direct, st);
-}
+void QS_CAT(adj_cols_, QS_N)(const AdjArgs& a, bool last, hipStream_t st) { launch_adj_cols_t<QS_N>(a, last, st); }
 int QS_CAT(col_tile_, QS_N)() { return ColCfg<QS_N>::TC; }
 void QS_CAT(rows_fft_, QS_N)(float2* d, long nrows, int dir, const float2* tw, float scale, hipStream_t st) {
   if (dir < 0) rows_fft_t<QS_N, -1>(d, nrows, tw, scale, st);

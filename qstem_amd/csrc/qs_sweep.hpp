@@ -45,6 +45,16 @@ __device__ __forceinline__ void buf_store_c64(float2 v, rsrc_t r, unsigned off) 
   using U2 = decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0u, 0, 0));
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(U2, v), r, off, 0, 0);
 }
+// 16-byte store with the whole offset in the VGPR offset and a zero soffset.  A MUBUF
+// store of more than 8 bytes reads its data VGPRs after issue, and the compiler puts
+// wait states before a later VALU write of those VGPRs only when the soffset is not an
+// SGPR; with an SGPR soffset it scheduled such writes directly after the store, and on
+// gfx950 the store then sometimes wrote the new values (LDS addresses in place of the
+// wave data, a few elements per launch, differing between runs).
+__device__ __forceinline__ void buf_store_f4(float4 v, rsrc_t r, unsigned off) {
+  using U4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(r, 0u, 0, 0));
+  __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, v), r, off, 0, 0);
+}
 
 // Byte offset of ky column iy inside a band-compact row, or kOut when iy lies in
 // the zero gap (band_a, ny-band_b) between the two halves of the band;
@@ -273,23 +283,18 @@ struct TileStage {
   // columns c, c+1 of row x0 + XS*i from u[i], w[i]
   __device__ __forceinline__ void out_regs(const float2* u, const float2* w, int c0) const {
     const unsigned co = (unsigned)c0 * 8u;
-    using U4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(rs, 0u, 0, 0));
 #pragma unroll
-    for (int i = 0; i < IT; i++)
-      __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, make_float4(u[i].x, u[i].y, w[i].x, w[i].y)), rs,
-                                             voff + co, i * rstep, 0);
+    for (int i = 0; i < IT; i++) buf_store_f4(make_float4(u[i].x, u[i].y, w[i].x, w[i].y), rs, voff + co + i * rstep);
   }
   __device__ __forceinline__ void out(const float2* tile, int c0) const {
     const unsigned co = (unsigned)c0 * 8u;
-    using U4 = decltype(__builtin_amdgcn_raw_buffer_load_b128(rs, 0u, 0, 0));
 #pragma unroll
     for (int i0 = 0; i0 < IT; i0 += IB) {
       if (i0 > 0) __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int i = i0; i < i0 + IB; i++) {
         const float2 u = tile[lbase + lstep(i)], w = tile[lbase + ST + lstep(i)];
-        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(U4, make_float4(u.x, u.y, w.x, w.y)), rs, voff + co,
-                                               i * rstep, 0);
+        buf_store_f4(make_float4(u.x, u.y, w.x, w.y), rs, voff + co + i * rstep);
       }
     }
   }
@@ -649,10 +654,7 @@ void blue_t(float2* d, long nrows, int n, const float2* chirp, const float2* vha
 //
 // Column sweep (slice s, reverse):  g_Phi = colFFT(g_X)  [last slice: 4 (I - T) Phi,
 // Phi = colFFT(X_{S-1})/nx, I = |Phi|^2]  ->  g_Psi = conj(P) g_Phi  ->  g_R = colIFFT(g_Psi).
-// DIRECT: at sizes where col_direct holds, the non-last sweep takes its staging loads
-// straight into the transform registers (one LDS column slot); measured faster only with
-// the band limit of dL/dt running beside it on the second stream, slower alone.
-template <int NX, bool LAST, bool DIRECT>
+template <int NX, bool LAST>
 __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_adj_cols(AdjArgs A) {
   const SweepArgs& a = A.f;
   constexpr int E = Plan<NX>::E;
@@ -669,46 +671,6 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_adj_cols(AdjArgs A) {
   const int pg = a.probe0 + p;
   const int c0 = tile_id * TC;
   const size_t off = (size_t)pg * a.nx * a.kcp;
-  if constexpr (col_direct<NX>() && !LAST && DIRECT) {
-    // as the forward column sweep at this size: the staging loads are the transform's
-    // registers, one LDS column slot for the exchanges
-    const int t = threadIdx.x % T;
-    Twiddles<NX> tw;
-    tw.init(a.twx, t);
-    const TileStage<NX> st(A.gx + off, a.nx, a.kcp);
-    float4 q[E];
-    st.in_regs(q, c0);
-    float2 u[E], w[E];
-#pragma unroll
-    for (int m = 0; m < E; m++) {
-      u[m] = make_float2(q[m].x, q[m].y);
-      w[m] = make_float2(q[m].z, q[m].w);
-    }
-    auto column = [&](float2 (&v)[E], int c) __attribute__((always_inline)) {
-      const int iy = c <= a.band_a ? c : (c < a.kc ? c - (a.band_a + 1) + (a.ny - a.band_b) : -1);
-      fft_wave<NX, -1, false, 0, -1, ColCfg<NX>::TWC>(v, tile, tw, t);
-      if (iy >= 0) {
-        const float ky2 = a.ky2[iy];
-        const float2 py = a.py[iy];
-#pragma unroll
-        for (int m = 0; m < E; m++) {
-          const int ix = t + T * m;
-          const float kx2 = a.kx2[ix];
-          const bool in = kx2 < a.k2max && (kx2 + ky2) < a.k2max;
-          v[m] = in ? cmulc(cmulc(v[m], py), a.px[ix]) : make_float2(0.f, 0.f);
-        }
-      } else {
-#pragma unroll
-        for (int m = 0; m < E; m++) v[m] = make_float2(0.f, 0.f);
-      }
-      fft_wave<NX, +1, false, 0, -1, ColCfg<NX>::TWC>(v, tile, tw, t);
-    };
-    column(u, c0);
-    xform_sync<NX>();  // the second column reuses the slot
-    column(w, c0 + 1);
-    st.out_regs(u, w, c0);
-    return;
-  }
   if (threadIdx.x == 0) red = 0.f;
   TileStage<NX>((LAST ? A.xprev : A.gx) + off, a.nx, a.kcp).in(tile, c0);
   __syncthreads();
@@ -767,8 +729,10 @@ __global__ __launch_bounds__(ColCfg<NX>::BLK) void k_adj_cols(AdjArgs A) {
 
 // Row sweep (slice s, reverse):  g_a = rowIFFT(g_R);  psi_s = rowIFFT(X_{s-1}) (or the
 // probe);  g_t += conj(psi_s) g_a;  g_psi = conj(t_s) g_a;  g_X_{s-1} = rowFFT(g_psi)
-// (s > 0) or sum_p g_psi -> gprobe (s = 0).
-template <int NY, bool FIRST>
+// (s > 0) or sum_p g_psi -> gprobe (s = 0).  BLF (the window is the whole slice): the
+// row of g_t is complete here, so its row FFT, band-compact in the potential grid's
+// band, goes to A.blstate (the band limit's first pass) instead of the row to A.gT.
+template <int NY, bool FIRST, bool BLF>
 __global__ __launch_bounds__(row_block<NY>()) void k_adj_rows(AdjArgs A) {
   const SweepArgs& a = A.f;
   constexpr int E = Plan<NY>::E;
@@ -814,7 +778,9 @@ __global__ __launch_bounds__(row_block<NY>()) void k_adj_rows(AdjArgs A) {
   for (int m = 0; m < E; m++) {
     const int iy = t + T * m;
     const float2 gtv = cmulc(g[m], w[m]);  // conj(psi) * g_a
-    if (A.single_writer == 2) {  // the window is the whole slice: this is the pixel's only term
+    if constexpr (BLF) {
+      w[m] = gtv;  // kept for the row FFT below
+    } else if (A.single_writer == 2) {  // the window is the whole slice: this is the pixel's only term
       *reinterpret_cast<float2*>(gt + 2 * iy) = gtv;
     } else if (A.single_writer) {  // one probe per launch: each transmission pixel has one writer
       float2* p2 = reinterpret_cast<float2*>(gt + 2 * iy);
@@ -825,6 +791,14 @@ __global__ __launch_bounds__(row_block<NY>()) void k_adj_rows(AdjArgs A) {
       atomicAdd(gt + 2 * iy + 1, gtv.y);
     }
     g[m] = cmulc(g[m], tr[iy]);  // conj(t) * g_a
+  }
+  if constexpr (BLF) {
+    // the band limit's first pass (k_sweep_rows mode 3) on the row x of g_t: row FFT,
+    // band columns of the potential grid stored (offsets in its band gap are kOut)
+    fft_wave<NY, -1>(w, slot, tw, t);
+    const rsrc_t rb = make_rsrc(A.blstate + (size_t)(x + ip.x) * A.bl_kcp, (long)A.bl_kcp * 8);
+#pragma unroll
+    for (int m = 0; m < E; m++) buf_store_c64(w[m], rb, band_off(t + T * m, A.bl_band_a, A.bl_gap));
   }
   if constexpr (FIRST) {
     float* gp = A.gprobe + 2 * (size_t)x * NY;
@@ -846,30 +820,30 @@ void launch_adj_rows_t(const AdjArgs& a, bool first, hipStream_t st) {
   constexpr int RPB = row_block<N>() / T;
   const long rows = (long)a.f.nprobe * a.f.nx;
   dim3 grid((unsigned)((rows + RPB - 1) / RPB));
-  if (first) hipLaunchKernelGGL((k_adj_rows<N, true>), grid, dim3(row_block<N>()), 0, st, a);
-  else hipLaunchKernelGGL((k_adj_rows<N, false>), grid, dim3(row_block<N>()), 0, st, a);
+  if (a.blstate) {
+    if (first) hipLaunchKernelGGL((k_adj_rows<N, true, true>), grid, dim3(row_block<N>()), 0, st, a);
+    else hipLaunchKernelGGL((k_adj_rows<N, false, true>), grid, dim3(row_block<N>()), 0, st, a);
+  } else {
+    if (first) hipLaunchKernelGGL((k_adj_rows<N, true, false>), grid, dim3(row_block<N>()), 0, st, a);
+    else hipLaunchKernelGGL((k_adj_rows<N, false, false>), grid, dim3(row_block<N>()), 0, st, a);
+  }
 }
 
 template <int N>
-void launch_adj_cols_t(const AdjArgs& a, bool last, bool direct, hipStream_t st) {
-  const bool dir = col_direct<N>() && !last && direct;
-  const size_t lds = dir ? sizeof(float2) * ColCfg<N>::STRIDE : col_lds_bytes_t<N>();
+void launch_adj_cols_t(const AdjArgs& a, bool last, hipStream_t st) {
+  const size_t lds = col_lds_bytes_t<N>();
   dim3 grid((unsigned)(a.f.nprobe * a.f.ntiles));
-  if (last) hipLaunchKernelGGL((k_adj_cols<N, true, false>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
-  else if (dir) hipLaunchKernelGGL((k_adj_cols<N, false, true>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
-  else hipLaunchKernelGGL((k_adj_cols<N, false, false>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
+  if (last) hipLaunchKernelGGL((k_adj_cols<N, true>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
+  else hipLaunchKernelGGL((k_adj_cols<N, false>), grid, dim3(ColCfg<N>::BLK), lds, st, a);
 }
 
 template <int N>
 hipError_t set_adj_lds_t() {
   const int b = col_lds_bytes_t<N>();
-  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, true, false>),
+  hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, true>),
                                      hipFuncAttributeMaxDynamicSharedMemorySize, b);
   if (e == hipSuccess)
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, false, false>),
-                            hipFuncAttributeMaxDynamicSharedMemorySize, b);
-  if (e == hipSuccess && col_direct<N>())
-    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, false, true>),
+    e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_adj_cols<N, false>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, b);
   return e;
 }
