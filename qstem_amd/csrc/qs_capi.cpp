@@ -229,6 +229,10 @@ struct qs_ctx {
   // adjoint
   std::vector<DevBuf> ckpt;
   DevBuf gx, target, lossp, gT, gprobe, gV, dsum, adj_ipos;
+  // band limit of dL/dt on the second stream: a second band-compact buffer, events
+  // "row sweep wrote buffer b" and "band limit done with buffer b"
+  DevBuf blstate2;
+  hipEvent_t evrow[2] = {nullptr, nullptr}, evbl[2] = {nullptr, nullptr};
   // forward pass kept by qs_forward_save for qs_adjoint_saved (checkpoints, the
   // adjoint seed in gx, the loss); cleared by every change of potential, probe or
   // optics (with the graph) and by the reverse pass itself
@@ -634,7 +638,7 @@ void qs_destroy(qs_ctx* c) {
                     &c->lx.tw, &c->lx.chirp_f, &c->lx.vhat_f, &c->lx.chirp_b, &c->lx.vhat_b,
                     &c->lz.tw, &c->lz.chirp_f, &c->lz.vhat_f, &c->lz.chirp_b, &c->lz.vhat_b,
                     &c->gx, &c->target, &c->lossp, &c->gT, &c->gprobe, &c->gV, &c->dsum,
-                    &c->blpx, &c->blpy, &c->blstate, &c->adj_ipos,
+                    &c->blpx, &c->blpy, &c->blstate, &c->blstate2, &c->adj_ipos,
                     &c->wx.tw, &c->wx.chirp_f, &c->wx.vhat_f, &c->wx.chirp_b, &c->wx.vhat_b,
                     &c->wy.tw, &c->wy.chirp_f, &c->wy.vhat_f, &c->wy.chirp_b, &c->wy.vhat_b, &c->gscratch, &c->gtiles};
   for (DevBuf* b : bufs) b->release();
@@ -643,6 +647,10 @@ void qs_destroy(qs_ctx* c) {
   c->olutptrs.release();
   for (DevBuf& b : c->ckpt) b.release();
   for (hipEvent_t e : c->evpool) (void)hipEventDestroy(e);
+  for (int b = 0; b < 2; b++) {
+    if (c->evrow[b]) (void)hipEventDestroy(c->evrow[b]);
+    if (c->evbl[b]) (void)hipEventDestroy(c->evbl[b]);
+  }
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->evfork) (void)hipEventDestroy(c->evfork);
@@ -2155,9 +2163,10 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
   const float mm0 = 1.0F + p.v0 / 511.0F;
   const double sigma = mm0 * wavelength(p.v0);
   HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * gslices));
-  // band-limited dL/dV slice by slice, on the context stream after the slice's reverse
-  // sweeps (its first pass fused into the reverse row sweep where the window is the
-  // whole slice)
+  // band-limited dL/dV slice by slice after the slice's reverse sweeps (its first pass
+  // fused into the reverse row sweep where the window is the whole slice; then the rest
+  // runs on the second stream beside the reverse sweeps of slice s-1, from two
+  // band-compact buffers used in turn)
   if (grad_V) HIPCHK(c, c->gV.ensure(sizeof(float) * slice_elems * S));
   HIPCHK(c, c->gprobe.ensure(sizeof(float2) * (size_t)p.nx * p.ny));
   qs::AdjArgs A = adj_args(c, nprobe);
@@ -2167,9 +2176,18 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
   // limit's first pass (row FFT into the band-compact state); its buffers are allocated
   // before the loop (the row sweep writes them)
   const bool fuse = grad_each && p.bandlimittrans && c->bl.on && whole;
+  const bool overlap = fuse && c->nstreams > 1 && S > 1;
+  bool pending[2] = {false, false};  // the band limit on the second stream still reads buffer b
   if (fuse) {
     const size_t bl_per = sizeof(float2) * (size_t)p.potNx * c->bl.kcp;
     HIPCHK(c, c->blstate.ensure(std::max(c->blstate.bytes, bl_per)));
+    if (overlap) {
+      HIPCHK(c, c->blstate2.ensure(bl_per));
+      for (int b = 0; b < 2; b++) {
+        if (!c->evrow[b]) HIPCHK(c, hipEventCreateWithFlags(&c->evrow[b], hipEventDisableTiming));
+        if (!c->evbl[b]) HIPCHK(c, hipEventCreateWithFlags(&c->evbl[b], hipEventDisableTiming));
+      }
+    }
     A.bl_band_a = c->bl.band_a;
     A.bl_gap = p.potNy - c->bl.band_b - c->bl.band_a - 1;
     A.bl_kcp = c->bl.kcp;
@@ -2192,18 +2210,28 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
       prev = w.state;
     }
     for (int s = s1; s >= s0; s--) {
+      const int b = overlap ? (s & 1) : 0;
+      DevBuf& bls = b ? c->blstate2 : c->blstate;
       A.f.trans = trans_base(c) + (size_t)s * slice_elems;
       A.gT = c->gT.as<float>() + (gslices > 1 ? 2 * (size_t)s * slice_elems : 0);
-      A.blstate = fuse ? c->blstate.as<float2>() : nullptr;
+      A.blstate = fuse ? bls.as<float2>() : nullptr;
       A.xprev = nullptr;
       if (s < S - 1) qs::launch_adj_cols(A, false, c->stream);
       if (s > 0) A.xprev = (s - 1 >= s0) ? c->ckpt[nseg + (s - 1 - s0)].as<float2>() : c->ckpt[j].as<float2>();
       if (grad_each && !whole) qs::launch_cplx_fill_zero(A.gT, 2 * slice_elems, c->stream);
+      if (pending[b]) HIPCHK(c, hipStreamWaitEvent(c->stream, c->evbl[b], 0));  // slice s+2's band limit read it
       qs::launch_adj_rows(A, s == 0, c->stream);
       if (grad_each) {
         const float* vs = c->V.as<float>() + (size_t)s * slice_elems;
         float* gs = c->gV.as<float>() + (size_t)s * slice_elems;
-        if (p.bandlimittrans) {
+        if (overlap) {
+          HIPCHK(c, hipEventRecord(c->evrow[b], c->stream));
+          HIPCHK(c, hipStreamWaitEvent(c->stream2, c->evrow[b], 0));
+          const int rc = bl_sweeps(c, nullptr, 1, vs, gs, (float)sigma, c->stream2, &bls, true);
+          if (rc != QS_OK) return rc;
+          HIPCHK(c, hipEventRecord(c->evbl[b], c->stream2));
+          pending[b] = true;
+        } else if (p.bandlimittrans) {
           const int rc = bl_sweeps(c, fuse ? nullptr : reinterpret_cast<float2*>(A.gT), 1, vs, gs, (float)sigma,
                                    nullptr, nullptr, fuse);
           if (rc != QS_OK) return rc;
@@ -2214,6 +2242,10 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
     }
   }
   HIPCHK(c, hipGetLastError());
+  if (overlap) {  // join: the first stream waits for the last band limits
+    HIPCHK(c, hipEventRecord(c->evjoin, c->stream2));
+    HIPCHK(c, hipStreamWaitEvent(c->stream, c->evjoin, 0));
+  }
   // dL/dV of every slice at once: band limit (self-adjoint) of dL/dt, then sigma Im(g conj(u))
   if (grad_V && !grad_each) {
     int rc = bandlimit_buf(c, c->gT.as<float2>(), S);

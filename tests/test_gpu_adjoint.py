@@ -341,3 +341,33 @@ def test_atoms_autograd_function():
         xy2 = (xy - step * xy.grad).detach()
         loss2 = atoms_multislice_loss(xy2, ms, atoms, ipos, target, atomRadius=R)
         assert float(loss2) < float(loss.detach())
+
+
+@pytest.mark.parametrize("n,bandlimit", [(2048, 0), (2048, 1), (1024, 0), (512, 1)])
+def test_adjoint_repeats_agree(n, bandlimit):
+    """Repeats of forward_save + adjoint_saved on one whole-slice window give the same
+    gradients: with one probe every dL/dV element has one writer and no atomics, so dL/dV
+    repeats bit for bit; dL/d(df0, Cs) end in an FP64 reduction whose atomics may order
+    differently.  (Wide stores whose data registers were overwritten before the store read
+    them made dL/dV differ between repeats by up to 1.5e-2 at 2048^2.)"""
+    from qstem_amd import Multislice, StemConfig
+    dx = 0.025
+    cfg = StemConfig(nx=n, ny=n, resolutionX=dx, resolutionY=dx, v0=200.0, slices=2, sliceThickness=2.0,
+                     potNx=n, potNy=n, bandlimittrans=bandlimit, Cs=1.0e4, df0=-40.0, alpha=25.0,
+                     electronScale=1.0)
+    V = smooth_potential(cfg.slices, n, n, seed=n + 1, vmax=20.0)
+    target = np.random.default_rng(n).random((1, n, n)).astype(np.float32) * 1e-9
+    ipos = np.zeros((1, 2), np.int32)
+    runs = []
+    with Multislice(cfg) as ms:
+        ms.set_potential(V)
+        ms.make_probe()
+        for _ in range(6):
+            ms.forward_save(ipos, target)
+            runs.append(ms.adjoint_saved(want_grad_V=True, want_grad_probe=True))
+    g0 = runs[0]
+    assert np.all(np.isfinite(g0["grad_V"])) and np.linalg.norm(g0["grad_V"]) > 0
+    for r, g in enumerate(runs[1:], 1):
+        assert np.array_equal(g["grad_V"], g0["grad_V"]), (r, float(np.abs(g["grad_V"] - g0["grad_V"]).max()))
+        for k in ("grad_df0", "grad_Cs"):
+            assert abs(g[k] - g0[k]) <= 1e-12 * abs(g0[k]) + 1e-30, (r, k, g[k], g0[k])
