@@ -141,6 +141,27 @@ __global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>:
 #pragma unroll
     for (int m = 0; m < E; m++) boff[m] = bo(m);
   }
+  // MODE 1 at 512: the state row of the next step is copied straight into this
+  // wavefront's LDS staging area (buffer loads with an LDS destination, no registers)
+  // while the current step is transformed; only the first step waits for its state.
+  // A band row has at most 384 columns at 512 (2/3 of the axis, whole 16-column
+  // tiles), three 1 KB copies.
+  constexpr bool DMA = MODE == 1 && NY == 512;
+  constexpr int kStg = 384;
+  __shared__ float2 stg[DMA ? (BLK / 64) * kStg : 1];
+  float2* const wstg = stg + (DMA ? wave_uniform(threadIdx.x >> 6) * kStg : 0);
+  auto prefetch = [&](int i) {
+    if constexpr (DMA) {
+      // the whole offset in the VGPR offset, so that the resource's range check covers
+      // it: the copy's tail past the last row reads zeros, and past the chunk's last
+      // step (kOut) nothing is read
+      const unsigned vo = (i < a.rw ? (unsigned)(i * TPW) * rowb : kOut) + 16u * lane;
+#pragma unroll
+      for (int j = 0; j < 3; j++)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rsi, (__attribute__((address_space(3))) void*)(wstg + 128 * j), 16,
+                                                 vo + 1024u * j, 0, 0, 0);
+    }
+  };
   int2 ip = make_int2(0, 0);
   const float2* trc = a.trans;  // this probe's configuration, slice s
   if constexpr (MODE <= 1) {
@@ -160,6 +181,17 @@ __global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>:
       const float2* src = a.probe + ((MODE == 3 ? (size_t)pg * a.nx : (size_t)0) + x0 + xr) * NY;
 #pragma unroll
       for (int m = 0; m < E; m++) dv[m] = src[t + T * m];
+    } else if constexpr (DMA) {
+#pragma unroll
+      for (int m = 0; m < E; m++) {
+        if ((GAP >> m) & 1) {
+          dv[m] = make_float2(0.f, 0.f);
+          continue;
+        }
+        const unsigned o = BOFF ? boff[m] : bo(m);
+        const float2 w = wstg[min(o >> 3, (unsigned)kStg - 1u)];  // index kept inside the area
+        dv[m] = o == kOut ? make_float2(0.f, 0.f) : w;
+      }
     } else {
       const unsigned so = (unsigned)(i * TPW) * rowb;
 #pragma unroll
@@ -204,10 +236,17 @@ __global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>:
     }
   };
   // one register set: at 4 wavefronts per SIMD the other wavefronts cover the
-  // load latency (measured equal to a two-set software pipeline at 3 per SIMD)
+  // load latency (measured equal to a two-set software pipeline at 3 per SIMD); at
+  // 512 the state rows come through the LDS copies (the transmission rows of the next
+  // step in registers as well measured slower: 121 VGPRs)
   float2 va[E], ta[E];
+  prefetch(0);
   for (int i = 0; i < a.rw; i++) {
     load(i, va, ta);
+    if constexpr (DMA) {
+      __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): this wavefront's staging reads are done
+      prefetch(i + 1);  // unconditional: one path into the step, so its waits count these copies
+    }
     step(i, va, ta);
   }
 }

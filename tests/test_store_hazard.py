@@ -88,8 +88,11 @@ def test_wide_buffer_stores_use_a_zero_soffset():
     assert len(calls) == 1, calls  # the only one is buf_store_f4's
 
 
-@pytest.mark.skipif(not os.path.exists(HIPCC), reason="hipcc not installed")
-def test_sweep_kernels_have_no_store_data_hazard():
+@pytest.fixture(scope="module")
+def assembly():
+    """name -> gfx950 assembly of the per-size sweep kernels and two compiled plans."""
+    if not os.path.exists(HIPCC):
+        pytest.skip("hipcc not installed")
     tmp = tempfile.mkdtemp(prefix="qs_isa_")
     try:
         jobs = [("sizes%d" % n, [HIPCC] + FLAGS + ["-DQS_N=%d" % n, os.path.join(CSRC, "qs_sizes.hip")])
@@ -101,10 +104,25 @@ def test_sweep_kernels_have_no_store_data_hazard():
             name, cmd = job
             out = os.path.join(tmp, name + ".s")
             subprocess.run(cmd + ["-o", out], check=True, capture_output=True)
-            return name, hazards(open(out).read())
+            return name, open(out).read()
 
         with ThreadPoolExecutor(max_workers=min(len(jobs), os.cpu_count() or 1)) as ex:
-            found = {name: h for name, h in ex.map(run, jobs) if h}
-        assert not found, found
+            return dict(ex.map(run, jobs))
     finally:
         shutil.rmtree(tmp, ignore_errors=True)
+
+
+def test_sweep_kernels_have_no_store_data_hazard(assembly):
+    found = {name: h for name, h in ((n, hazards(a)) for n, a in assembly.items()) if h}
+    assert not found, found
+
+
+def test_lds_copies_are_range_checked(assembly):
+    """The 512^2 row sweep copies state rows into LDS with buffer loads; each carries its
+    whole offset in the VGPR offset (soffset 0), which the resource's range check covers,
+    so a copy's tail past the last row reads zeros instead of memory past the buffer."""
+    copies = [line.strip() for a in assembly.values() for line in a.split("\n")
+              if re.match(r"\s*buffer_load_\w+ .* lds$", line)]
+    assert any("buffer_load_dwordx4" in c for c in copies), "no LDS copies in the 512^2 row sweep"
+    bad = [c for c in copies if not re.match(r"buffer_load_\w+ v\d+, s\[\d+:\d+\], 0 offen lds$", c)]
+    assert not bad, bad
