@@ -267,10 +267,11 @@ int qs_run_groups(const qs_group *groups, int ngroup, const int *bin_of_slice, i
  * Outputs (nullable): loss; grad_V [nslice][potNx][potNy] = dL/dV (needs the potential
  * set by qs_set_potential or qs_build_potential); grad_probe[2] = dL/d(df0), dL/d(Cs) in
  * the units of qs_params (Angstrom).  Device memory: ceil(nslice/K) + K wave states
- * (K = checkpoint interval), dL/dV of all slices, and dL/dt of one slice (two, with
- * two band-limit scratch buffers, when the band limit of one slice runs on the second
- * stream of qs_set_streams beside the reverse sweeps of the next; all slices when the
- * transmission grid is band-limited but not a power of two).  No
+ * (K = checkpoint interval), dL/dV of all slices, and dL/dt of one slice (all slices
+ * when the transmission grid is band-limited but not a power of two; where one window
+ * covers a band-limited power-of-two slice, instead two band-compact buffers of its
+ * row transforms, the band limit of one slice running on the second stream of
+ * qs_set_streams beside the reverse sweeps of the next).  No
  * reference counterpart (QSTEM has no adjoint). */
 int qs_adjoint(qs_ctx *ctx, const int *ipos, int nprobe, const float *target, int ckpt_every, float *loss,
                float *grad_V, float *grad_probe);

@@ -2162,7 +2162,6 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
   const int gslices = grad_V && !grad_each ? S : 1;
   const float mm0 = 1.0F + p.v0 / 511.0F;
   const double sigma = mm0 * wavelength(p.v0);
-  HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * gslices));
   // band-limited dL/dV slice by slice after the slice's reverse sweeps (its first pass
   // fused into the reverse row sweep where the window is the whole slice; then the rest
   // runs on the second stream beside the reverse sweeps of slice s-1, from two
@@ -2176,6 +2175,7 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
   // limit's first pass (row FFT into the band-compact state); its buffers are allocated
   // before the loop (the row sweep writes them)
   const bool fuse = grad_each && p.bandlimittrans && c->bl.on && whole;
+  if (!fuse) HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * gslices));  // fused: dL/dt stays band-compact
   const bool overlap = fuse && c->nstreams > 1 && S > 1;
   bool pending[2] = {false, false};  // the band limit on the second stream still reads buffer b
   if (fuse) {
@@ -2213,7 +2213,7 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
       const int b = overlap ? (s & 1) : 0;
       DevBuf& bls = b ? c->blstate2 : c->blstate;
       A.f.trans = trans_base(c) + (size_t)s * slice_elems;
-      A.gT = c->gT.as<float>() + (gslices > 1 ? 2 * (size_t)s * slice_elems : 0);
+      A.gT = fuse ? nullptr : c->gT.as<float>() + (gslices > 1 ? 2 * (size_t)s * slice_elems : 0);
       A.blstate = fuse ? bls.as<float2>() : nullptr;
       A.xprev = nullptr;
       if (s < S - 1) qs::launch_adj_cols(A, false, c->stream);

@@ -812,7 +812,7 @@ __global__ __launch_bounds__(row_block<NY>()) void k_adj_rows(AdjArgs A) {
   }
   const int2 ip = a.ipos[pg];
   const float2* tr = a.trans + (size_t)(x + ip.x) * a.potNy + ip.y;
-  float* gt = A.gT + 2 * ((size_t)(x + ip.x) * a.potNy + ip.y);
+  float* gt = BLF ? nullptr : A.gT + 2 * ((size_t)(x + ip.x) * a.potNy + ip.y);  // BLF: no dL/dt buffer
 #pragma unroll
   for (int m = 0; m < E; m++) {
     const int iy = t + T * m;
