@@ -86,6 +86,19 @@ __device__ __forceinline__ void sincos_phase(float x, float& s, float& c) {
   c = ((q + 1) & 2) ? -c0 : c0;
 }
 
+// Columns per tile of the column sweeps at length N: at 512 one whole 128-byte line of
+// each state row (16 columns).
+template <int N> constexpr int col_tc() { return N == 512 ? 16 : (N <= 1024 ? 8 : (N == 2048 ? 4 : 2)); }
+
+// Largest band-compact row pitch (complex values) at length N: the 2/3 band keeps at
+// most 2 floor(N/3) + 1 columns, padded to whole column tiles and then to 16.
+template <int N>
+constexpr int band_pitch_max() {
+  constexpr int tc = col_tc<N>();
+  constexpr int kc = 2 * (N / 3) + 1;
+  return ((kc + tc - 1) / tc * tc + 15) / 16 * 16;
+}
+
 // Sweep A: rows.
 // mode 0: input is the shared real-space probe (first slice of a fresh run)
 // mode 1: input is the band-compact state (a.state_in if set, else a.state); row IFFT first
@@ -141,13 +154,14 @@ __global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>:
 #pragma unroll
     for (int m = 0; m < E; m++) boff[m] = bo(m);
   }
-  // MODE 1 at 512: the state row of the next step is copied straight into this
+  // MODE 1 up to 512: the state rows of the next step are copied straight into this
   // wavefront's LDS staging area (buffer loads with an LDS destination, no registers)
   // while the current step is transformed; only the first step waits for its state.
-  // A band row has at most 384 columns at 512 (2/3 of the axis, whole 16-column
-  // tiles), three 1 KB copies.
-  constexpr bool DMA = MODE == 1 && NY == 512;
+  // A step's TPW band rows hold at most 384 complex values (three 1 KB copies): the
+  // band keeps at most 2 floor(NY/3) + 1 columns, padded to whole tiles and 16.
+  constexpr bool DMA = MODE == 1 && NY <= 512;
   constexpr int kStg = 384;
+  static_assert(!DMA || TPW * band_pitch_max<NY>() <= kStg, "a step's band rows exceed the staging area");
   __shared__ float2 stg[DMA ? (BLK / 64) * kStg : 1];
   float2* const wstg = stg + (DMA ? wave_uniform(threadIdx.x >> 6) * kStg : 0);
   auto prefetch = [&](int i) {
@@ -257,7 +271,7 @@ template <int NX> struct ColCfg {
   // TC columns per tile; BLK threads per block.  At 512 a tile is 16 columns =
   // one whole 128-byte line of each state row, staged by a 512-thread block, so
   // no line is fetched by two blocks.
-  static constexpr int TC = NX == 512 ? 16 : (NX <= 1024 ? 8 : (NX == 2048 ? 4 : 2));
+  static constexpr int TC = col_tc<NX>();
   static constexpr int BLK = NX == 512 ? 512 : row_block<NX>();
   // tiles grouped on one XCD: those sharing a 128-byte line (16 columns); pairs
   // of whole-line tiles at 512 (measured faster than no grouping)
