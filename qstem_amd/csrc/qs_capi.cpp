@@ -2162,20 +2162,18 @@ static int adj_backward(qs_ctx* c, const char* who, float* grad_V, float* grad_p
   const int gslices = grad_V && !grad_each ? S : 1;
   const float mm0 = 1.0F + p.v0 / 511.0F;
   const double sigma = mm0 * wavelength(p.v0);
-  // band-limited dL/dV slice by slice after the slice's reverse sweeps (its first pass
-  // fused into the reverse row sweep where the window is the whole slice; then the rest
-  // runs on the second stream beside the reverse sweeps of slice s-1, from two
-  // band-compact buffers used in turn)
   if (grad_V) HIPCHK(c, c->gV.ensure(sizeof(float) * slice_elems * S));
   HIPCHK(c, c->gprobe.ensure(sizeof(float2) * (size_t)p.nx * p.ny));
   qs::AdjArgs A = adj_args(c, nprobe);
   const bool whole = A.single_writer == 2;
-  // one window covers the slice and the band limit runs as band-compact sweeps: each
-  // row of dL/dt is complete in the reverse row sweep, which then also does the band
-  // limit's first pass (row FFT into the band-compact state); its buffers are allocated
-  // before the loop (the row sweep writes them)
+  // fuse: one window covers the slice and the band limit runs as band-compact sweeps.
+  // Each row of dL/dt is then complete in the reverse row sweep, which also does the
+  // band limit's first pass (row FFT into the band-compact state), so dL/dt is never
+  // stored at full size; the band limit's other two sweeps of slice s run on the second
+  // stream beside the reverse sweeps of slice s-1 (overlap), from two band-compact
+  // buffers used in turn.  Its buffers are allocated before the loop.
   const bool fuse = grad_each && p.bandlimittrans && c->bl.on && whole;
-  if (!fuse) HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * gslices));  // fused: dL/dt stays band-compact
+  if (!fuse) HIPCHK(c, c->gT.ensure(sizeof(float2) * slice_elems * gslices));
   const bool overlap = fuse && c->nstreams > 1 && S > 1;
   bool pending[2] = {false, false};  // the band limit on the second stream still reads buffer b
   if (fuse) {
