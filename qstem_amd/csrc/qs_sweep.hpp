@@ -263,6 +263,8 @@ __global__ __launch_bounds__(row_block<NY>(), Plan<NY>::E <= 8 && NY / Plan<NY>:
     }
     step(i, va, ta);
   }
+  // vmcnt(0): the last (empty) copy has landed before the workgroup's LDS can be released
+  if constexpr (DMA) __builtin_amdgcn_s_waitcnt(0x0F70);
 }
 
 // ============================================================================
