@@ -82,3 +82,23 @@ def test_multislice_many_slices(oracle, n, slices):
         for d in range(det.shape[1]):
             if np.any(det[b, d] != 0):
                 assert rel_l2(out["det"][b, d], det[b, d]) < TOL, (b, d)
+
+
+@pytest.mark.parametrize("n,nprobe,pad,slices", [(512, 32, 64, 6), (1024, 4, 0, 4), (2048, 1, 0, 4), (4096, 1, 0, 3)])
+def test_exit_waves_repeat_bit_for_bit(n, nprobe, pad, slices):
+    """Repeated runs give identical exit waves: the slice sweeps have no atomics or
+    order-dependent sums, so any difference is a race.  (Wide stores whose data registers
+    were overwritten before the store had read them made results differ between runs.)
+    Two probe groups on the two HIP streams at 512^2 and 1024^2, one probe otherwise."""
+    cfg = small_config(nx=n, ny=n, pad=pad, slices=slices, dx=0.03)
+    V = smooth_potential(cfg.slices, cfg.potNx, cfg.potNy, seed=n)
+    ipos = probe_positions(cfg, nprobe, seed=3) if pad else np.zeros((nprobe, 2), np.int32)
+    with _ctx(cfg) as ms:
+        ms.set_potential(V)
+        ms.make_probe()
+        if nprobe > 1:
+            ms.set_batch(nprobe // 2)  # two probe groups, one per stream
+        runs = [ms.run_stem(ipos, want_exit=True)["exit"] for _ in range(4)]
+    assert np.all(np.isfinite(runs[0])) and np.abs(runs[0]).max() > 0
+    for r, ex in enumerate(runs[1:], 1):
+        assert np.array_equal(ex, runs[0]), (r, float(np.abs(ex - runs[0]).max()))
